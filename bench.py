@@ -1,0 +1,276 @@
+"""NNP molecular-liquid MD benchmark on one MI355X.
+
+Workload (24k-atom water NNP step): 8000 flexible 3-site water molecules (24000 atoms) in a
+62.1 A box; tabulated LJ + shifted-force Coulomb to 9 A (0.01 A table); symmetry-function NNP to
+6 A (16 radial + 16 angular per element pair, D = 80) with a per-element 3 x 128 tanh MLP;
+harmonic bonds/angles; full lists to 11 A (2 A skin) rebuilt every 10 steps; O-O/O-H/H-H g(r)
+on 0-12 A in 0.02 A bins and S(q) on 1000 q in 0.3-20 1/A every 10 steps; FP64 velocity Verlet
+at dt = 0.5 fs.  A step is one MD step of this workload; steps replay captured hipGraphs.
+
+value = atom-steps/s of the BF16-MFMA form (whole job, one GPU, inputs resident in HBM); the
+FP32 form runs the same steps with FP32 network arithmetic.  Force errors are relative RMS
+against the FP64 oracle on the starting snapshot.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NMOL, BL = 8000, 62.1
+RC_PAIR, SKIN, RC_NNP, R_GR = 9.0, 2.0, 6.0, 12.0
+WIDTHS = [80, 128, 128, 128, 1]
+DT = 0.5
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # TFLOP/s dense
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_engine(sysm, par, gmax, prec):
+    from nnpengine_amd import model as M
+    from nnpengine_amd import system as S
+    from nnpengine_amd.engine import Engine
+    ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
+    sfp = M.sf_params(RC_NNP)
+    e = Engine(sysm)
+    e.neigh_init(RC_PAIR, SKIN, R_GR, RC_NNP)
+    e.pair_table(RC_PAIR, ft, ut)
+    e.bonded_set_from_system()
+    e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    e.mlp_set(WIDTHS, par, 0.5, prec, gmax)
+    e.gr_init(450, R_GR, RC_PAIR)
+    e.sq_init(1000, 0.3, 20.0)
+    return e
+
+
+MD_KW = dict(dt=DT, fp64=True, rebuild_every=10, pair=True, bonded=True, nnp=2, gr_every=10, gr_long_every=10,
+             sq=True, use_graph=True)
+
+
+def time_steps(e, steps, warmup):
+    e.md_run(warmup, log_every=0, **MD_KW)
+    e.sync()
+    t0 = time.perf_counter()
+    e.md_run(steps, log_every=0, **MD_KW)
+    e.sync()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(sysm, par):
+    """The oracle's float32 restatement of the same step on the host (one thread), on a bounded
+    sample: 2 steps of the 24k-atom workload."""
+    from nnpengine_amd import model as M
+    from nnpengine_amd import system as S
+    from oracle import pyoracle
+    ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
+    sfp = M.sf_params(RC_NNP)
+    o = pyoracle.Oracle(sysm, 0)
+    o.neigh_init(RC_PAIR, SKIN, R_GR)
+    o.pair_set(RC_PAIR, ft, ut)
+    o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    o.mlp_set(WIDTHS, par, 0.5)
+    o.gr_init(450, R_GR, RC_PAIR)
+    o.set_dt(DT)
+    nsteps = 2
+    t_reb = t_gr = t_step = 0.0
+    for s in range(nsteps):
+        t0 = time.perf_counter()
+        if s > 0:
+            o.update_x()
+        o.rebuild(True)
+        t1 = time.perf_counter()
+        o.gr_tally(True)
+        o.sq(1000, 0.3, 20.0)
+        t2 = time.perf_counter()
+        o.clear_f()
+        o.bonded_compute(False)
+        o.pair_compute(False)
+        G = o.sf_descriptor()
+        _, _, dG = o.mlp_eval(G)
+        o.sf_force(dG, False)
+        o.update_v()
+        t3 = time.perf_counter()
+        t_reb += t1 - t0
+        t_gr += t2 - t1
+        t_step += t3 - t2
+    # the workload rebuilds lists and tallies g(r)/S(q) every 10th step
+    per_step = (t_step + (t_reb + t_gr) / 10) / nsteps
+    total = t_reb + t_gr + t_step
+    return {"value": sysm.natom / per_step, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nsteps} steps of the 24000-atom workload ({total:.1f} s): forces (pair, bonded, NNP "
+                      f"descriptors, FP64 MLP, NNP forces) + Verlet every step, list rebuild and 12 A g(r) + S(q) "
+                      f"charged at 1/10 per step; oracle float32 restatement, 1 thread"}
+
+
+def force_errors(sysm, par, gmax, e32, e16):
+    """NNP and total force relative RMS vs the FP64 oracle on the starting snapshot."""
+    from nnpengine_amd import model as M
+    from nnpengine_amd import system as S
+    from oracle import pyoracle
+    from nnpengine_amd.engine import Engine
+    ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
+    sfp = M.sf_params(RC_NNP)
+    o = pyoracle.Oracle(sysm, 1)
+    o.neigh_init(RC_PAIR, SKIN, R_GR)
+    o.rebuild(True)
+    o.pair_set(RC_PAIR, ft, ut)
+    o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    o.mlp_set(WIDTHS, par, 0.5)
+    o.clear_f()
+    G = o.sf_descriptor()
+    _, _, dG = o.mlp_eval(G)
+    o.sf_force(dG)
+    f_nnp64 = o.get("f")
+    o.bonded_compute()
+    o.pair_compute()
+    f_tot64 = o.get("f")
+
+    def rr(a, b):
+        return float(np.sqrt(((a - b) ** 2).mean() / (b ** 2).mean()))
+
+    out = {}
+    for name in ("fp32", "bf16"):
+        e = make_engine(sysm, par, gmax, name)
+        e.build()
+        e.sf_descriptor()
+        e.clear_force()
+        e.mlp_eval(False)
+        e.sf_force(False)
+        f_nnp = e.get("f")
+        e.pair_compute(False)
+        e.bonded_compute(False)
+        e.sf_descriptor()
+        e.mlp_eval(False)
+        e.sf_force(False)
+        f_tot = e.get("f")
+        out[name] = {"nnp_force_rel_rms": rr(f_nnp, f_nnp64), "total_force_rel_rms": rr(f_tot, f_tot64)}
+        e.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-err", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is "
+              "described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    from nnpengine_amd import model as M
+    from nnpengine_amd import system as S
+    from oracle import pyoracle
+
+    sysm = S.water_box(NMOL, BL, seed=7)
+    # network: seeded weights, descriptor standardisation from the starting snapshot (FP64 oracle)
+    sfp = M.sf_params(RC_NNP)
+    o = pyoracle.Oracle(sysm, 1)
+    o.neigh_init(RC_PAIR, SKIN, R_GR)
+    o.rebuild(True)
+    o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    G0 = o.sf_descriptor()
+    mean, std = M.standardisation(G0, sysm.typ, sysm.ntype)
+    par = M.fold_standardisation(M.mlp_init(WIDTHS, sysm.ntype, 2024), WIDTHS, sysm.ntype, mean, std)
+    gmax = float(np.abs(G0).max())
+    del o
+    log(f"[bench] system {sysm.natom} atoms, descriptor max {gmax:.3g}")
+
+    res = {}
+    engines = {}
+    for prec in ("bf16", "fp32"):
+        e = make_engine(sysm, par, gmax, prec)
+        t = time_steps(e, args.steps, args.warmup)
+        res[prec] = t
+        engines[prec] = e
+        st = e.md_stats()
+        log(f"[bench] {prec}: {args.steps} steps in {t:.3f} s = {t / args.steps * 1e3:.4f} ms/step; stats {st}")
+
+    # per-step latency distribution of the BF16 form (events around single graph replays)
+    eb = engines["bf16"]
+    lat = []
+    for _ in range(200):
+        t0 = time.perf_counter()
+        eb.md_run(1, log_every=0, **MD_KW)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat)
+
+    # per-stage profile (HIP events on the engine stream) of the BF16 form
+    stages = {}
+    for st in ("pair", "bonded", "sf_desc", "mlp", "sf_adjoint", "sf_force", "rebuild", "gr_full", "verlet", "sq"):
+        stages[st] = eb.profile(st, reps=20)
+    ms32_mlp = engines["fp32"].profile("mlp", reps=20)
+    log("[bench] stage ms:", {k: round(v, 4) for k, v in stages.items()}, "fp32 mlp", round(ms32_mlp, 4))
+
+    # roofline of the dominant kernel
+    n_long, n_ghost = eb.list_size(0)
+    n_short, _ = eb.list_size(1)
+    N = sysm.natom
+    flop_atom = 4 * sum(WIDTHS[i] * WIDTHS[i + 1] for i in range(len(WIDTHS) - 1))
+    dom = max(("pair", "sf_desc", "mlp", "sf_adjoint", "sf_force", "gr_full"), key=lambda k: stages[k])
+    nmt = sysm.ntype * 4 * 11 + sysm.ntype * 16
+    if dom == "mlp":
+        roof = {"kernel": "k_mlp<bf16>", "bound": "mfma", "achieved": flop_atom * N / (stages["mlp"] * 1e-3) / 1e12,
+                "peak": MFMA_PEAK["bf16"], "unit": "TFLOP/s"}
+    else:
+        # algorithmic bytes per launch: list entries (4 B) + per-atom rows read/written once
+        bytes_per = {"pair": 4 * n_long + N * (16 + 4 + 4 + 16),
+                     "sf_desc": 4 * n_short + N * (16 + 4 + 4 + 4 * nmt + 4 * 80),
+                     "sf_adjoint": N * 4 * (nmt + 80 + nmt),
+                     "sf_force": 4 * n_short + N * (16 + 4 + 4 + 4 * nmt + 16),
+                     "gr_full": N * (16 + 4 + 4)}[dom]
+        roof = {"kernel": dom, "bound": "hbm", "achieved": bytes_per / (stages[dom] * 1e-3) / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    mlp_roof = {"achieved_tflops_bf16": flop_atom * N / (stages["mlp"] * 1e-3) / 1e12,
+                "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
+                "flop_per_atom": flop_atom}
+
+    errs = None if args.no_err else force_errors(sysm, par, gmax, None, None)
+    cpu = None if args.no_cpu else cpu_baseline(sysm, par)
+
+    t_b, t_f = res["bf16"], res["fp32"]
+    value = N * args.steps / t_b
+    out = {
+        "metric": "NNP-MD atom-steps/s @24k atoms, BF16-MFMA vs FP32 at stated err; p99 latency",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_b / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded water lattice, Maxwell-Boltzmann 300 K, random-init NNP weights)",
+        "config": {"workload": "24000-atom water NNP-MD step (8000 molecules, L=62.1 A, LJ+Coulomb table 9 A, "
+                               "symmetry-function NNP 6 A D=80, MLP 3x128 tanh, lists 11 A rebuilt every 10 steps, "
+                               "g(r) 0-12 A + S(q) every 10 steps, FP64 Verlet dt=0.5 fs)",
+                   "natom": N, "dt_fs": DT},
+        "ns_per_day": args.steps / t_b * DT * 0.0864 / 1.0,
+        "fp32_form": {"value": N * args.steps / t_f, "ms_per_step": t_f / args.steps * 1e3},
+        "bf16_speedup_vs_fp32": t_f / t_b,
+        "force_error_vs_fp64": errs,
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "stage_ms": stages,
+        "mlp": mlp_roof,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

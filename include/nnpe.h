@@ -1,0 +1,193 @@
+/* nnpe.h -- C ABI of the MI355X (gfx950) engine for NNPEngine's molecular-liquid MD hot path.
+ *
+ * Plain C: an opaque context, host pointers + sizes, int status codes (0 = NNPE_OK).  All
+ * device state (positions, lists, tables, MLP weights) lives in the context's HBM buffers; the
+ * host passes and receives per-atom arrays in atom-id order (the reference's input order).
+ * Every call is ordered on the context's HIP stream; getters synchronise it.
+ *
+ * Each entry point names the reference interface it replaces (paths in the reference repo).
+ */
+#ifndef NNPE_H
+#define NNPE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nnpe_ctx nnpe_ctx;
+
+enum nnpe_status {
+  NNPE_OK = 0,
+  NNPE_ERR_ARG = 1,             /* bad argument / shape */
+  NNPE_ERR_HIP = 2,             /* HIP runtime error (message in nnpe_last_error) */
+  NNPE_ERR_STATE = 3,           /* call out of order (e.g. compute before a list build) */
+  NNPE_ERR_NEIGH_OVERFLOW = 4,  /* per-atom list capacity exceeded ("nei_list overflow", src/neighlist.cpp:327) */
+  NNPE_ERR_GHOST_OVERFLOW = 5,  /* ghost capacity exceeded ("ghost overflow", src/environ.cpp:440) */
+  NNPE_ERR_CELL_RANGE = 6,      /* atom outside the cell grid (the CHECK_BINNING case, src/environ.cpp:245-259) */
+  NNPE_ERR_NO_DEVICE = 7,       /* no HIP device */
+  NNPE_ERR_UNSUPPORTED = 8      /* configuration this build does not support */
+};
+
+/* per-atom state selectors for nnpe_state_get / nnpe_state_set */
+enum nnpe_state_kind { NNPE_X = 0, NNPE_V = 1, NNPE_F = 2, NNPE_FPREV = 3 };
+
+/* energy/virial terms accumulated by the compute calls (nnpe_energy_get) */
+enum nnpe_term { NNPE_TERM_PAIR = 0, NNPE_TERM_BOND = 1, NNPE_TERM_ANGLE = 2, NNPE_TERM_DIHEDRAL = 3,
+                 NNPE_TERM_NNP_PAIR = 4, NNPE_TERM_NN = 5, NNPE_TERM_SF = 6, NNPE_TERM_KE = 7,
+                 NNPE_NTERM = 8 };
+
+const char* nnpe_status_string(int status);
+const char* nnpe_last_error(const nnpe_ctx* ctx);
+
+/* Lifecycle.  One context per device and system. */
+int nnpe_create(nnpe_ctx** out, int device);
+int nnpe_destroy(nnpe_ctx* ctx);
+int nnpe_synchronize(nnpe_ctx* ctx);
+
+/* ENVIRON::initialize (src/environ.cpp:22-141) after the host has parsed the configuration and
+ * molecule files (MOLECULE::from_file, src/molecule.cpp:25-131).
+ * x, v: [natom][3]; q, mass, typ, mol_id, intra_id: [natom]; mol_type: [nmol];
+ * mt_special: concatenated [mt_natom[t]][mt_natom[t]] special classes at mt_special_off[t].
+ * v may be NULL (zero velocities). */
+int nnpe_system_set(nnpe_ctx* ctx, int natom, int ntype, float bl,
+                    const float* x, const float* v, const float* q, const float* mass,
+                    const int* typ, const int* mol_id, const int* intra_id,
+                    int nmol, const int* mol_type,
+                    int nmoltype, const int* mt_natom, const int* mt_special_off, const int* mt_special);
+
+/* Per-atom state in atom-id order as double [natom][3] (what = nnpe_state_kind).
+ * Replaces the host reads/writes of ENVIRON::x / v / f / f_prev (src/environ.h:31-32). */
+int nnpe_state_get(nnpe_ctx* ctx, int what, double* out);
+int nnpe_state_set(nnpe_ctx* ctx, int what, const double* in);
+
+/* ENVIRON::init_grid + NEIGHLIST::build_bin_list + LOCAL::set_ghost_lim
+ * (src/environ.cpp:143-155, src/neighlist.cpp:35-168, src/run.cpp:72-82).
+ * comm cutoff = skin + max(gr_cutoff, pair_cutoff); cell size = (pair_cutoff + skin) / 2.
+ * short_cutoff > 0 also builds a second list to short_cutoff + skin (NNP environments). */
+int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, float gr_cutoff, float short_cutoff);
+/* Rebuild: LOCAL::refold_and_tag, ENVIRON::sort, generate_ghost, tidyup_ghost,
+ * update_ghost_pos<true>, sort_ghost, update_ghost_and_info, NEIGHLIST::build<true>
+ * (src/run.cpp:249-276).  Lists are full (each atom lists every neighbour and image). */
+int nnpe_neigh_build(nnpe_ctx* ctx);
+/* NEIGHLIST::isvalid (src/neighlist.cpp:178-187): *valid = 1 if no atom moved more than skin/2. */
+int nnpe_neigh_isvalid(nnpe_ctx* ctx, int* valid);
+/* Number of directed entries (which = 0 pair list, 1 short list) and the ghost count. */
+int nnpe_neigh_size(nnpe_ctx* ctx, int which, long long* n_entries, int* n_ghost);
+/* Export directed entries as atom ids + image code ((sx+1)*9+(sy+1)*3+(sz+1), 13 = same image)
+ * + special class; cap = array capacity. */
+int nnpe_neigh_export(nnpe_ctx* ctx, int which, int* ai, int* aj, int* code, int* special,
+                      long long cap, long long* n_out);
+
+/* LJTableCoulDSF (src/pair.cpp:171-263): tables force[npair][nbin+1] (F/r) and
+ * potential[npair][nbin+1] at r = (i+1/2) dr, last entry 0. */
+int nnpe_pair_table(nnpe_ctx* ctx, float cutoff, int nbin, const float* force, const float* potential);
+/* LJTableCoulDSF::compute<ev,false> (src/pair.cpp:289-419).  Overwrites the force array. */
+int nnpe_pair_compute(nnpe_ctx* ctx, int ev);
+
+/* BOND / ANGLE / DIHEDRAL setParam + autoGenerate (src/bond.cpp:48-103, src/angle.cpp:25-82,
+ * src/dihedral.cpp:46-104).  kind 0 bond (terms [n][3]), 1 angle ([n][4]), 2 dihedral ([n][5]);
+ * terms are (type, atom ids...); coef [ntyp][4] in the classes' stored form. */
+int nnpe_bonded_set(nnpe_ctx* ctx, int kind, int ntyp, const float* coef, int nterm, const int* terms);
+/* BOND/ANGLE/DIHEDRAL::compute<ev> (src/bond.cpp:108-137, src/angle.cpp:87-136,
+ * src/dihedral.cpp:109-166) on unwrapped molecule positions.  Adds to the force array. */
+int nnpe_bonded_compute(nnpe_ctx* ctx, int ev);
+
+/* Energy and virial sums (FP64) of the ev compute calls since the last clear:
+ * out[2*term] = energy, out[2*term+1] = virial, term = nnpe_term. */
+int nnpe_energy_get(nnpe_ctx* ctx, double* out);
+int nnpe_energy_clear(nnpe_ctx* ctx);
+
+/* VERLET (src/verlet.cpp:29-97).  fp64 = 0: the reference's FP32 update; fp64 = 1: FP64
+ * positions/velocities/accelerations with FP32 forces. */
+int nnpe_verlet_set(nnpe_ctx* ctx, double dt, int fp64);
+int nnpe_verlet_update_x(nnpe_ctx* ctx);
+int nnpe_verlet_update_v(nnpe_ctx* ctx);
+int nnpe_verlet_compute_ke(nnpe_ctx* ctx, double* ke);
+int nnpe_verlet_scale_v(nnpe_ctx* ctx, double s);
+/* LOCAL::clear_vector(ENVIRON::f) (src/environ.cpp:526-534). */
+int nnpe_clear_force(nnpe_ctx* ctx);
+
+/* GR (src/grsq.cpp:28-80): nbin_inner bins to r_max_inner, extended to r_max. */
+int nnpe_gr_init(nnpe_ctx* ctx, int nbin_inner, float r_max, float r_max_inner, int* nbin_total);
+/* GR::tally<true>(list) (full = 0, src/grsq.cpp:94-149) or GR::tally(bin_list, local)
+ * (full = 1, src/grsq.cpp:151-205).  Counts are 2 per unordered pair. */
+int nnpe_gr_tally(nnpe_ctx* ctx, int full);
+/* GR::reduce_local (src/grsq.cpp:207-219): counts [npair][nbin] and h(r) = g(r) - 1. */
+int nnpe_gr_get(nnpe_ctx* ctx, long long* counts, double* h);
+/* SQ (src/grsq.cpp:258-322) with q_i = q_min + (i+1/2)(q_max-q_min)/nq, accumulated in FP64. */
+int nnpe_sq_init(nnpe_ctx* ctx, int nq, double q_min, double q_max);
+int nnpe_sq_compute(nnpe_ctx* ctx, double* out);
+
+/* NNPOTENTIAL_TBL ctor (src/pair_nnp.cpp:433-598): tables built from the weight-file arrays
+ * coef_p [npair][4][6] (eps, sigma, c8, c10, c12, qij per special class) and
+ * coef_w / coef_d / coef_r [npair][80] (20 descriptors x 4 special classes). */
+int nnpe_nnp_tbl_set(nnpe_ctx* ctx, float cutoff, int nbin, const float* coef_p, const float* coef_w,
+                     const float* coef_d, const float* coef_r);
+/* NNPOTENTIAL_TBL::computeDescriptor (src/pair_nnp.cpp:625-688): 20 descriptors per atom. */
+int nnpe_nnp_tbl_descriptor(nnpe_ctx* ctx);
+/* NNPOTENTIAL_TBL::compute<ev> (src/pair_nnp.cpp:710-793) with the current dE/dG; adds to the force. */
+int nnpe_nnp_tbl_force(nnpe_ctx* ctx, int ev);
+
+/* Radial + angular symmetry-function descriptors (16 radial centres mu[16] with exponent eta_r
+ * per neighbour element; 4 angular exponents eta[4] x (zeta, lambda) in {1,2} x {+1,-1} per
+ * element pair), cutoff rc with fc = (cos(pi r/rc)+1)/2.  D = 16*ntype + 16*npair. */
+int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, float eta_r, int neta, const float* eta);
+int nnpe_sf_descriptor(nnpe_ctx* ctx);
+/* Symmetry-function forces from the current dE/dG; adds to the force, virial in NNPE_TERM_SF. */
+int nnpe_sf_force(nnpe_ctx* ctx, int ev);
+
+/* TFCALL buffers (src/pair_nnp.cpp:490-496, :697-706) in atom-id order: what 0 = descriptors
+ * (inputs[0]), 1 = dE/dG (outputs[1]), 2 = per-atom network energy; *D = row width. */
+int nnpe_nnp_get(nnpe_ctx* ctx, int what, float* out, int* D);
+/* Provide dE/dG [natom][D] from the host (the reference's external-network path). */
+int nnpe_nnp_set_dedg(nnpe_ctx* ctx, const float* dG, int D);
+
+/* Per-element MLP replacing TFCALL::evaluate (src/pair_nnp.cpp:92-97).  widths[nlayer+1]
+ * (input D, hidden widths multiples of 32, output 1); params per element, per layer:
+ * W [in][out] row-major then b [out].  E_i = out_scale * network(G_i).
+ * precision 0 = FP32, 1 = BF16, 2 = FP16 (MFMA, FP32 accumulation, per-layer scaling);
+ * in_absmax = largest |G| expected (FP16 input scaling; <= 0 for none). */
+int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const float* params, float out_scale,
+                 int precision, float in_absmax);
+/* nnEval: descriptors -> per-atom energy and dE/dG; energy sum in NNPE_TERM_NN when ev. */
+int nnpe_mlp_eval(nnpe_ctx* ctx, int ev);
+
+/* MD loop of classical / nnp_run (src/run.cpp:230-393, :1127-1353) without the thermostat:
+ * updateX -> rebuild -> g(r)/S(q) -> forces -> updateV -> log. */
+typedef struct nnpe_md_params {
+  double dt;          /* fs (the reference hard-codes 1 fs, src/run.cpp:226) */
+  int fp64;           /* 1: FP64 positions/velocities/accelerations */
+  int rebuild_every;  /* > 0: rebuild every n steps, steps replayed from captured hipGraphs when
+                         use_graph; 0: NEIGHLIST::isvalid decides each step (host read-back) */
+  int pair;           /* tabulated LJ + shifted-force Coulomb */
+  int bonded;         /* bonds / angles / dihedrals */
+  int nnp;            /* 0 none, 1 NNPOTENTIAL_TBL descriptors, 2 symmetry functions; needs nnpe_mlp_set */
+  int gr_every;       /* g(r) list tally every n steps (0 = never) */
+  int gr_long_every;  /* full stencil tally every n steps (multiple of gr_every, 0 = never) */
+  int sq;             /* S(q) after each tally */
+  int log_every;      /* energy/virial record every n steps (0 = never) */
+  int use_graph;      /* replay captured step graphs (fixed-interval rebuilds only) */
+} nnpe_md_params;
+/* Run nsteps; log_out receives [nlog][2*NNPE_NTERM] records (energy, virial per term; the
+ * NNPE_TERM_KE slot holds the kinetic energy) for steps s with s % log_every == 0. */
+int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* params, int nsteps, double* log_out, int log_cap, int* nlog);
+/* out[8]: steps done, list builds, late fixed-interval rebuilds (skin violations), max entries
+ * per atom (pair list, short list), ghosts, list capacities (pair, short). */
+int nnpe_md_stats(nnpe_ctx* ctx, long long* out);
+/* Restart the step counter (the next step evaluates forces on the positions as they are). */
+int nnpe_md_reset_step(nnpe_ctx* ctx);
+
+/* Per-stage timing (no reference counterpart: the reference has no timers).  Average ms of one
+ * launch of `stage` over `reps` back-to-back launches, HIP events on the context stream. */
+enum nnpe_stage { NNPE_STAGE_PAIR = 0, NNPE_STAGE_BONDED = 1, NNPE_STAGE_SF_DESC = 2, NNPE_STAGE_MLP = 3,
+                  NNPE_STAGE_SF_ADJOINT = 4, NNPE_STAGE_SF_FORCE = 5, NNPE_STAGE_REBUILD = 6,
+                  NNPE_STAGE_GR_FULL = 7, NNPE_STAGE_GR_LIST = 8, NNPE_STAGE_VERLET = 9, NNPE_STAGE_SQ = 10,
+                  NNPE_STAGE_TBL_DESC = 11, NNPE_STAGE_TBL_FORCE = 12 };
+int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NNPE_H */

@@ -1,0 +1,287 @@
+// grsq.hip -- partial g(r) histograms and the S(q) transform.
+//
+// GR (src/grsq.cpp:28-219): bin = int(sqrt(rsq) * 1/dr) in FP32 exactly as the reference, so
+// the integer counts are bit-identical.  Each workgroup keeps a private LDS histogram
+// (npair x nbin ints, LDS atomics), flushed once per workgroup with global 64-bit atomics
+// (integers: order-independent, deterministic).
+//   inner tally  (GR::tally<true>, :94-149): every directed entry of the full pair list within
+//                r_max_inner adds 1 (2 per unordered pair, as the reference's +2/+1 rule);
+//                bins >= nbin_inner keep their previous counts, as the reference's do.
+//   full tally   (GR::tally(bin_list), :151-205): cell stencil to r_max; real j > i add 2,
+//                ghost images add 1.
+// SQ (src/grsq.cpp:258-322): S_ab(q) = sum_r 4 pi rho dr r sin(q r)/q h_ab(r) in FP64, one
+// workgroup per q point with a fixed-order reduction.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace {
+
+constexpr int kHistLdsMax = 48 * 1024;
+
+__device__ __forceinline__ int pair_index(int a, int b, int ntype) {
+  int i = min(a, b), j = max(a, b);
+  return i * ntype - (i * (i - 1)) / 2 + (j - i);
+}
+
+__device__ __forceinline__ int gr_bin(float rsq, float rdr) { return int(__fmul_rn(sqrtf(rsq), rdr)); }
+
+__global__ void k_gr_zero(unsigned long long* count, int npair, int nbin, int nzero) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npair * nzero) return;
+  count[(k / nzero) * nbin + (k % nzero)] = 0ull;
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(256) k_gr_inner(int n, int stride, const uint32_t* __restrict__ nbr,
+                                                  const int* __restrict__ cnt, const float4* __restrict__ xq,
+                                                  const int* __restrict__ typ, int ntype, int npair, float bl, float m_bl,
+                                                  float rmaxsq, float rdr, int nbin_inner, int nbin,
+                                                  unsigned long long* __restrict__ count) {
+  extern __shared__ unsigned int s_hist[];
+  if (kLds) {
+    for (int k = threadIdx.x; k < npair * nbin_inner; k += blockDim.x) s_hist[k] = 0u;
+    __syncthreads();
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float4 xi = xq[i];
+    const int ti = typ[i];
+    const int m = cnt[i];
+    for (int k = 0; k < m; ++k) {
+      const uint32_t e = nbr[(size_t)k * stride + i];
+      const int j = NNPE_ENTRY_J(e);
+      const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(e), bl, m_bl);
+      const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+      if (!(rsq < rmaxsq)) continue;
+      const int b = gr_bin(rsq, rdr);
+      if (b >= nbin_inner) continue;
+      const int p = pair_index(ti, typ[j], ntype);
+      if (kLds) atomicAdd(&s_hist[p * nbin_inner + b], 1u);
+      else atomicAdd(&count[(size_t)p * nbin + b], 1ull);
+    }
+  }
+  if (kLds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < npair * nbin_inner; k += blockDim.x) {
+      unsigned int v = s_hist[k];
+      if (v) atomicAdd(&count[(size_t)(k / nbin_inner) * nbin + (k % nbin_inner)], (unsigned long long)v);
+    }
+  }
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(256) k_gr_full(int n, int num_gx, const int4* __restrict__ stencil, int nst,
+                                                 const float4* __restrict__ xq, const int* __restrict__ typ,
+                                                 const int* __restrict__ cell, const int* __restrict__ cstart,
+                                                 const int* __restrict__ gstart, const int* __restrict__ gs_src,
+                                                 const int* __restrict__ gs_code, int ntype, int npair, float bl,
+                                                 float m_bl, float rmaxsq, float rdr, int nbin,
+                                                 unsigned long long* __restrict__ count) {
+  extern __shared__ unsigned int s_hist[];
+  if (kLds) {
+    for (int k = threadIdx.x; k < npair * nbin; k += blockDim.x) s_hist[k] = 0u;
+    __syncthreads();
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float4 xi = xq[i];
+    const int ti = typ[i];
+    const int ci = cell[i];
+    const int cz = ci % num_gx, cy = (ci / num_gx) % num_gx, cx = ci / (num_gx * num_gx);
+    for (int s = 0; s < nst; ++s) {
+      const int4 st = stencil[s];
+      const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
+      if (ax < 0 || ay < 0 || az < 0 || ax >= num_gx || ay >= num_gx || az >= num_gx) continue;
+      const int c = (ax * num_gx + ay) * num_gx + az;
+      for (int j = max(i + 1, cstart[c]), je = cstart[c + 1]; j < je; ++j) {
+        const float4 xj = xq[j];
+        const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+        if (!(rsq < rmaxsq)) continue;
+        const int b = gr_bin(rsq, rdr);
+        if (b >= nbin) continue;
+        const int p = pair_index(ti, typ[j], ntype);
+        if (kLds) atomicAdd(&s_hist[p * nbin + b], 2u);
+        else atomicAdd(&count[(size_t)p * nbin + b], 2ull);
+      }
+      for (int g = gstart[c], ge = gstart[c + 1]; g < ge; ++g) {
+        const int src = gs_src[g];
+        const float4 xj = image_pos(xq[src], gs_code[g], bl, m_bl);
+        const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+        if (!(rsq < rmaxsq)) continue;
+        const int b = gr_bin(rsq, rdr);
+        if (b >= nbin) continue;
+        const int p = pair_index(ti, typ[src], ntype);
+        if (kLds) atomicAdd(&s_hist[p * nbin + b], 1u);
+        else atomicAdd(&count[(size_t)p * nbin + b], 1ull);
+      }
+    }
+  }
+  if (kLds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < npair * nbin; k += blockDim.x) {
+      unsigned int v = s_hist[k];
+      if (v) atomicAdd(&count[k], (unsigned long long)v);
+    }
+  }
+}
+
+__global__ void k_gr_reduce(int total, const unsigned long long* count, const double* norm, double* h) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < total) h[k] = double(count[k]) * norm[k] - 1.0;
+}
+
+// one workgroup per q: S_p(q) = sum_r ftmx[q][r] h_p[r]
+__global__ void __launch_bounds__(256) k_sq(int nq, int nbin, int npair, const double* ftmx, const double* h, double* out) {
+  const int q = blockIdx.x;
+  __shared__ double red[256];
+  for (int p = 0; p < npair; ++p) {
+    double acc = 0;
+    for (int r = threadIdx.x; r < nbin; r += 256) acc += ftmx[(size_t)q * nbin + r] * h[(size_t)p * nbin + r];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)p * nq + q] = red[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+extern "C" int nnpe_gr_init(nnpe_ctx* ctx, int nbi, float r_max, float r_max_inner, int* nbin_total) {
+  if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (!ctx || nbi <= 0) return NNPE_ERR_ARG;
+  if (!ctx->have_grid) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_gr_init before nnpe_neigh_init");
+  // GR ctor (src/grsq.cpp:28-47)
+  if (r_max_inner <= 0) {
+    r_max = r_max <= 0 ? ctx->h_bl : r_max;
+    ctx->gr_nbin_inner = ctx->gr_nbin = nbi;
+    ctx->gr_rmaxsq_inner = ctx->gr_rmaxsq = r_max * r_max;
+    ctx->gr_dr = r_max / nbi; ctx->gr_rdr = nbi / r_max;
+  } else {
+    ctx->gr_nbin_inner = nbi;
+    ctx->gr_dr = r_max_inner / nbi; ctx->gr_rdr = nbi / r_max_inner;
+    ctx->gr_rmaxsq_inner = r_max_inner * r_max_inner;
+    r_max = r_max <= 0 ? ctx->h_bl : r_max;
+    if (r_max < r_max_inner) return ctx_fail(ctx, NNPE_ERR_ARG, "r_max_inner larger than r_max");
+    ctx->gr_nbin = int(r_max * ctx->gr_rdr);
+    const float t = ctx->gr_nbin * ctx->gr_dr;
+    ctx->gr_rmaxsq = t * t;
+  }
+  if (r_max > ctx->gr_cutoff + ctx->skin + 1e-4f && r_max > ctx->pair_cutoff + ctx->skin + 1e-4f)
+    return ctx_fail(ctx, NNPE_ERR_ARG, "g(r) range beyond the ghost shell");
+  const int npair = ctx->npair, nb = ctx->gr_nbin;
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gr_count, size_t(npair) * nb));
+  NNPE_CHECK_HIP(ctx, hipMemset(ctx->gr_count, 0, sizeof(unsigned long long) * npair * nb));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gr_h, size_t(npair) * nb));
+  // normalisation (src/grsq.cpp:56-70) in FP64
+  std::vector<double> norm(size_t(npair) * nb);
+  const double pi4 = 4 * 3.141592653589793;
+  const double rc_dr3 = std::pow(double(ctx->bl) * double(ctx->gr_rdr), 3) / pi4;
+  int ip = 0;
+  for (int i = 0; i < ctx->ntype; ++i)
+    for (int j = i; j < ctx->ntype; ++j, ++ip) {
+      const double ni = ctx->h_typecount[i], nj = ctx->h_typecount[j];
+      const double normg = rc_dr3 / (ni * (i == j ? (nj - 1) : 2 * nj));
+      for (int k = 0; k < nb; ++k) norm[size_t(ip) * nb + k] = normg / (double(k) * (k + 1) + 1.0 / 3);
+    }
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gr_norm, norm.size()));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->gr_norm, norm.data(), norm.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (nbin_total) *nbin_total = nb;
+  return NNPE_OK;
+}
+
+int gr_enqueue(nnpe_ctx* ctx, bool full) {
+  const int N = ctx->natom, npair = ctx->npair, nb = ctx->gr_nbin;
+  hipStream_t s = ctx->stream;
+  if (!full) {
+    const int nz = ctx->gr_nbin_inner;
+    hipLaunchKernelGGL(k_gr_zero, dim3(blocks_for(npair * nz, 256)), dim3(256), 0, s, ctx->gr_count, npair, nb, nz);
+    const size_t lds = size_t(npair) * nz * sizeof(unsigned int);
+    if (lds <= kHistLdsMax)
+      hipLaunchKernelGGL(k_gr_inner<true>, dim3(blocks_for(N, 256)), dim3(256), lds, s, N, ctx->stride, ctx->nbr_long,
+                         ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq_inner,
+                         ctx->gr_rdr, nz, nb, ctx->gr_count);
+    else
+      hipLaunchKernelGGL(k_gr_inner<false>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->stride, ctx->nbr_long,
+                         ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq_inner,
+                         ctx->gr_rdr, nz, nb, ctx->gr_count);
+  } else {
+    hipLaunchKernelGGL(k_gr_zero, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, s, ctx->gr_count, npair, nb, nb);
+    const size_t lds = size_t(npair) * nb * sizeof(unsigned int);
+    if (lds <= kHistLdsMax)
+      hipLaunchKernelGGL(k_gr_full<true>, dim3(blocks_for(N, 256)), dim3(256), lds, s, N, ctx->num_gx, ctx->stencil_gr,
+                         ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
+                         ctx->gs_code, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
+    else
+      hipLaunchKernelGGL(k_gr_full<false>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->num_gx, ctx->stencil_gr,
+                         ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
+                         ctx->gs_code, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
+  }
+  NNPE_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_gr_reduce, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, s, npair * nb, ctx->gr_count, ctx->gr_norm,
+                     ctx->gr_h);
+  NNPE_CHECK_LAUNCH(ctx);
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_gr_tally(nnpe_ctx* ctx, int full) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!ctx->have_list || !ctx->gr_count) return ctx_fail(ctx, NNPE_ERR_STATE, "g(r) tally needs nnpe_gr_init and a list");
+  if (!full && ctx->gr_rmaxsq_inner > ctx->cutsq_list) return ctx_fail(ctx, NNPE_ERR_ARG, "inner g(r) range beyond the list");
+  return gr_enqueue(ctx, full != 0);
+}
+
+extern "C" int nnpe_gr_get(nnpe_ctx* ctx, long long* counts, double* h) {
+  if (!ctx || !ctx->gr_count) return ctx_fail(ctx, NNPE_ERR_STATE, "no g(r)");
+  const size_t n = size_t(ctx->npair) * ctx->gr_nbin;
+  NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (counts) NNPE_CHECK_HIP(ctx, hipMemcpy(counts, ctx->gr_count, n * sizeof(long long), hipMemcpyDeviceToHost));
+  if (h) NNPE_CHECK_HIP(ctx, hipMemcpy(h, ctx->gr_h, n * sizeof(double), hipMemcpyDeviceToHost));
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_sq_init(nnpe_ctx* ctx, int nq, double q_min, double q_max) {
+  if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (!ctx || nq <= 0 || !(q_max > q_min) || q_min < 0) return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_sq_init: bad argument");
+  if (!ctx->gr_count) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_sq_init before nnpe_gr_init");
+  ctx->sq_nq = nq; ctx->sq_qmin = q_min; ctx->sq_qmax = q_max;
+  const int nb = ctx->gr_nbin;
+  std::vector<double> ft(size_t(nq) * nb);
+  const double dq = (q_max - q_min) / nq, dr = double(ctx->gr_dr);
+  const double rho4pidr = 4 * 3.141592653589793 * ctx->natom / std::pow(double(ctx->bl), 3) * dr;
+  for (int i = 0; i < nq; ++i) {
+    const double q = q_min + (i + 0.5) * dq;
+    for (int k = 0; k < nb; ++k) {
+      const double r = (k + 0.5) * dr;
+      ft[size_t(i) * nb + k] = std::sin(q * r) * rho4pidr * r / q;
+    }
+  }
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->sq_ftmx, ft.size()));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->sq_ftmx, ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->sq_out, size_t(ctx->npair) * nq));
+  return NNPE_OK;
+}
+
+int sq_enqueue(nnpe_ctx* ctx) {
+  hipLaunchKernelGGL(k_sq, dim3(ctx->sq_nq), dim3(256), 0, ctx->stream, ctx->sq_nq, ctx->gr_nbin, ctx->npair, ctx->sq_ftmx,
+                     ctx->gr_h, ctx->sq_out);
+  NNPE_CHECK_LAUNCH(ctx);
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_sq_compute(nnpe_ctx* ctx, double* out) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!ctx->sq_ftmx) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_sq_compute before nnpe_sq_init");
+  int rc = sq_enqueue(ctx);
+  if (rc) return rc;
+  if (out) {
+    NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(out, ctx->sq_out, sizeof(double) * ctx->npair * ctx->sq_nq, hipMemcpyDeviceToHost));
+  }
+  return NNPE_OK;
+}

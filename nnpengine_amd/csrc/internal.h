@@ -1,0 +1,235 @@
+// internal.h -- device context of the nnpe engine (not part of the ABI).
+//
+// HBM layout (one context = one system):
+//  * real atoms sorted by cell; per-atom SoA arrays of 16-byte vectors so a wave's loads are
+//    coalesced 1 KiB transactions: xq = (x, y, z, q) FP32, v, f, fprev (xyz + pad);
+//    FP64 integration state xd/vd/ad as double4 when that mode is on;
+//  * info = (atom id, molecule id, intra id, molecule type), typ, 1/m, cell, inverse map;
+//  * ghosts are not materialised: a ghost is (source row, image code) sorted by ghost cell,
+//    its position is xq[src] + image shift computed in FP32 exactly as the reference's
+//    x_ghost = x + (+-L) (src/environ.cpp:431-433, :505-507);
+//  * neighbour lists are ELL-padded: entry k of atom i at nbr[k * stride + i], so lane i of a
+//    wave reads consecutive words; an entry packs (j row : 25 bits, image code : 5 bits,
+//    special class : 2 bits).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nnpe.h"
+
+#define NNPE_ENTRY_J(e) ((int)((e) & 0x1FFFFFFu))
+#define NNPE_ENTRY_CODE(e) ((int)(((e) >> 25) & 31u))
+#define NNPE_ENTRY_SP(e) ((int)((e) >> 30))
+#define NNPE_MAKE_ENTRY(j, code, sp) ((uint32_t)(j) | ((uint32_t)(code) << 25) | ((uint32_t)(sp) << 30))
+#define NNPE_MAX_ATOMS (1 << 25)
+
+// device counters, one small block read back by the host when needed
+struct DevCounters {
+  int nghost;          // ghosts generated by the last build
+  int overflow_long;   // max entries of any atom when > capacity
+  int overflow_short;
+  int ghost_overflow;
+  int cell_range;      // atoms outside the grid
+  int invalid;         // set by the isvalid kernel
+  int max_long, max_short;
+  int skin_violations; // fixed-interval rebuilds that came after an atom moved > skin/2
+  int log_count;       // MD log records written
+};
+
+struct NnpState;  // descriptor tables and moment buffers (nnp.hip)
+struct MlpState;  // per-element network weights (mlp.hip)
+
+struct nnpe_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // ---- system ---------------------------------------------------------------------------
+  int natom = 0, ntype = 0, npair = 0, nmol = 0, nmoltype = 0, stride = 0;
+  float bl = 0, r_bl = 0, h_bl = 0, m_bl = 0;
+  bool have_system = false;
+  float4 *xq = nullptr, *v = nullptr, *f = nullptr, *fprev = nullptr, *xprev = nullptr;
+  double4 *xd = nullptr, *vd = nullptr, *ad = nullptr;
+  float* rm = nullptr;
+  int* typ = nullptr;
+  int4* info = nullptr;     // atom id, mol id, intra id, molecule type
+  int* inv = nullptr;       // atom id -> row
+  int* cell = nullptr;      // cell of each row
+  int* xmol_code = nullptr; // image code of the unwrapped-molecule position of each row
+  // permutation scratch
+  float4 *t_xq = nullptr, *t_v = nullptr, *t_fprev = nullptr, *t_xprev = nullptr;
+  double4 *t_xd = nullptr, *t_vd = nullptr, *t_ad = nullptr;
+  float* t_rm = nullptr;
+  int *t_typ = nullptr, *t_cell = nullptr, *order = nullptr;
+  int4* t_info = nullptr;
+  int* mol_first = nullptr;                      // atom id of each molecule's intra-0 atom
+  int *mt_natom = nullptr, *mt_special_off = nullptr, *mt_special = nullptr;
+  std::vector<int> h_mt_natom;
+
+  // ---- grid / ghosts / lists -----------------------------------------------------------
+  bool have_grid = false, have_list = false;
+  float pair_cutoff = 0, skin = 0, gr_cutoff = 0, short_cutoff = 0, comm_cutoff = 0;
+  float cutsq_list = 0, cutsq_short = 0, dx2 = 0;
+  int num_gx = 0, num_gxsq = 0, num_grid = 0;
+  float g_dx = 0, g_rdx = 0, g_offset = 0, ghost_lim = 0;
+  int *cell_start = nullptr, *cell_cnt = nullptr, *cell_fill = nullptr;   // [num_grid (+1)]
+  int *gcell_start = nullptr, *gcell_cnt = nullptr, *gcell_fill = nullptr;
+  int ghost_cap = 0;
+  int *g_src = nullptr, *g_code = nullptr, *g_cell = nullptr;  // unsorted ghosts
+  int *gs_src = nullptr, *gs_code = nullptr;                  // sorted by cell
+  int4* stencil_pair = nullptr; int n_stencil_pair = 0;
+  int4* stencil_gr = nullptr; int n_stencil_gr = 0;
+  int cap_long = 0, cap_short = 0;
+  uint32_t *nbr_long = nullptr, *nbr_short = nullptr;
+  int *cnt_long = nullptr, *cnt_short = nullptr;
+  DevCounters* counters = nullptr;     // device
+  DevCounters* h_counters = nullptr;   // pinned host mirror
+  int* scan_tmp = nullptr;             // block sums for the cell scans
+  int nbuild = 0;
+
+  // ---- energy / virial partial sums (deterministic two-stage reduction) ----------------
+  double* ev_partial = nullptr;   // [NNPE_NTERM][2][max_blocks]
+  double* ev_sum = nullptr;       // [NNPE_NTERM][2] accumulated
+  double* h_ev = nullptr;         // pinned
+  int ev_max_blocks = 0;
+
+  // ---- pair table --------------------------------------------------------------------------
+  bool have_pair = false;
+  float pt_cut = 0, pt_cutsq = 0, pt_rdr = 0, pt_fc = 0, pt_vc = 0;
+  int pt_nbin = 0;
+  float* pt_tab = nullptr;  // [npair][nbin+1] force then [npair][nbin+1] potential
+
+  // ---- bonded ------------------------------------------------------------------------------
+  int nbtyp = 0, natyp = 0, ndtyp = 0, nbond = 0, nangle = 0, ndih = 0;
+  float *bcoef = nullptr, *acoef = nullptr, *dcoef = nullptr;
+  int *bterm = nullptr, *aterm = nullptr, *dterm = nullptr;   // atom ids as given
+  int *bstart = nullptr, *astart = nullptr, *dstart = nullptr; // per-molecule offsets [nmol+1]
+
+  // ---- verlet --------------------------------------------------------------------------------
+  double dt = 1.0;
+  bool fp64 = false;
+
+  // ---- g(r) / S(q) --------------------------------------------------------------------------
+  int gr_nbin = 0, gr_nbin_inner = 0;
+  float gr_dr = 0, gr_rdr = 0, gr_rmaxsq = 0, gr_rmaxsq_inner = 0;
+  unsigned long long* gr_count = nullptr;  // [npair][gr_nbin]
+  double* gr_h = nullptr;
+  double* gr_norm = nullptr;
+  int sq_nq = 0;
+  double sq_qmin = 0, sq_qmax = 0;
+  double* sq_out = nullptr;
+  double* sq_ftmx = nullptr;  // [nq][nbin]
+  std::vector<int> h_typecount;
+
+  // ---- NNP: descriptors G and dE/dG by row, per-atom network energies ---------------------
+  NnpState* nnp = nullptr;
+  MlpState* mlp = nullptr;
+  float* G = nullptr;
+  float* dG = nullptr;
+  int D_cur = 0, D_cap = 0;
+  float* e_atom = nullptr;              // [natom] by row
+  int* type_idx = nullptr;              // rows grouped by element: [ntype][typecount]
+  int* type_fill = nullptr;             // [ntype]
+  std::vector<int> h_type_off;          // element offsets into type_idx
+
+  // ---- MD driver -----------------------------------------------------------------------------
+  long long md_step = 0;                // steps completed by nnpe_md_run / the step calls
+  void* md = nullptr;                   // captured step graphs (md.hip)
+  double* md_log = nullptr;             // device log records
+  int md_log_cap = 0;
+};
+
+// enqueue helpers (no host synchronisation; safe inside stream capture)
+int neigh_enqueue_build(nnpe_ctx* ctx);
+int pair_enqueue(nnpe_ctx* ctx, bool ev);
+int bonded_enqueue(nnpe_ctx* ctx, bool ev);
+int verlet_enqueue_x(nnpe_ctx* ctx);
+int verlet_enqueue_v(nnpe_ctx* ctx);
+int ke_enqueue(nnpe_ctx* ctx);
+int gr_enqueue(nnpe_ctx* ctx, bool full);
+int sq_enqueue(nnpe_ctx* ctx);
+int nnp_tbl_desc_enqueue(nnpe_ctx* ctx);
+int nnp_tbl_force_enqueue(nnpe_ctx* ctx, bool ev);
+int sf_desc_enqueue(nnpe_ctx* ctx);
+int sf_force_enqueue(nnpe_ctx* ctx, bool ev);
+int sf_adjoint_enqueue(nnpe_ctx* ctx);
+int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev);
+int mlp_enqueue(nnpe_ctx* ctx, bool ev);
+void md_state_free(nnpe_ctx* ctx);
+int neigh_check_and_grow(nnpe_ctx* ctx, bool* rebuilt);
+
+// NNP helpers shared by nnp.hip / mlp.hip / md.hip
+NnpState* nnp_state_of(nnpe_ctx* ctx);
+void nnp_state_free(nnpe_ctx* ctx);
+void mlp_state_free(nnpe_ctx* ctx);
+int nnp_desc_reserve(nnpe_ctx* ctx, int D);
+int sf_dim(int ntype);
+int type_index_enqueue(nnpe_ctx* ctx);
+
+// ---- helpers -------------------------------------------------------------------------------
+int ctx_fail(nnpe_ctx* ctx, int code, const std::string& msg);
+#define NNPE_CHECK_HIP(ctx, call)                                                               \
+  do {                                                                                          \
+    hipError_t _e = (call);                                                                     \
+    if (_e != hipSuccess) return ctx_fail((ctx), NNPE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+#define NNPE_CHECK_LAUNCH(ctx) NNPE_CHECK_HIP(ctx, hipGetLastError())
+
+template <class T>
+inline hipError_t dev_alloc(T*& p, size_t n) {
+  if (p) { (void)hipFree(p); p = nullptr; }
+  if (n == 0) n = 1;
+  return hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+}
+template <class T>
+inline void dev_free(T*& p) {
+  if (p) { (void)hipFree(p); p = nullptr; }
+}
+
+inline int blocks_for(long long n, int bs) { return int((n + bs - 1) / bs); }
+
+// image code -> shift component, as the reference writes the ghost shift vector
+__device__ __forceinline__ float image_shift(int c, float bl, float m_bl) {
+  return c == 1 ? 0.0f : (c > 1 ? bl : m_bl);
+}
+__device__ __forceinline__ float4 image_pos(float4 xs, int code, float bl, float m_bl) {
+  if (code == 13) return xs;
+  float4 r = xs;
+  r.x = __fadd_rn(xs.x, image_shift(code / 9, bl, m_bl));
+  r.y = __fadd_rn(xs.y, image_shift((code / 3) % 3, bl, m_bl));
+  r.z = __fadd_rn(xs.z, image_shift(code % 3, bl, m_bl));
+  return r;
+}
+// (a-b)^2 summed left to right with IEEE single rounding and no contraction, as the
+// reference's sqr(dx)+sqr(dy)+sqr(dz) (src/neighlist.cpp:260)
+__device__ __forceinline__ float dist2_rn(float dx, float dy, float dz) {
+  return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+// energy/virial: block reduction of one double per thread into ev_partial[slot][blockIdx]
+template <int BS>
+__device__ __forceinline__ void block_sum2_store(double a, double b, double* part_a, double* part_b) {
+  __shared__ double sa[BS / 64], sb[BS / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { sa[w] = a; sb[w] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0, tb = 0;
+    for (int k = 0; k < BS / 64; ++k) { ta += sa[k]; tb += sb[k]; }
+    part_a[blockIdx.x] = ta;
+    part_b[blockIdx.x] = tb;
+  }
+}
+
+// host-side entry points shared between translation units
+hipError_t ev_reserve(nnpe_ctx* ctx, int nblocks);
+int ev_finish(nnpe_ctx* ctx, int term, int nblocks);  // add partials of a term into ev_sum
+double* ev_part(nnpe_ctx* ctx, int term, int which);  // partial array for term (0 energy, 1 virial)
+int counters_read(nnpe_ctx* ctx);

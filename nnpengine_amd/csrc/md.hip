@@ -1,0 +1,273 @@
+// md.hip -- the MD step driver (the omp-parallel loop of src/run.cpp:230-393 for classical and
+// :1127-1353 for nnp_run) on one HIP stream, with each distinct step shape captured once into a
+// hipGraph and replayed.
+//
+// Step (in the reference's order):
+//   updateX -> [rebuild: refold, sort, ghosts, lists] -> [g(r) tally (+ S(q))] ->
+//   forces: tabulated pair (writes f), bonded (+=), NNP descriptors -> MLP -> NNP forces (+=)
+//   -> updateV -> [kinetic energy + energy/virial log record]
+// Rebuilds are either every `rebuild_every` steps (device-side skin check counts any step that
+// came too late), or decided by NEIGHLIST::isvalid each step (host read-back, no graphs).
+// Graph safety: every enqueue helper launches kernels only (no host reads, no memsets, fixed
+// buffers); energy records are appended by a kernel at a device-held cursor.
+#include <algorithm>
+#include <map>
+
+#include "internal.h"
+
+namespace {
+
+enum StepBits { kRebuild = 1, kGr = 2, kGrFull = 4, kLog = 8 };
+
+struct MdGraphs {
+  std::map<int, hipGraphExec_t> exec;
+  std::map<int, hipGraph_t> graph;
+};
+
+__global__ void k_skin_check(int n, float dx2, const float4* xq, const float4* xprev, DevCounters* cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < n) {
+    float4 a = xq[i], b = xprev[i];
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    bad = dx * dx + dy * dy + dz * dz > dx2;
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&cnt->invalid, 1);
+}
+
+__global__ void k_skin_accum(DevCounters* cnt) {
+  if (cnt->invalid) cnt->skin_violations += 1;
+  cnt->invalid = 0;
+}
+
+__global__ void k_log_append(double* ev_sum, double* log, int cap, int nval, DevCounters* cnt) {
+  const int slot = cnt->log_count;
+  if (threadIdx.x < nval) {
+    if (slot < cap) log[(size_t)slot * nval + threadIdx.x] = ev_sum[threadIdx.x];
+    ev_sum[threadIdx.x] = 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) cnt->log_count = slot + 1;
+}
+
+__global__ void k_md_reset(DevCounters* cnt) {
+  cnt->log_count = 0;
+  cnt->skin_violations = 0;
+  cnt->invalid = 0;
+}
+
+__global__ void k_clear_force(int n, float4* f) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = make_float4(0, 0, 0, 0);
+}
+
+MdGraphs* graphs_of(nnpe_ctx* ctx) {
+  if (!ctx->md) ctx->md = new MdGraphs;
+  return static_cast<MdGraphs*>(ctx->md);
+}
+
+int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
+  int rc;
+  hipStream_t s = ctx->stream;
+  if (!first && (rc = verlet_enqueue_x(ctx))) return rc;
+  if (key & kRebuild) {
+    if (!first && P.rebuild_every > 0) {
+      hipLaunchKernelGGL(k_skin_check, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, s, ctx->natom, ctx->dx2, ctx->xq,
+                         ctx->xprev, ctx->counters);
+      hipLaunchKernelGGL(k_skin_accum, dim3(1), dim3(1), 0, s, ctx->counters);
+    }
+    if ((rc = neigh_enqueue_build(ctx))) return rc;
+  }
+  if (key & kGr) {
+    if ((rc = gr_enqueue(ctx, (key & kGrFull) != 0))) return rc;
+    if (P.sq && (rc = sq_enqueue(ctx))) return rc;
+  }
+  const bool ev = (key & kLog) != 0;
+  if (P.pair) {
+    if ((rc = pair_enqueue(ctx, ev))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_clear_force, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, s, ctx->natom, ctx->f);
+  }
+  if (P.bonded && (rc = bonded_enqueue(ctx, ev))) return rc;
+  if (P.nnp == 1) {
+    if ((rc = nnp_tbl_desc_enqueue(ctx)) || (rc = mlp_enqueue(ctx, ev)) || (rc = nnp_tbl_force_enqueue(ctx, ev))) return rc;
+  } else if (P.nnp == 2) {
+    if ((rc = sf_desc_enqueue(ctx)) || (rc = mlp_enqueue(ctx, ev)) || (rc = sf_force_enqueue(ctx, ev))) return rc;
+  }
+  if ((rc = verlet_enqueue_v(ctx))) return rc;
+  if (ev) {
+    if ((rc = ke_enqueue(ctx))) return rc;
+    hipLaunchKernelGGL(k_log_append, dim3(1), dim3(64), 0, s, ctx->ev_sum, ctx->md_log, ctx->md_log_cap, 2 * NNPE_NTERM,
+                       ctx->counters);
+  }
+  NNPE_CHECK_LAUNCH(ctx);
+  return NNPE_OK;
+}
+
+}  // namespace
+
+void md_state_free(nnpe_ctx* ctx) {
+  if (ctx->md) {
+    MdGraphs* G = static_cast<MdGraphs*>(ctx->md);
+    for (auto& kv : G->exec) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : G->graph) (void)hipGraphDestroy(kv.second);
+    delete G;
+    ctx->md = nullptr;
+  }
+}
+
+extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, double* log_out, int log_cap, int* nlog) {
+  if (!ctx || !p || nsteps < 0) return NNPE_ERR_ARG;
+  const nnpe_md_params P = *p;
+  if (!ctx->have_grid) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_md_run before nnpe_neigh_init");
+  if (P.pair && !ctx->have_pair) return ctx_fail(ctx, NNPE_ERR_STATE, "pair term without a table");
+  if (P.nnp && !ctx->mlp) return ctx_fail(ctx, NNPE_ERR_STATE, "NNP term without nnpe_mlp_set");
+  if (P.nnp == 1 && (!ctx->nnp || !nnp_state_of(ctx) || ctx->mlp == nullptr)) return ctx_fail(ctx, NNPE_ERR_STATE, "NNP tables missing");
+  if ((P.gr_every > 0) && !ctx->gr_count) return ctx_fail(ctx, NNPE_ERR_STATE, "g(r) requested without nnpe_gr_init");
+  if (P.sq && !ctx->sq_ftmx) return ctx_fail(ctx, NNPE_ERR_STATE, "S(q) requested without nnpe_sq_init");
+  if (P.gr_long_every > 0 && (P.gr_every <= 0 || P.gr_long_every % P.gr_every))
+    return ctx_fail(ctx, NNPE_ERR_ARG, "gr_long_every must be a multiple of gr_every");
+  if (ctx->dt != P.dt || ctx->fp64 != (P.fp64 != 0)) {
+    ctx->dt = P.dt;
+    ctx->fp64 = P.fp64 != 0;
+    md_state_free(ctx);
+  }
+  // log buffer
+  const int nrec = P.log_every > 0 ? nsteps / P.log_every + 2 : 1;
+  if (nrec > ctx->md_log_cap) {
+    md_state_free(ctx);  // graphs hold the old log pointer
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->md_log, size_t(nrec) * 2 * NNPE_NTERM));
+    ctx->md_log_cap = nrec;
+  }
+  hipLaunchKernelGGL(k_md_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->counters);
+  NNPE_CHECK_LAUNCH(ctx);
+  MdGraphs* G = graphs_of(ctx);
+  int rc;
+  for (int k = 0; k < nsteps; ++k) {
+    const long long s = ctx->md_step;
+    const bool first = s == 0 || !ctx->have_list;
+    int key = 0;
+    if (P.gr_every > 0 && s % P.gr_every == 0) key |= kGr | ((P.gr_long_every > 0 && s % P.gr_long_every == 0) ? kGrFull : 0);
+    if (P.log_every > 0 && s % P.log_every == 0) key |= kLog;
+    if (P.rebuild_every <= 0) {
+      // NEIGHLIST::isvalid every step (src/run.cpp:236-246): host decision, eager launches
+      if (!first && (rc = verlet_enqueue_x(ctx))) return rc;
+      int valid = 0;
+      if (!first && (rc = nnpe_neigh_isvalid(ctx, &valid))) return rc;
+      if (!valid) {
+        if ((rc = nnpe_neigh_build(ctx))) return rc;
+      }
+      // the rest of the step without the position update
+      nnpe_md_params Q = P;
+      if ((rc = enqueue_step(ctx, Q, key, true))) return rc;
+    } else {
+      if (first || s % P.rebuild_every == 0) key |= kRebuild;
+      if (first) {
+        // list first (with the capacity check and growth), then the step on the positions as read
+        if ((rc = nnpe_neigh_build(ctx))) return rc;
+        G = graphs_of(ctx);
+        if ((rc = enqueue_step(ctx, P, key & ~kRebuild, true))) return rc;
+      } else if (P.use_graph) {
+        auto it = G->exec.find(key);
+        if (it == G->exec.end()) {
+          hipGraph_t g = nullptr;
+          NNPE_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+          int rcs = enqueue_step(ctx, P, key, false);
+          hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+          if (rcs) return rcs;
+          NNPE_CHECK_HIP(ctx, ce);
+          hipGraphExec_t ex = nullptr;
+          NNPE_CHECK_HIP(ctx, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+          G->graph[key] = g;
+          it = G->exec.emplace(key, ex).first;
+        }
+        NNPE_CHECK_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
+      } else {
+        if ((rc = enqueue_step(ctx, P, key, false))) return rc;
+      }
+    }
+    ctx->md_step++;
+  }
+  if ((rc = counters_read(ctx))) return rc;
+  const DevCounters c = *ctx->h_counters;
+  if (nlog) *nlog = std::min(c.log_count, ctx->md_log_cap);
+  if (log_out && c.log_count > 0) {
+    const int n = std::min(std::min(c.log_count, ctx->md_log_cap), log_cap);
+    NNPE_CHECK_HIP(ctx, hipMemcpy(log_out, ctx->md_log, sizeof(double) * 2 * NNPE_NTERM * n, hipMemcpyDeviceToHost));
+  }
+  if (c.cell_range) return ctx_fail(ctx, NNPE_ERR_CELL_RANGE, "atoms left the cell grid during the run");
+  if (c.ghost_overflow || c.overflow_long || c.overflow_short) {
+    bool grown = false;
+    rc = neigh_check_and_grow(ctx, &grown);
+    md_state_free(ctx);
+    return rc ? rc : ctx_fail(ctx, NNPE_ERR_NEIGH_OVERFLOW, "list or ghost capacity exceeded during the run (grown; rerun)");
+  }
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_md_stats(nnpe_ctx* ctx, long long* out) {
+  if (!ctx || !out) return NNPE_ERR_ARG;
+  int rc = counters_read(ctx);
+  if (rc) return rc;
+  const DevCounters c = *ctx->h_counters;
+  out[0] = ctx->md_step;
+  out[1] = ctx->nbuild;
+  out[2] = c.skin_violations;
+  out[3] = c.max_long;
+  out[4] = c.max_short;
+  out[5] = c.nghost;
+  out[6] = ctx->cap_long;
+  out[7] = ctx->cap_short;
+  return NNPE_OK;
+}
+
+// Average duration of one stage over `reps` back-to-back launches, HIP events on the context's
+// stream (the stream the MD kernels run on).  The stage sees the current state; stages that
+// accumulate (forces += ...) are timed as launched, their results are not meant to be used.
+extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* avg_ms) {
+  if (!ctx || reps <= 0 || !avg_ms) return NNPE_ERR_ARG;
+  if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "profile needs a built list");
+  auto launch = [&](int st) -> int {
+    switch (st) {
+      case NNPE_STAGE_PAIR: return ctx->have_pair ? pair_enqueue(ctx, false) : NNPE_ERR_STATE;
+      case NNPE_STAGE_BONDED: return bonded_enqueue(ctx, false);
+      case NNPE_STAGE_SF_DESC: return sf_desc_enqueue(ctx);
+      case NNPE_STAGE_MLP: return ctx->mlp ? mlp_enqueue(ctx, false) : NNPE_ERR_STATE;
+      case NNPE_STAGE_SF_ADJOINT: return sf_adjoint_enqueue(ctx);
+      case NNPE_STAGE_SF_FORCE: return sf_force_only_enqueue(ctx, false);
+      case NNPE_STAGE_REBUILD: return neigh_enqueue_build(ctx);
+      case NNPE_STAGE_GR_FULL: return ctx->gr_count ? gr_enqueue(ctx, true) : NNPE_ERR_STATE;
+      case NNPE_STAGE_GR_LIST: return ctx->gr_count ? gr_enqueue(ctx, false) : NNPE_ERR_STATE;
+      case NNPE_STAGE_VERLET: {
+        int rc = verlet_enqueue_x(ctx);
+        return rc ? rc : verlet_enqueue_v(ctx);
+      }
+      case NNPE_STAGE_SQ: return ctx->sq_ftmx ? sq_enqueue(ctx) : NNPE_ERR_STATE;
+      case NNPE_STAGE_TBL_DESC: return nnp_tbl_desc_enqueue(ctx);
+      case NNPE_STAGE_TBL_FORCE: return nnp_tbl_force_enqueue(ctx, false);
+      default: return NNPE_ERR_ARG;
+    }
+  };
+  int rc = launch(stage);  // warm-up (and argument check)
+  if (rc) return rc == NNPE_ERR_STATE ? ctx_fail(ctx, rc, "stage not configured") : rc;
+  hipEvent_t e0, e1;
+  NNPE_CHECK_HIP(ctx, hipEventCreate(&e0));
+  NNPE_CHECK_HIP(ctx, hipEventCreate(&e1));
+  NNPE_CHECK_HIP(ctx, hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps && !rc; ++r) rc = launch(stage);
+  NNPE_CHECK_HIP(ctx, hipEventRecord(e1, ctx->stream));
+  NNPE_CHECK_HIP(ctx, hipEventSynchronize(e1));
+  float ms = 0;
+  NNPE_CHECK_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *avg_ms = double(ms) / reps;
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_md_reset_step(nnpe_ctx* ctx) {
+  if (!ctx) return NNPE_ERR_ARG;
+  ctx->md_step = 0;
+  return NNPE_OK;
+}

@@ -1,0 +1,482 @@
+// mlp.hip -- per-element MLP energy and its input gradient on MFMA (replaces the TensorFlow
+// call TFCALL::evaluate, src/pair_nnp.cpp:92-97, behind the same contract: descriptors in,
+// per-atom energy and dE/dG out).
+//
+// Network of element e:  E_i = s_out * (w_out . a_L + b_out),  a_l = tanh(a_{l-1} W_l + b_l),
+// a_0 = G_i.  Backward (hand-written): d_L = s_out * w_out * (1 - a_L^2),
+// d_{l-1} = (d_l W_l^T) * (1 - a_{l-1}^2),  dE/dG = d_1 W_1^T.
+//
+// Kernel shape: one wave (64 lanes) owns a tile of 32 atoms of one element and runs the whole
+// forward and backward for them.  Activations live in a wave-private LDS slab
+// [32 x width] per layer (padded rows, no bank-conflicting strides); every GEMM is
+// 32 x N x K on v_mfma_f32_32x32x16_{bf16,f16} (K = 16 per instruction) or
+// v_mfma_f32_32x32x2f32 for the FP32 form, with FP32 accumulation.  Weights are read as MFMA B
+// fragments straight from global memory (L2-resident: 86 KB per element in BF16), in two
+// layouts: [out][in] for the forward products, [in][out] for the backward ones, so each lane's
+// B fragment is one contiguous 16-byte load.
+// Precision forms: FP32 (fp32 storage, fp32 MFMA), BF16 and FP16 (16-bit storage of
+// activations and weights, fp32 accumulation, bias/tanh/epilogues in fp32).  Per-layer
+// power-of-two scales keep FP16 operands in range: input scale s_g, weight scale s_w[l],
+// gradient scale s_d[l]; each accumulator is unscaled in fp32 before use.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int kMaxLayers = 6;   // weight layers including the output layer
+constexpr int kMaxType = 4;
+
+struct MlpDev {
+  int nh;                       // hidden layers (weight layers - 1)
+  int D, Dp;                    // descriptor width and its 32-multiple padding
+  int H[kMaxLayers];            // hidden widths
+  int ld[kMaxLayers + 1];       // LDS row strides (elements): [0] = G slab, [1 + l] = layer l
+  int slab_off[kMaxLayers + 1]; // element offsets of the slabs in LDS
+  int tile_off[kMaxType + 1];   // first tile of each element
+  int count[kMaxType];
+  int type_off[kMaxType];
+  const void* Wt[kMaxType][kMaxLayers];  // [out][in] storage type
+  const void* W[kMaxType][kMaxLayers];   // [in][out] storage type
+  const float* b[kMaxType][kMaxLayers];
+  const float* wout[kMaxType];
+  float bout[kMaxType];
+  float out_scale;
+  float s_g, s_w[kMaxLayers], s_d[kMaxLayers];
+};
+
+template <int KIND> struct Prec;
+template <> struct Prec<0> {  // FP32
+  typedef float T;
+  static constexpr int KS = 2;
+  __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], c, 0, 0, 0);
+  }
+  __device__ static T cvt(float v) { return v; }
+  __device__ static float back(T v) { return v; }
+};
+template <> struct Prec<1> {  // BF16
+  typedef __bf16 T;
+  static constexpr int KS = 16;
+  __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8*>(a), *reinterpret_cast<const bf16x8*>(b), c, 0, 0, 0);
+  }
+  __device__ static T cvt(float v) { return (__bf16)v; }
+  __device__ static float back(T v) { return (float)v; }
+};
+template <> struct Prec<2> {  // FP16
+  typedef _Float16 T;
+  static constexpr int KS = 16;
+  __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(*reinterpret_cast<const f16x8*>(a), *reinterpret_cast<const f16x8*>(b), c, 0, 0, 0);
+  }
+  __device__ static T cvt(float v) { return (_Float16)v; }
+  __device__ static float back(T v) { return (float)v; }
+};
+
+// C/D fragment of a 32x32 MFMA: lane l, element j -> row 8*(j/4) + 4*(l/32) + j%4, col l%32
+__device__ __forceinline__ int crow(int lane, int j) { return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3); }
+
+// out[32 x 32 block cb] = A[32 x K] (LDS, row stride lda) * B, B fragment rows from global
+// `bm` laid out [n][K] (k contiguous).  A/B fragments: lane l holds row/col l%32 and the
+// KS/2 consecutive k starting at ks*KS + (KS/2)*(l/32).
+template <int KIND>
+__device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, int lda, const typename Prec<KIND>::T* bm,
+                                             int K, int cb, int lane) {
+  typedef Prec<KIND> P;
+  constexpr int KS = P::KS, HALF = KS / 2;
+  f32x16 acc;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+  const typename P::T* arow = A + (lane & 31) * lda + HALF * (lane >> 5);
+  const typename P::T* brow = bm + (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
+  for (int k0 = 0; k0 < K; k0 += KS) acc = P::mma(arow + k0, brow + k0, acc);
+  return acc;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
+                                           float* __restrict__ e_atom, const int* __restrict__ type_idx) {
+  typedef typename Prec<KIND>::T T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
+  const int lane = threadIdx.x;
+  const int tile = blockIdx.x;
+  int e = 0;
+  while (e + 1 < kMaxType && tile >= P.tile_off[e + 1]) ++e;
+  const int r0 = (tile - P.tile_off[e]) * 32;
+  const int nrows = min(32, P.count[e] - r0);
+  const int* rows = type_idx + P.type_off[e] + r0;
+  const int D = P.D, Dp = P.Dp, nh = P.nh;
+  T* Gs = lds + P.slab_off[0];
+  const int ldg = P.ld[0];
+
+  // stage the descriptor rows (scaled), zero padding rows/cols
+  for (int idx = lane; idx < 32 * Dp; idx += 64) {
+    const int r = idx / Dp, c = idx - r * Dp;
+    float v = 0.0f;
+    if (r < nrows && c < D) v = G[(size_t)rows[r] * D + c] * P.s_g;
+    Gs[r * ldg + c] = Prec<KIND>::cvt(v);
+  }
+  __syncthreads();
+
+  // forward
+  for (int l = 0; l < nh; ++l) {
+    const T* src = l == 0 ? Gs : lds + P.slab_off[l];
+    const int lds_src = P.ld[l];
+    const int K = l == 0 ? Dp : P.H[l - 1];
+    T* dst = lds + P.slab_off[l + 1];
+    const int ldd = P.ld[l + 1];
+    const T* wt = reinterpret_cast<const T*>(P.Wt[e][l]);
+    const float* bias = P.b[e][l];
+    const float un = 1.0f / ((l == 0 ? P.s_g : 1.0f) * P.s_w[l]);
+    for (int cb = 0; cb < P.H[l] / 32; ++cb) {
+      f32x16 acc = gemm_block<KIND>(src, lds_src, wt, K, cb, lane);
+      const int col = cb * 32 + (lane & 31);
+      const float bc = bias[col];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) dst[crow(lane, j) * ldd + col] = Prec<KIND>::cvt(tanhf(acc[j] * un + bc));
+    }
+    __syncthreads();
+  }
+
+  // energy of each row: lanes l and l+32 split the last hidden layer
+  const int HL = P.H[nh - 1];
+  const T* aL = lds + P.slab_off[nh];
+  const int ldl = P.ld[nh];
+  {
+    const int r = lane & 31, half = lane >> 5;
+    const float* wo = P.wout[e];
+    float s = 0.0f;
+    for (int k = half * (HL / 2); k < (half + 1) * (HL / 2); ++k) s += Prec<KIND>::back(aL[r * ldl + k]) * wo[k];
+    s += __shfl_xor(s, 32, 64);
+    if (half == 0 && r < nrows) e_atom[rows[r]] = P.out_scale * (s + P.bout[e]);
+  }
+  __syncthreads();
+
+  // backward: d_L = s_out * w_out * (1 - a_L^2), stored in place of a_L (scaled by s_d[nh-1])
+  {
+    T* dL = lds + P.slab_off[nh];
+    const float* wo = P.wout[e];
+    const float sc = P.out_scale * P.s_d[nh - 1];
+    for (int idx = lane; idx < 32 * HL; idx += 64) {
+      const int r = idx / HL, c = idx - r * HL;
+      const float a = Prec<KIND>::back(dL[r * ldl + c]);
+      dL[r * ldl + c] = Prec<KIND>::cvt(sc * wo[c] * (1.0f - a * a));
+    }
+  }
+  __syncthreads();
+  for (int l = nh - 1; l >= 1; --l) {
+    // d_{l-1} = (d_l W_l^T) * (1 - a_{l-1}^2): A = d_l [32 x H_l], B[k=out][n=in] = W_l [in][out]
+    const T* dl = lds + P.slab_off[l + 1];
+    const int ldl2 = P.ld[l + 1];
+    T* prev = lds + P.slab_off[l];
+    const int ldp = P.ld[l];
+    const T* w = reinterpret_cast<const T*>(P.W[e][l]);
+    const float un = 1.0f / (P.s_d[l] * P.s_w[l]);
+    const float sd = P.s_d[l - 1];
+    for (int cb = 0; cb < P.H[l - 1] / 32; ++cb) {
+      f32x16 acc = gemm_block<KIND>(dl, ldl2, w, P.H[l], cb, lane);
+      const int col = cb * 32 + (lane & 31);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        T* p = prev + crow(lane, j) * ldp + col;
+        const float a = Prec<KIND>::back(*p);
+        *p = Prec<KIND>::cvt(acc[j] * un * (1.0f - a * a) * sd);
+      }
+    }
+    __syncthreads();
+  }
+  // dE/dG = d_1 W_1^T: A = d_1 [32 x H_0], B = W_0 [Dp][H_0]
+  {
+    const T* d1 = lds + P.slab_off[1];
+    const T* w = reinterpret_cast<const T*>(P.W[e][0]);
+    const float un = 1.0f / (P.s_d[0] * P.s_w[0]);
+    for (int cb = 0; cb < Dp / 32; ++cb) {
+      f32x16 acc = gemm_block<KIND>(d1, P.ld[1], w, P.H[0], cb, lane);
+      const int col = cb * 32 + (lane & 31);
+      if (col < D) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int r = crow(lane, j);
+          if (r < nrows) dG[(size_t)rows[r] * D + col] = acc[j] * un;
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_type_index(int n, const int* typ, const int* off, int* fill, int* idx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = typ[i];
+  idx[off[t] + atomicAdd(&fill[t], 1)] = i;
+}
+
+__global__ void k_zero_small(int* p, int n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ void k_esum(int n, const float* e, double* part, double* part2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  block_sum2_store<256>(i < n ? double(e[i]) : 0.0, 0.0, part, part2);
+}
+
+}  // namespace
+
+struct MlpState {
+  MlpDev dev{};
+  int kind = 1;
+  int nl = 0;
+  std::vector<int> widths;
+  size_t lds_bytes = 0;
+  int ntiles = 0;
+  std::vector<void*> bufs;
+  int* d_type_off = nullptr;
+};
+
+void mlp_state_free(nnpe_ctx* ctx) {
+  MlpState* S = ctx->mlp;
+  if (!S) return;
+  for (void* p : S->bufs) (void)hipFree(p);
+  dev_free(S->d_type_off);
+  delete S;
+  ctx->mlp = nullptr;
+}
+
+static float pow2_scale(double maxabs) {
+  if (!(maxabs > 0) || !std::isfinite(maxabs)) return 1.0f;
+  return float(std::ldexp(1.0, -int(std::floor(std::log2(maxabs)))));
+}
+
+// rows of each element (rebuilt after every re-sort)
+int type_index_enqueue(nnpe_ctx* ctx) {
+  if (!ctx->type_idx) return NNPE_OK;
+  MlpState* S = ctx->mlp;
+  hipLaunchKernelGGL(k_zero_small, dim3(1), dim3(64), 0, ctx->stream, ctx->type_fill, ctx->ntype);
+  hipLaunchKernelGGL(k_type_index, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->typ,
+                     S->d_type_off, ctx->type_fill, ctx->type_idx);
+  NNPE_CHECK_LAUNCH(ctx);
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const float* params, float out_scale,
+                            int precision, float in_absmax) {
+  if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (!ctx || nlayer < 2 || nlayer > kMaxLayers || !widths || !params || precision < 0 || precision > 2)
+    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_mlp_set: bad argument (2..6 weight layers, precision 0/1/2)");
+  if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
+  if (ctx->ntype > kMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "at most 4 elements");
+  if (widths[nlayer] != 1) return ctx_fail(ctx, NNPE_ERR_ARG, "the output layer must have width 1");
+  for (int l = 1; l < nlayer; ++l)
+    if (widths[l] % 32 || widths[l] <= 0) return ctx_fail(ctx, NNPE_ERR_ARG, "hidden widths must be multiples of 32");
+  mlp_state_free(ctx);
+  MlpState* S = ctx->mlp = new MlpState;
+  S->kind = precision; S->nl = nlayer;
+  S->widths.assign(widths, widths + nlayer + 1);
+  MlpDev& d = S->dev;
+  d.nh = nlayer - 1;
+  d.D = widths[0];
+  d.Dp = (d.D + 31) / 32 * 32;
+  for (int l = 0; l < d.nh; ++l) d.H[l] = widths[l + 1];
+  d.out_scale = out_scale;
+  const int esz = precision == 0 ? 4 : 2;
+  const int pad = precision == 0 ? 4 : 8;  // LDS row padding (elements)
+  d.ld[0] = d.Dp + pad;
+  for (int l = 0; l < d.nh; ++l) d.ld[l + 1] = d.H[l] + pad;
+  int off = 0;
+  for (int s = 0; s <= d.nh; ++s) { d.slab_off[s] = off; off += 32 * d.ld[s]; }
+  S->lds_bytes = size_t(off) * esz;
+  if (S->lds_bytes > 64 * 1024) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "MLP tile does not fit the 64 KiB LDS budget");
+  // tiles of 32 atoms per element
+  std::vector<int> toff(ctx->ntype + 1, 0);
+  ctx->h_type_off.assign(ctx->ntype, 0);
+  int acc = 0;
+  for (int t = 0; t < ctx->ntype; ++t) {
+    ctx->h_type_off[t] = acc;
+    acc += ctx->h_typecount[t];
+    d.count[t] = ctx->h_typecount[t];
+    d.type_off[t] = ctx->h_type_off[t];
+    toff[t + 1] = toff[t] + (ctx->h_typecount[t] + 31) / 32;
+  }
+  for (int t = 0; t <= kMaxType; ++t) d.tile_off[t] = toff[std::min(t, ctx->ntype)];
+  S->ntiles = toff[ctx->ntype];
+  // weights: per element, per layer W [in][out] then b [out]
+  size_t pofs = 0;
+  std::vector<double> maxw(nlayer, 0.0), rowsum(nlayer, 0.0);
+  for (int t = 0; t < ctx->ntype; ++t) {
+    size_t o = pofs + 0;
+    for (int l = 0; l < nlayer; ++l) {
+      const int in = widths[l], out = widths[l + 1];
+      for (int i = 0; i < in; ++i) {
+        double rs = 0;
+        for (int k = 0; k < out; ++k) {
+          double w = std::fabs(params[o + size_t(i) * out + k]);
+          maxw[l] = std::max(maxw[l], w);
+          rs += w;
+        }
+        rowsum[l] = std::max(rowsum[l], rs);
+      }
+      o += size_t(in) * out + out;
+    }
+    pofs = o;
+  }
+  // per-layer scales (FP16 form only; BF16 and FP32 have the FP32 exponent range)
+  d.s_g = 1.0f;
+  for (int l = 0; l < kMaxLayers; ++l) { d.s_w[l] = 1.0f; d.s_d[l] = 1.0f; }
+  if (precision == 2) {
+    d.s_g = in_absmax > 0 ? pow2_scale(in_absmax) : 1.0f;
+    for (int l = 0; l < d.nh; ++l) d.s_w[l] = pow2_scale(maxw[l]);
+    // gradient bounds: |d_L| <= s_out max|w_out|, |d_{l-1}| <= |d_l| * max_i sum_j |W_l[i][j]|
+    double bnd = std::fabs(out_scale) * maxw[nlayer - 1];
+    d.s_d[d.nh - 1] = pow2_scale(bnd);
+    for (int l = d.nh - 1; l >= 1; --l) { bnd *= rowsum[l]; d.s_d[l - 1] = pow2_scale(bnd); }
+  }
+  pofs = 0;
+  for (int t = 0; t < ctx->ntype; ++t) {
+    for (int l = 0; l < nlayer; ++l) {
+      const int in = widths[l], out = widths[l + 1];
+      const float* W = params + pofs;
+      const float* b = W + size_t(in) * out;
+      pofs += size_t(in) * out + out;
+      if (l == nlayer - 1) {  // output layer: fp32 vector + scalar
+        float* dw = nullptr;
+        NNPE_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&dw), sizeof(float) * in));
+        S->bufs.push_back(dw);
+        NNPE_CHECK_HIP(ctx, hipMemcpy(dw, W, sizeof(float) * in, hipMemcpyHostToDevice));
+        d.wout[t] = dw;
+        d.bout[t] = b[0];
+        continue;
+      }
+      const int inp = l == 0 ? d.Dp : in;
+      const float sw = d.s_w[l];
+      std::vector<float> wt(size_t(out) * inp, 0.0f), wn(size_t(inp) * out, 0.0f);
+      for (int i = 0; i < in; ++i)
+        for (int k = 0; k < out; ++k) {
+          wt[size_t(k) * inp + i] = W[size_t(i) * out + k] * sw;
+          wn[size_t(i) * out + k] = W[size_t(i) * out + k] * sw;
+        }
+      auto upload = [&](const std::vector<float>& h, const void*& dst) -> int {
+        void* p = nullptr;
+        NNPE_CHECK_HIP(ctx, hipMalloc(&p, h.size() * esz));
+        S->bufs.push_back(p);
+        if (precision == 0) {
+          NNPE_CHECK_HIP(ctx, hipMemcpy(p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        } else {
+          std::vector<uint16_t> hb(h.size());
+          for (size_t k = 0; k < h.size(); ++k) {
+            if (precision == 1) {
+              uint32_t u; std::memcpy(&u, &h[k], 4);
+              uint32_t lsb = (u >> 16) & 1u;  // round to nearest even
+              hb[k] = uint16_t((u + 0x7FFFu + lsb) >> 16);
+            } else {
+              _Float16 hv = (_Float16)h[k];
+              std::memcpy(&hb[k], &hv, 2);
+            }
+          }
+          NNPE_CHECK_HIP(ctx, hipMemcpy(p, hb.data(), hb.size() * 2, hipMemcpyHostToDevice));
+        }
+        dst = p;
+        return NNPE_OK;
+      };
+      int rc = upload(wt, d.Wt[t][l]);
+      if (rc) return rc;
+      rc = upload(wn, d.W[t][l]);
+      if (rc) return rc;
+      float* db = nullptr;
+      NNPE_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&db), sizeof(float) * out));
+      S->bufs.push_back(db);
+      NNPE_CHECK_HIP(ctx, hipMemcpy(db, b, sizeof(float) * out, hipMemcpyHostToDevice));
+      d.b[t][l] = db;
+    }
+  }
+  NNPE_CHECK_HIP(ctx, dev_alloc(S->d_type_off, ctx->ntype));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(S->d_type_off, ctx->h_type_off.data(), sizeof(int) * ctx->ntype, hipMemcpyHostToDevice));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->type_idx, ctx->natom));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->type_fill, ctx->ntype));
+  int rc = nnp_desc_reserve(ctx, d.D);
+  if (rc) return rc;
+  return type_index_enqueue(ctx);
+}
+
+int mlp_enqueue(nnpe_ctx* ctx, bool ev) {
+  MlpState* S = ctx->mlp;
+  const MlpDev& d = S->dev;
+  if (S->ntiles > 0) {
+    if (S->kind == 0)
+      hipLaunchKernelGGL(k_mlp<0>, dim3(S->ntiles), dim3(64), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
+    else if (S->kind == 1)
+      hipLaunchKernelGGL(k_mlp<1>, dim3(S->ntiles), dim3(64), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
+    else
+      hipLaunchKernelGGL(k_mlp<2>, dim3(S->ntiles), dim3(64), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
+    NNPE_CHECK_LAUNCH(ctx);
+  }
+  if (ev) {
+    const int nb = blocks_for(ctx->natom, 256);
+    hipLaunchKernelGGL(k_esum, dim3(nb), dim3(256), 0, ctx->stream, ctx->natom, ctx->e_atom, ev_part(ctx, NNPE_TERM_NN, 0),
+                       ev_part(ctx, NNPE_TERM_NN, 1));
+    NNPE_CHECK_LAUNCH(ctx);
+    return ev_finish(ctx, NNPE_TERM_NN, nb);
+  }
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_mlp_eval(nnpe_ctx* ctx, int ev) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!ctx->mlp) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_mlp_eval before nnpe_mlp_set");
+  if (ctx->D_cur != ctx->mlp->dev.D) return ctx_fail(ctx, NNPE_ERR_STATE, "descriptor width differs from the network input");
+  return mlp_enqueue(ctx, ev != 0);
+}
+
+// descriptor / gradient / per-atom energy exchange in atom-id order (the TFCALL input and
+// output buffers of the reference, src/pair_nnp.cpp:490-496, :697-706)
+__global__ void k_rows_to_ids(int n, int D, const int4* info, const float* src, float* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = info[i].x;
+  for (int k = 0; k < D; ++k) dst[(size_t)a * D + k] = src[(size_t)i * D + k];
+}
+__global__ void k_ids_to_rows(int n, int D, const int4* info, const float* src, float* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = info[i].x;
+  for (int k = 0; k < D; ++k) dst[(size_t)i * D + k] = src[(size_t)a * D + k];
+}
+
+extern "C" int nnpe_nnp_get(nnpe_ctx* ctx, int what, float* out, int* D) {
+  if (!ctx || what < 0 || what > 2) return NNPE_ERR_ARG;
+  if (!ctx->G || !ctx->D_cur) return ctx_fail(ctx, NNPE_ERR_STATE, "no descriptors computed");
+  const int Dc = what == 2 ? 1 : ctx->D_cur;
+  if (D) *D = Dc;
+  if (!out) return NNPE_OK;
+  float* tmp = nullptr;
+  NNPE_CHECK_HIP(ctx, hipMallocAsync(reinterpret_cast<void**>(&tmp), sizeof(float) * size_t(ctx->natom) * Dc, ctx->stream));
+  const float* src = what == 0 ? ctx->G : what == 1 ? ctx->dG : ctx->e_atom;
+  hipLaunchKernelGGL(k_rows_to_ids, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, Dc, ctx->info, src, tmp);
+  NNPE_CHECK_LAUNCH(ctx);
+  NNPE_CHECK_HIP(ctx, hipMemcpyAsync(out, tmp, sizeof(float) * size_t(ctx->natom) * Dc, hipMemcpyDeviceToHost, ctx->stream));
+  NNPE_CHECK_HIP(ctx, hipFreeAsync(tmp, ctx->stream));
+  NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_nnp_set_dedg(nnpe_ctx* ctx, const float* dGin, int D) {
+  if (!ctx || !dGin || D <= 0) return NNPE_ERR_ARG;
+  int rc = nnp_desc_reserve(ctx, D);
+  if (rc) return rc;
+  float* tmp = nullptr;
+  NNPE_CHECK_HIP(ctx, hipMallocAsync(reinterpret_cast<void**>(&tmp), sizeof(float) * size_t(ctx->natom) * D, ctx->stream));
+  NNPE_CHECK_HIP(ctx, hipMemcpyAsync(tmp, dGin, sizeof(float) * size_t(ctx->natom) * D, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_ids_to_rows, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, D, ctx->info, tmp, ctx->dG);
+  NNPE_CHECK_LAUNCH(ctx);
+  NNPE_CHECK_HIP(ctx, hipFreeAsync(tmp, ctx->stream));
+  NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->D_cur = D;
+  return NNPE_OK;
+}

@@ -1,0 +1,572 @@
+// neigh.hip -- cell grid, cell sort, ghost images and the full Verlet neighbour lists.
+//
+// Follows the reference's list construction (src/environ.cpp:143-155, :234-522,
+// src/neighlist.cpp:35-331) with the same FP32 arithmetic so that the set of interacting
+// (atom, atom, image) triples is bit-identical:
+//   refold x -= rint(x/L)*L, cell = int((offset + x) * 1/dx), ghosts for atoms within the comm
+//   cutoff of a face with x_ghost = x + (+-L), the per-cell stencil with "no distance check"
+//   cells, and the test (xi-xj)^2+(yi-yj)^2+(zi-zj)^2 < (rc+skin)^2 without contraction.
+// The GPU build differs in mechanics only: counting sorts with atomics followed by a per-cell
+// sort on the atom id (deterministic order), ghosts as (source row, image code) records, and a
+// full list written ELL-padded (thread i writes entry k at k*stride+i, so a wave's stores
+// coalesce) instead of per-thread half lists.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace {
+
+constexpr int kStencilNoCheck = 1;
+
+__global__ void k_zero_i32(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_reset_counters(DevCounters* c) {
+  if (threadIdx.x == 0) {
+    c->nghost = 0; c->overflow_long = 0; c->overflow_short = 0; c->ghost_overflow = 0;
+    c->cell_range = 0; c->max_long = 0; c->max_short = 0;
+  }
+}
+
+__device__ __forceinline__ int cell_coord(float g_offset, float x, float g_rdx) {
+  return int(__fmul_rn(__fadd_rn(g_offset, x), g_rdx));
+}
+
+// LOCAL::refold_and_tag (src/environ.cpp:234-262): wrap into the box and bin into the grid.
+__global__ void k_refold(int n, bool fp64, float bl, float r_bl, float g_offset, float g_rdx, int num_gx,
+                         float4* xq, double4* xd, int* cell_out, int* cell_cnt, DevCounters* cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = xq[i];
+  if (fp64) {
+    double4 d = xd[i];
+    double dbl = bl, drbl = 1.0 / double(bl);
+    d.x -= rint(d.x * drbl) * dbl; d.y -= rint(d.y * drbl) * dbl; d.z -= rint(d.z * drbl) * dbl;
+    xd[i] = d;
+    p.x = float(d.x); p.y = float(d.y); p.z = float(d.z);
+  } else {
+    p.x = __fsub_rn(p.x, __fmul_rn(rintf(__fmul_rn(p.x, r_bl)), bl));
+    p.y = __fsub_rn(p.y, __fmul_rn(rintf(__fmul_rn(p.y, r_bl)), bl));
+    p.z = __fsub_rn(p.z, __fmul_rn(rintf(__fmul_rn(p.z, r_bl)), bl));
+  }
+  xq[i] = p;
+  int cx = cell_coord(g_offset, p.x, g_rdx), cy = cell_coord(g_offset, p.y, g_rdx), cz = cell_coord(g_offset, p.z, g_rdx);
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= num_gx || cy >= num_gx || cz >= num_gx) {
+    atomicAdd(&cnt->cell_range, 1);
+    cx = min(max(cx, 0), num_gx - 1); cy = min(max(cy, 0), num_gx - 1); cz = min(max(cz, 0), num_gx - 1);
+  }
+  int c = (cx * num_gx + cy) * num_gx + cz;
+  cell_out[i] = c;
+  atomicAdd(&cell_cnt[c], 1);
+}
+
+// exclusive scan of n counts into start[0..n] with one 1024-thread block
+__global__ void __launch_bounds__(1024) k_scan_excl(const int* cnt, int* start, int n) {
+  __shared__ int s[1024];
+  int t = threadIdx.x;
+  int chunk = (n + 1023) / 1024;
+  int b = min(t * chunk, n), e = min(b + chunk, n);
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += cnt[k];
+  s[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = s[t] - sum;
+  for (int k = b; k < e; ++k) { start[k] = run; run += cnt[k]; }
+  if (t == 1023) start[n] = s[1023];
+}
+
+__global__ void k_scatter(int n, const int* cell_of, const int* start, int* fill, int* order) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell_of[i];
+  order[start[c] + atomicAdd(&fill[c], 1)] = i;
+}
+
+// deterministic order inside a cell: ascending atom id (insertion sort of a short segment)
+__global__ void k_cell_sort(int ncell, const int* start, int* order, const int4* info) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = start[c], e = start[c + 1];
+  for (int k = b + 1; k < e; ++k) {
+    int v = order[k], key = info[v].x, m = k - 1;
+    while (m >= b && info[order[m]].x > key) { order[m + 1] = order[m]; --m; }
+    order[m + 1] = v;
+  }
+}
+
+// ENVIRON::sort data movement (src/environ.cpp:319-361): rows permuted into cell order
+__global__ void k_gather(int n, bool fp64, const int* order, const int* cell_old, const float4* xq, const float4* v,
+                         const float4* fprev, const float* rm, const int* typ, const int4* info, const double4* xd,
+                         const double4* vd, const double4* ad, float4* xq2, float4* v2, float4* fprev2, float* rm2,
+                         int* typ2, int4* info2, double4* xd2, double4* vd2, double4* ad2, int* cell_new, int* inv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int o = order[i];
+  xq2[i] = xq[o]; v2[i] = v[o]; fprev2[i] = fprev[o]; rm2[i] = rm[o]; typ2[i] = typ[o];
+  int4 in = info[o];
+  info2[i] = in;
+  cell_new[i] = cell_old[o];
+  inv[in.x] = i;
+  if (fp64) { xd2[i] = xd[o]; vd2[i] = vd[o]; ad2[i] = ad[o]; }
+}
+
+__global__ void k_copy_back(int n, bool fp64, const float4* xq2, const float4* v2, const float4* fprev2, const float* rm2,
+                            const int* typ2, const int4* info2, const double4* xd2, const double4* vd2, const double4* ad2,
+                            float4* xq, float4* v, float4* fprev, float* rm, int* typ, int4* info, double4* xd, double4* vd,
+                            double4* ad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  xq[i] = xq2[i]; v[i] = v2[i]; fprev[i] = fprev2[i]; rm[i] = rm2[i]; typ[i] = typ2[i]; info[i] = info2[i];
+  if (fp64) { xd[i] = xd2[i]; vd[i] = vd2[i]; ad[i] = ad2[i]; }
+}
+
+// LOCAL::generate_ghost + update_ghost_pos<true> (src/environ.cpp:401-442, :488-514)
+__global__ void k_ghost_gen(int n, float ghost_lim, float bl, float m_bl, float g_offset, float g_rdx, int num_gx,
+                            int cap, const float4* xq, int* g_src, int* g_code, int* g_cell, int* gcell_cnt,
+                            DevCounters* cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = xq[i];
+  const float mg = -ghost_lim;
+  int xs = p.x > ghost_lim ? -1 : 0, xe = p.x < mg ? 1 : 0;
+  int ys = p.y > ghost_lim ? -1 : 0, ye = p.y < mg ? 1 : 0;
+  int zs = p.z > ghost_lim ? -1 : 0, ze = p.z < mg ? 1 : 0;
+  if (xs == xe && ys == ye && zs == ze) return;
+  for (int a = xs; a <= xe; ++a)
+    for (int b = ys; b <= ye; ++b)
+      for (int c = zs; c <= ze; ++c) {
+        if (a == 0 && b == 0 && c == 0) continue;
+        int code = (a + 1) * 9 + (b + 1) * 3 + (c + 1);
+        int slot = atomicAdd(&cnt->nghost, 1);
+        if (slot >= cap) { atomicAdd(&cnt->ghost_overflow, 1); continue; }
+        float4 g = image_pos(p, code, bl, m_bl);
+        int cx = cell_coord(g_offset, g.x, g_rdx), cy = cell_coord(g_offset, g.y, g_rdx), cz = cell_coord(g_offset, g.z, g_rdx);
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= num_gx || cy >= num_gx || cz >= num_gx) {
+          atomicAdd(&cnt->cell_range, 1);
+          cx = min(max(cx, 0), num_gx - 1); cy = min(max(cy, 0), num_gx - 1); cz = min(max(cz, 0), num_gx - 1);
+        }
+        int gc = (cx * num_gx + cy) * num_gx + cz;
+        g_src[slot] = i; g_code[slot] = code; g_cell[slot] = gc;
+        atomicAdd(&gcell_cnt[gc], 1);
+      }
+}
+
+__global__ void k_ghost_scatter(int cap, const DevCounters* cnt, const int* g_src, const int* g_code, const int* g_cell,
+                                const int* gstart, int* gfill, int* gs_src, int* gs_code) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int ng = min(cnt->nghost, cap);
+  if (s >= ng) return;
+  int c = g_cell[s];
+  int pos = gstart[c] + atomicAdd(&gfill[c], 1);
+  gs_src[pos] = g_src[s];
+  gs_code[pos] = g_code[s];
+}
+
+// ENVIRON::sort_ghost order made deterministic: ascending (source atom id, image code)
+__global__ void k_ghost_cell_sort(int ncell, const int* gstart, int* gs_src, int* gs_code, const int4* info) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = gstart[c], e = gstart[c + 1];
+  for (int k = b + 1; k < e; ++k) {
+    int s = gs_src[k], cd = gs_code[k];
+    long long key = (long long)info[s].x * 32 + cd;
+    int m = k - 1;
+    while (m >= b && (long long)info[gs_src[m]].x * 32 + gs_code[m] > key) {
+      gs_src[m + 1] = gs_src[m]; gs_code[m + 1] = gs_code[m]; --m;
+    }
+    gs_src[m + 1] = s; gs_code[m + 1] = cd;
+  }
+}
+
+// LOCAL::update_ghost_and_info x_mol part (src/environ.cpp:461-469): the image of an atom whose
+// every coordinate lies within L/2 of its molecule's intra-0 atom, among the images the atom
+// has ghosts for; the atom itself otherwise.
+__global__ void k_xmol(int n, float ghost_lim, float bl, float m_bl, float h_bl, const float4* xq, const int4* info,
+                       const int* mol_first, const int* inv, int* xmol_code) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int4 in = info[i];
+  int code_sel = 13;
+  if (in.z != 0) {
+    float4 p = xq[i];
+    float4 r = xq[inv[mol_first[in.y]]];
+    const float mg = -ghost_lim;
+    int xs = p.x > ghost_lim ? -1 : 0, xe = p.x < mg ? 1 : 0;
+    int ys = p.y > ghost_lim ? -1 : 0, ye = p.y < mg ? 1 : 0;
+    int zs = p.z > ghost_lim ? -1 : 0, ze = p.z < mg ? 1 : 0;
+    for (int a = xs; a <= xe; ++a)
+      for (int b = ys; b <= ye; ++b)
+        for (int c = zs; c <= ze; ++c) {
+          if (a == 0 && b == 0 && c == 0) continue;
+          int code = (a + 1) * 9 + (b + 1) * 3 + (c + 1);
+          float4 g = image_pos(p, code, bl, m_bl);
+          if (fabsf(__fsub_rn(g.x, r.x)) < h_bl && fabsf(__fsub_rn(g.y, r.y)) < h_bl && fabsf(__fsub_rn(g.z, r.z)) < h_bl)
+            code_sel = code;
+        }
+  }
+  xmol_code[i] = code_sel;
+}
+
+// NEIGHLIST::build (src/neighlist.cpp:190-331) as a full list: every real neighbour j != i and
+// every ghost image within (rc+skin), or in a stencil cell flagged "no check".  Optionally a
+// second (short) list of the entries within a smaller cutoff.
+template <bool kShort>
+__global__ void __launch_bounds__(256) k_build(int n, int stride, int num_gx, const int4* __restrict__ stencil, int nst,
+                                               float cutsq, float cutsq_s, float bl, float m_bl,
+                                               const float4* __restrict__ xq, const int4* __restrict__ info,
+                                               const int* __restrict__ cell, const int* __restrict__ cstart,
+                                               const int* __restrict__ gstart, const int* __restrict__ gs_src,
+                                               const int* __restrict__ gs_code, const int* __restrict__ mt_natom,
+                                               const int* __restrict__ mt_off, const int* __restrict__ mt_sp,
+                                               int cap, int cap_s, uint32_t* __restrict__ nbr, int* __restrict__ cnt_out,
+                                               uint32_t* __restrict__ nbr_s, int* __restrict__ cnt_s_out,
+                                               float4* __restrict__ xprev, DevCounters* counters) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 xi = xq[i];
+  xprev[i] = xi;
+  const int4 ii = info[i];
+  const int ci = cell[i];
+  const int cz = ci % num_gx, cy = (ci / num_gx) % num_gx, cx = ci / (num_gx * num_gx);
+  const int mtn = mt_natom[ii.w];
+  const int* sp_row = mt_sp + mt_off[ii.w] + ii.z * mtn;
+  int nl = 0, ns = 0;
+  auto emit = [&](int j, int code, float rsq, int4 ij) {
+    int sp = (ij.y == ii.y) ? sp_row[ij.z] : 0;
+    uint32_t e = NNPE_MAKE_ENTRY(j, code, sp);
+    if (nl < cap) nbr[(size_t)nl * stride + i] = e;
+    ++nl;
+    if (kShort && rsq < cutsq_s) {
+      if (ns < cap_s) nbr_s[(size_t)ns * stride + i] = e;
+      ++ns;
+    }
+  };
+  for (int s = 0; s < nst; ++s) {
+    const int4 st = stencil[s];
+    const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
+    if (ax < 0 || ay < 0 || az < 0 || ax >= num_gx || ay >= num_gx || az >= num_gx) continue;
+    const int c = (ax * num_gx + ay) * num_gx + az;
+    const bool nocheck = st.w & kStencilNoCheck;
+    for (int j = cstart[c], je = cstart[c + 1]; j < je; ++j) {
+      if (j == i) continue;
+      const float4 xj = xq[j];
+      const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+      if (nocheck || rsq < cutsq) emit(j, 13, rsq, info[j]);
+    }
+    for (int g = gstart[c], ge = gstart[c + 1]; g < ge; ++g) {
+      const int src = gs_src[g], code = gs_code[g];
+      const float4 xj = image_pos(xq[src], code, bl, m_bl);
+      const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+      if (nocheck || rsq < cutsq) emit(src, code, rsq, info[src]);
+    }
+  }
+  cnt_out[i] = min(nl, cap);
+  if (nl > cap) atomicMax(&counters->overflow_long, nl);
+  atomicMax(&counters->max_long, nl);
+  if (kShort) {
+    cnt_s_out[i] = min(ns, cap_s);
+    if (ns > cap_s) atomicMax(&counters->overflow_short, ns);
+    atomicMax(&counters->max_short, ns);
+  }
+}
+
+// NEIGHLIST::isvalid (src/neighlist.cpp:178-187)
+__global__ void k_isvalid(int n, float dx2, const float4* xq, const float4* xprev, DevCounters* cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < n) {
+    float4 a = xq[i], b = xprev[i];
+    float d = dist2_rn(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y), __fsub_rn(a.z, b.z));
+    bad = d > dx2;
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&cnt->invalid, 1);
+}
+
+__global__ void k_clear_invalid(DevCounters* cnt) { cnt->invalid = 0; }
+
+// list export in atom ids
+__global__ void k_export(int n, int stride, const uint32_t* nbr, const int* cnt, const long long* offs, const int4* info,
+                         int* ai, int* aj, int* code, int* sp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long long o = offs[i];
+  int a = info[i].x;
+  for (int k = 0; k < cnt[i]; ++k) {
+    uint32_t e = nbr[(size_t)k * stride + i];
+    ai[o + k] = a;
+    aj[o + k] = info[NNPE_ENTRY_J(e)].x;
+    code[o + k] = NNPE_ENTRY_CODE(e);
+    sp[o + k] = NNPE_ENTRY_SP(e);
+  }
+}
+
+// relative stencil with the reference's masks (src/neighlist.cpp:35-142): all cells whose
+// minimum squared cell distance is < ceil((cutoff/dx)^2); "no check" when the maximum is too.
+std::vector<int4> make_stencil(float cutoff, float g_rdx, int num_gx) {
+  float t = cutoff * g_rdx;
+  const int max_d = int(std::ceil(t * t));
+  std::vector<int4> out;
+  auto sq = [](int v) { return v * v; };
+  for (int dx = -(num_gx - 1); dx <= num_gx - 1; ++dx) {
+    int mx, mxx;
+    if (dx == 0) { mx = 0; mxx = 1; } else { int a = std::abs(dx); if ((mx = sq(a - 1)) >= max_d) continue; mxx = sq(a + 1); }
+    for (int dy = -(num_gx - 1); dy <= num_gx - 1; ++dy) {
+      int my, mxy;
+      if (dy == 0) { my = mx; mxy = mxx + 1; } else { int a = std::abs(dy); if ((my = mx + sq(a - 1)) >= max_d) continue; mxy = mxx + sq(a + 1); }
+      for (int dz = -(num_gx - 1); dz <= num_gx - 1; ++dz) {
+        int mz, mxz;
+        if (dz == 0) { mz = my; mxz = mxy + 1; } else { int a = std::abs(dz); if ((mz = my + sq(a - 1)) >= max_d) continue; mxz = mxy + sq(a + 1); }
+        (void)mz;
+        out.push_back(make_int4(dx, dy, dz, mxz < max_d ? kStencilNoCheck : 0));
+      }
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, float gr_cutoff, float short_cutoff) {
+  if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_neigh_init before nnpe_system_set");
+  if (!(pair_cutoff > 0) || !(skin >= 0) || short_cutoff > pair_cutoff)
+    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_neigh_init: bad cutoffs");
+  ctx->pair_cutoff = pair_cutoff; ctx->skin = skin; ctx->gr_cutoff = gr_cutoff; ctx->short_cutoff = short_cutoff;
+  // src/run.cpp:76-79
+  const float comm = skin + std::max(gr_cutoff, pair_cutoff);
+  ctx->comm_cutoff = comm;
+  if (!(ctx->h_bl > comm))
+    return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "box too small: L/2 must exceed skin + max(rc, r_gr) (single-image ghosts)");
+  ctx->ghost_lim = ctx->h_bl - comm;
+  const float gl = float(0.5 * (skin + pair_cutoff));
+  ctx->g_rdx = 1.0f / (ctx->g_dx = gl);
+  ctx->g_offset = comm + ctx->h_bl;
+  ctx->num_gx = int(std::ceil((ctx->g_offset * 2) * ctx->g_rdx));
+  ctx->num_gxsq = ctx->num_gx * ctx->num_gx;
+  ctx->num_grid = ctx->num_gxsq * ctx->num_gx;
+  const float lc = pair_cutoff + skin;
+  ctx->cutsq_list = lc * lc;  // NEIGHLIST ctor: cutsq = sqr(cutoff + skin)
+  const float sc = short_cutoff + skin;
+  ctx->cutsq_short = short_cutoff > 0 ? sc * sc : 0.0f;
+  const float hs = skin / 2;
+  ctx->dx2 = hs * hs;
+  auto sp = make_stencil(lc, ctx->g_rdx, ctx->num_gx);
+  auto sg = make_stencil(gr_cutoff + skin, ctx->g_rdx, ctx->num_gx);
+  ctx->n_stencil_pair = int(sp.size()); ctx->n_stencil_gr = int(sg.size());
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->stencil_pair, sp.size()));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->stencil_pair, sp.data(), sp.size() * sizeof(int4), hipMemcpyHostToDevice));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->stencil_gr, sg.size()));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->stencil_gr, sg.data(), sg.size() * sizeof(int4), hipMemcpyHostToDevice));
+  const int G = ctx->num_grid;
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_start, G + 1)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_cnt, G));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_fill, G));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gcell_start, G + 1)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gcell_cnt, G));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gcell_fill, G));
+  NNPE_CHECK_HIP(ctx, hipMemset(ctx->cell_start, 0, sizeof(int) * (G + 1)));
+  NNPE_CHECK_HIP(ctx, hipMemset(ctx->gcell_start, 0, sizeof(int) * (G + 1)));
+  // ghost capacity: the shell volume fraction with a 30% margin (+4096), at most 7 images/atom
+  double vin = double(ctx->bl), vout = double(ctx->bl) + 2.0 * comm;
+  double frac = (vout * vout * vout - vin * vin * vin) / (vin * vin * vin);
+  long long gcap = (long long)(1.3 * frac * ctx->natom) + 4096;
+  gcap = std::min<long long>(gcap, 7LL * ctx->natom + 64);
+  ctx->ghost_cap = int(gcap);
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_src, gcap)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_code, gcap));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_cell, gcap)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gs_src, gcap));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gs_code, gcap));
+  // list capacities: 1.5x the ideal-gas count of the shell at the system's mean density (+64),
+  // rounded to 64; grown on overflow
+  double rho = ctx->natom / (vin * vin * vin);
+  auto capfor = [&](float rc) {
+    double c = 4.0 / 3.0 * 3.141592653589793 * double(rc) * rc * rc * rho;
+    int k = int(1.5 * c) + 64;
+    return (k + 63) / 64 * 64;
+  };
+  ctx->cap_long = capfor(lc);
+  ctx->cap_short = short_cutoff > 0 ? capfor(sc) : 0;
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_long, size_t(ctx->cap_long) * ctx->stride));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cnt_long, ctx->stride));
+  NNPE_CHECK_HIP(ctx, hipMemset(ctx->cnt_long, 0, sizeof(int) * ctx->stride));
+  if (ctx->cap_short) {
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_short, size_t(ctx->cap_short) * ctx->stride));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cnt_short, ctx->stride));
+    NNPE_CHECK_HIP(ctx, hipMemset(ctx->cnt_short, 0, sizeof(int) * ctx->stride));
+  }
+  ctx->have_grid = true;
+  ctx->have_list = false;
+  return NNPE_OK;
+}
+
+// Enqueue the rebuild kernels (no host synchronisation; graph-capturable).
+int neigh_enqueue_build(nnpe_ctx* ctx) {
+  const int N = ctx->natom, G = ctx->num_grid, bs = 256;
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ctx->counters);
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->cell_cnt, G);
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->cell_fill, G);
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->gcell_cnt, G);
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->gcell_fill, G);
+  hipLaunchKernelGGL(k_refold, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->bl, ctx->r_bl, ctx->g_offset,
+                     ctx->g_rdx, ctx->num_gx, ctx->xq, ctx->xd, ctx->t_cell, ctx->cell_cnt, ctx->counters);
+  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->cell_cnt, ctx->cell_start, G);
+  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->t_cell, ctx->cell_start, ctx->cell_fill, ctx->order);
+  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(G, 64)), dim3(64), 0, s, G, ctx->cell_start, ctx->order, ctx->info);
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->order, ctx->t_cell, ctx->xq, ctx->v,
+                     ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->t_xq, ctx->t_v, ctx->t_fprev,
+                     ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->cell, ctx->inv);
+  // copy the permuted rows back so every array keeps its buffer (captured graphs replay with
+  // fixed pointers)
+  hipLaunchKernelGGL(k_copy_back, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->t_xq, ctx->t_v, ctx->t_fprev,
+                     ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->xq, ctx->v, ctx->fprev, ctx->rm,
+                     ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad);
+  NNPE_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_ghost_gen, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->ghost_lim, ctx->bl, ctx->m_bl, ctx->g_offset,
+                     ctx->g_rdx, ctx->num_gx, ctx->ghost_cap, ctx->xq, ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_cnt,
+                     ctx->counters);
+  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->gcell_cnt, ctx->gcell_start, G);
+  hipLaunchKernelGGL(k_ghost_scatter, dim3(blocks_for(ctx->ghost_cap, bs)), dim3(bs), 0, s, ctx->ghost_cap, ctx->counters,
+                     ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_start, ctx->gcell_fill, ctx->gs_src, ctx->gs_code);
+  hipLaunchKernelGGL(k_ghost_cell_sort, dim3(blocks_for(G, 64)), dim3(64), 0, s, G, ctx->gcell_start, ctx->gs_src, ctx->gs_code,
+                     ctx->info);
+  hipLaunchKernelGGL(k_xmol, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->ghost_lim, ctx->bl, ctx->m_bl, ctx->h_bl, ctx->xq,
+                     ctx->info, ctx->mol_first, ctx->inv, ctx->xmol_code);
+  if (ctx->cap_short) {
+    hipLaunchKernelGGL(k_build<true>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->stride, ctx->num_gx, ctx->stencil_pair,
+                       ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
+                       ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom,
+                       ctx->mt_special_off, ctx->mt_special, ctx->cap_long, ctx->cap_short, ctx->nbr_long, ctx->cnt_long,
+                       ctx->nbr_short, ctx->cnt_short, ctx->xprev, ctx->counters);
+  } else {
+    hipLaunchKernelGGL(k_build<false>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->stride, ctx->num_gx, ctx->stencil_pair,
+                       ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
+                       ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom,
+                       ctx->mt_special_off, ctx->mt_special, ctx->cap_long, ctx->cap_short, ctx->nbr_long, ctx->cnt_long,
+                       ctx->nbr_short, ctx->cnt_short, ctx->xprev, ctx->counters);
+  }
+  NNPE_CHECK_LAUNCH(ctx);
+  ctx->nbuild++;
+  return type_index_enqueue(ctx);  // rows of each element moved with the sort
+}
+
+// Check the build's counters; grow capacities and rebuild on list/ghost overflow.
+int neigh_check_and_grow(nnpe_ctx* ctx, bool* rebuilt) {
+  *rebuilt = false;
+  int rc = counters_read(ctx);
+  if (rc) return rc;
+  DevCounters c = *ctx->h_counters;
+  if (c.cell_range) return ctx_fail(ctx, NNPE_ERR_CELL_RANGE, "atoms outside the cell grid");
+  bool grow = false;
+  if (c.ghost_overflow || c.nghost > ctx->ghost_cap) {
+    ctx->ghost_cap = c.nghost + c.nghost / 4 + 1024;
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_src, ctx->ghost_cap)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_code, ctx->ghost_cap));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_cell, ctx->ghost_cap)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gs_src, ctx->ghost_cap));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gs_code, ctx->ghost_cap));
+    grow = true;
+  }
+  if (c.overflow_long) {
+    ctx->cap_long = (c.overflow_long + c.overflow_long / 4 + 63) / 64 * 64;
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_long, size_t(ctx->cap_long) * ctx->stride));
+    grow = true;
+  }
+  if (c.overflow_short) {
+    ctx->cap_short = (c.overflow_short + c.overflow_short / 4 + 63) / 64 * 64;
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_short, size_t(ctx->cap_short) * ctx->stride));
+    grow = true;
+  }
+  if (grow) {
+    md_state_free(ctx);
+    // the positions were refolded and sorted already; a second build sees the same set
+    int r = neigh_enqueue_build(ctx);
+    if (r) return r;
+    r = counters_read(ctx);
+    if (r) return r;
+    c = *ctx->h_counters;
+    if (c.ghost_overflow) return ctx_fail(ctx, NNPE_ERR_GHOST_OVERFLOW, "ghost overflow after growth");
+    if (c.overflow_long || c.overflow_short) return ctx_fail(ctx, NNPE_ERR_NEIGH_OVERFLOW, "nei_list overflow after growth");
+    *rebuilt = true;
+  }
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_neigh_build(nnpe_ctx* ctx) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!ctx->have_grid) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_neigh_build before nnpe_neigh_init");
+  int rc = neigh_enqueue_build(ctx);
+  if (rc) return rc;
+  bool rebuilt;
+  rc = neigh_check_and_grow(ctx, &rebuilt);
+  if (rc) return rc;
+  ctx->have_list = true;
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_neigh_isvalid(nnpe_ctx* ctx, int* valid) {
+  if (!ctx || !valid) return NNPE_ERR_ARG;
+  if (!ctx->have_list) { *valid = 0; return NNPE_OK; }
+  hipLaunchKernelGGL(k_clear_invalid, dim3(1), dim3(1), 0, ctx->stream, ctx->counters);
+  hipLaunchKernelGGL(k_isvalid, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->dx2, ctx->xq,
+                     ctx->xprev, ctx->counters);
+  NNPE_CHECK_LAUNCH(ctx);
+  int rc = counters_read(ctx);
+  if (rc) return rc;
+  *valid = ctx->h_counters->invalid ? 0 : 1;
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_neigh_size(nnpe_ctx* ctx, int which, long long* n_entries, int* n_ghost) {
+  if (!ctx || (which != 0 && which != 1)) return NNPE_ERR_ARG;
+  if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "no list");
+  if (which == 1 && !ctx->cap_short) return ctx_fail(ctx, NNPE_ERR_STATE, "no short list");
+  std::vector<int> h(ctx->natom);
+  NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(h.data(), which ? ctx->cnt_short : ctx->cnt_long, sizeof(int) * ctx->natom, hipMemcpyDeviceToHost));
+  long long s = 0;
+  for (int v : h) s += v;
+  if (n_entries) *n_entries = s;
+  int rc = counters_read(ctx);
+  if (rc) return rc;
+  if (n_ghost) *n_ghost = ctx->h_counters->nghost;
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_neigh_export(nnpe_ctx* ctx, int which, int* ai, int* aj, int* code, int* sp, long long cap,
+                                 long long* n_out) {
+  if (!ctx || !ai || !aj || !code || !sp || (which != 0 && which != 1)) return NNPE_ERR_ARG;
+  if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "no list");
+  if (which == 1 && !ctx->cap_short) return ctx_fail(ctx, NNPE_ERR_STATE, "no short list");
+  const int N = ctx->natom;
+  std::vector<int> h(N);
+  NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int* dcnt = which ? ctx->cnt_short : ctx->cnt_long;
+  NNPE_CHECK_HIP(ctx, hipMemcpy(h.data(), dcnt, sizeof(int) * N, hipMemcpyDeviceToHost));
+  std::vector<long long> offs(N);
+  long long tot = 0;
+  for (int i = 0; i < N; ++i) { offs[i] = tot; tot += h[i]; }
+  if (n_out) *n_out = tot;
+  if (tot > cap) return ctx_fail(ctx, NNPE_ERR_ARG, "export capacity too small");
+  long long* doffs = nullptr;
+  int *dai = nullptr, *daj = nullptr, *dc = nullptr, *ds = nullptr;
+  NNPE_CHECK_HIP(ctx, dev_alloc(doffs, N));
+  NNPE_CHECK_HIP(ctx, dev_alloc(dai, tot)); NNPE_CHECK_HIP(ctx, dev_alloc(daj, tot));
+  NNPE_CHECK_HIP(ctx, dev_alloc(dc, tot)); NNPE_CHECK_HIP(ctx, dev_alloc(ds, tot));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(doffs, offs.data(), sizeof(long long) * N, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_export, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->stride,
+                     which ? ctx->nbr_short : ctx->nbr_long, dcnt, doffs, ctx->info, dai, daj, dc, ds);
+  NNPE_CHECK_LAUNCH(ctx);
+  NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(ai, dai, sizeof(int) * tot, hipMemcpyDeviceToHost));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(aj, daj, sizeof(int) * tot, hipMemcpyDeviceToHost));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(code, dc, sizeof(int) * tot, hipMemcpyDeviceToHost));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(sp, ds, sizeof(int) * tot, hipMemcpyDeviceToHost));
+  dev_free(doffs); dev_free(dai); dev_free(daj); dev_free(dc); dev_free(ds);
+  return NNPE_OK;
+}

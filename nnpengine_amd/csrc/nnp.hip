@@ -1,0 +1,644 @@
+// nnp.hip -- NNP environment descriptors and NNP forces.
+//
+// Two descriptor forms share the MLP (mlp.hip) through the per-atom arrays G [N][D] (input of
+// the network) and dG [N][D] (dE/dG, its output), both indexed by row:
+//
+// (1) NNPOTENTIAL_TBL, the reference's form (src/pair_nnp.cpp:433-793): 20 radial descriptors
+//     per atom, G_i[k] = sum_j lerp(tbl_desc[pair][20*special + k]); the force of a pair is
+//     fpair = lerp(tbl_f_pair[pair][special]) + sum_k (dEdG_i[k] + dEdG_j[k]) * lerp(tbl_f_desc)
+//     with the reference's arithmetic.  Tables are re-laid [pair][special][bin][k] so the 20
+//     values a lerp needs are two contiguous 80-byte runs.
+//
+// (2) Symmetry functions, radial + separable angular (the benchmark workloads):
+//       G^R_{i,a,n}   = sum_{j in a} exp(-eta_r (r_ij - mu_n)^2) fc(r_ij)
+//       G^A_{i,ab,m,zeta,lambda} = 2^(1-zeta) sum_{j in a, k in b, j != k}
+//                                   (1 + lambda cos t_jik)^zeta w_m(r_ij) w_m(r_ik)
+//     with w_m(r) = exp(-eta_m r^2) fc(r), fc(r) = (cos(pi r/rc) + 1)/2, zeta in {1,2},
+//     lambda = +-1.  The angular sums are evaluated from per-element moment sums
+//     A0 = sum w, A1 = sum w r^, A2 = sum w r^ r^, S = sum w^2 (one pass over the neighbours,
+//     O(N * neighbours) instead of O(N * neighbours^2)).  Forces use the adjoint of those
+//     moments: every atom turns dE/dG into dE/d(moments) once, then each pair's force is a
+//     function of the pair vector and the two atoms' adjoints, gathered over the full list, so
+//     each atom's force has one writer.
+#include <cmath>
+
+#include "internal.h"
+
+namespace {
+
+constexpr int kND = 20;        // NNPOTENTIAL_TBL::n_descriptor (src/pair_nnp.h:65)
+constexpr int kNRad = 16;
+constexpr int kNEta = 4;
+constexpr int kNMom = 11;      // A0, A1[3], A2[xx,yy,zz,xy,xz,yz], S
+constexpr float kPi = 3.14159265358979f;
+
+__device__ __forceinline__ int pair_index(int a, int b, int ntype) {
+  int i = min(a, b), j = max(a, b);
+  return i * ntype - (i * (i - 1)) / 2 + (j - i);
+}
+
+// ---------------------------------------------------------------------------------------------
+// (1) tabulated reference form
+
+// NNPOTENTIAL_TBL::computeDescriptor over a full list: each atom sums its own neighbours
+__global__ void __launch_bounds__(256) k_tbl_desc(int n, int stride, const uint32_t* __restrict__ nbr,
+                                                  const int* __restrict__ cnt, const float4* __restrict__ xq,
+                                                  const int* __restrict__ typ, int ntype, float bl, float m_bl,
+                                                  float cutsq, float rdr, int nb1, const float* __restrict__ tdesc,
+                                                  float* __restrict__ G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 xi = xq[i];
+  const int ti = typ[i];
+  float acc[kND];
+#pragma unroll
+  for (int k = 0; k < kND; ++k) acc[k] = 0.0f;
+  const int m = cnt[i];
+  for (int e = 0; e < m; ++e) {
+    const uint32_t ent = nbr[(size_t)e * stride + i];
+    const int j = NNPE_ENTRY_J(ent);
+    const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
+    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const float rsq = dx * dx + dy * dy + dz * dz;
+    if (!(rsq < cutsq)) continue;
+    const float r = sqrtf(rsq);
+    float ipw1 = r * rdr - 0.5f;
+    const int t1 = int(ipw1);
+    const float ipw2 = ipw1 - float(t1);
+    ipw1 = 1.0f - ipw2;
+    const int p = pair_index(ti, typ[j], ntype);
+    const float* t = tdesc + ((size_t)(p * 4 + NNPE_ENTRY_SP(ent)) * nb1 + t1) * kND;
+#pragma unroll
+    for (int k = 0; k < kND; ++k) acc[k] += t[k] * ipw1 + t[kND + k] * ipw2;
+  }
+#pragma unroll
+  for (int k = 0; k < kND; ++k) G[(size_t)i * kND + k] = acc[k];
+}
+
+// NNPOTENTIAL_TBL::compute<ev> over a full list (each directed entry: half energy / virial)
+template <bool EV>
+__global__ void __launch_bounds__(256) k_tbl_force(int n, int stride, const uint32_t* __restrict__ nbr,
+                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
+                                                   const int* __restrict__ typ, int ntype, float bl, float m_bl,
+                                                   float cutsq, float rdr, int nb1, const float* __restrict__ tfdesc,
+                                                   const float* __restrict__ tpair, const float* __restrict__ tfpair,
+                                                   const float* __restrict__ dG, float4* __restrict__ f, double* pe,
+                                                   double* pw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ea = 0, wa = 0;
+  if (i < n) {
+    const float4 xi = xq[i];
+    const int ti = typ[i];
+    float gi[kND];
+#pragma unroll
+    for (int k = 0; k < kND; ++k) gi[k] = dG[(size_t)i * kND + k];
+    float fx = 0, fy = 0, fz = 0, e = 0, w = 0;
+    const int m = cnt[i];
+    for (int q = 0; q < m; ++q) {
+      const uint32_t ent = nbr[(size_t)q * stride + i];
+      const int j = NNPE_ENTRY_J(ent), sp = NNPE_ENTRY_SP(ent);
+      const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
+      float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const float rsq = dx * dx + dy * dy + dz * dz;
+      if (!(rsq < cutsq)) continue;
+      const float r = sqrtf(rsq);
+      float ipw1 = r * rdr - 0.5f;
+      const int t1 = int(ipw1), t2 = t1 + 1;
+      const float ipw2 = ipw1 - float(t1);
+      ipw1 = 1.0f - ipw2;
+      const int p = pair_index(ti, typ[j], ntype);
+      const float* fp = tfpair + (size_t)(p * 4 + sp) * nb1;
+      float fpair = fp[t1] * ipw1 + fp[t2] * ipw2;
+      const float* t = tfdesc + ((size_t)(p * 4 + sp) * nb1 + t1) * kND;
+      const float* gj = dG + (size_t)j * kND;
+#pragma unroll
+      for (int k = 0; k < kND; ++k) fpair += (gi[k] + gj[k]) * (t[k] * ipw1 + t[kND + k] * ipw2);
+      fx += (dx *= fpair); fy += (dy *= fpair); fz += (dz *= fpair);
+      if (EV) {
+        const float* pu = tpair + (size_t)(p * 4 + sp) * nb1;
+        e += 0.5f * (pu[t1] * ipw1 + pu[t2] * ipw2);
+        w += 0.5f * fpair * rsq;
+      }
+    }
+    float4 fo = f[i];
+    fo.x += fx; fo.y += fy; fo.z += fz;
+    f[i] = fo;
+    ea = e; wa = w;
+  }
+  if (EV) block_sum2_store<256>(ea, wa, pe, pw);
+}
+
+// ---------------------------------------------------------------------------------------------
+// (2) symmetry functions
+
+struct SfParams {
+  float rc, rcsq, eta_r;
+  float mu[kNRad];
+  float eta[kNEta];
+};
+
+__device__ __forceinline__ void sf_cut(float r, float rc, float& fc, float& dfc) {
+  float s, c;
+  sincosf(kPi * r / rc, &s, &c);
+  fc = 0.5f * (c + 1.0f);
+  dfc = -0.5f * kPi / rc * s;
+}
+
+// moments per atom: [NT][kNEta][kNMom] then radial [NT][kNRad]
+template <int NT>
+__global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, const uint32_t* __restrict__ nbr,
+                                                 const int* __restrict__ cnt, const float4* __restrict__ xq,
+                                                 const int* __restrict__ typ, float bl, float m_bl, SfParams P,
+                                                 float* __restrict__ Mout, float* __restrict__ G) {
+  constexpr int NP = NT * (NT + 1) / 2;
+  constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
+  constexpr int D = NT * kNRad + NP * kNEta * 4;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 xi = xq[i];
+  float A[NT][kNEta][kNMom];
+  float R[NT][kNRad];
+#pragma unroll
+  for (int a = 0; a < NT; ++a) {
+#pragma unroll
+    for (int m = 0; m < kNEta; ++m)
+#pragma unroll
+      for (int c = 0; c < kNMom; ++c) A[a][m][c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kNRad; ++k) R[a][k] = 0.0f;
+  }
+  const int cn = cnt[i];
+  for (int q = 0; q < cn; ++q) {
+    const uint32_t ent = nbr[(size_t)q * stride + i];
+    const int j = NNPE_ENTRY_J(ent);
+    const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
+    const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
+    const float rsq = ux * ux + uy * uy + uz * uz;
+    if (!(rsq < P.rcsq)) continue;
+    const float r = sqrtf(rsq), ir = 1.0f / r;
+    const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
+    float fc, dfc;
+    sf_cut(r, P.rc, fc, dfc);
+    const int tj = typ[j];
+    float rad[kNRad];
+#pragma unroll
+    for (int k = 0; k < kNRad; ++k) {
+      const float d = r - P.mu[k];
+      rad[k] = __expf(-P.eta_r * d * d) * fc;
+    }
+    float wv[kNEta];
+#pragma unroll
+    for (int m = 0; m < kNEta; ++m) wv[m] = __expf(-P.eta[m] * rsq) * fc;
+#pragma unroll
+    for (int a = 0; a < NT; ++a) {
+      if (tj == a) {
+#pragma unroll
+        for (int k = 0; k < kNRad; ++k) R[a][k] += rad[k];
+#pragma unroll
+        for (int m = 0; m < kNEta; ++m) {
+          const float w = wv[m], wx = w * hx, wy = w * hy, wz = w * hz;
+          A[a][m][0] += w;
+          A[a][m][1] += wx; A[a][m][2] += wy; A[a][m][3] += wz;
+          A[a][m][4] += wx * hx; A[a][m][5] += wy * hy; A[a][m][6] += wz * hz;
+          A[a][m][7] += wx * hy; A[a][m][8] += wx * hz; A[a][m][9] += wy * hz;
+          A[a][m][10] += w * w;
+        }
+      }
+    }
+  }
+  float* mo = Mout + (size_t)i * NMT;
+#pragma unroll
+  for (int a = 0; a < NT; ++a)
+#pragma unroll
+    for (int m = 0; m < kNEta; ++m)
+#pragma unroll
+      for (int c = 0; c < kNMom; ++c) mo[(a * kNEta + m) * kNMom + c] = A[a][m][c];
+#pragma unroll
+  for (int a = 0; a < NT; ++a)
+#pragma unroll
+    for (int k = 0; k < kNRad; ++k) mo[NT * kNEta * kNMom + a * kNRad + k] = R[a][k];
+  float* g = G + (size_t)i * D;
+#pragma unroll
+  for (int a = 0; a < NT; ++a)
+#pragma unroll
+    for (int k = 0; k < kNRad; ++k) g[a * kNRad + k] = R[a][k];
+  int pp = 0;
+#pragma unroll
+  for (int a = 0; a < NT; ++a)
+#pragma unroll
+    for (int b = a; b < NT; ++b, ++pp)
+#pragma unroll
+      for (int m = 0; m < kNEta; ++m) {
+        const float* X = A[a][m];
+        const float* Y = A[b][m];
+        float P0 = X[0] * Y[0], P1 = X[1] * Y[1] + X[2] * Y[2] + X[3] * Y[3];
+        float P2 = X[4] * Y[4] + X[5] * Y[5] + X[6] * Y[6] + 2.0f * (X[7] * Y[7] + X[8] * Y[8] + X[9] * Y[9]);
+        if (a == b) { P0 -= X[10]; P1 -= X[10]; P2 -= X[10]; }
+        float* o = g + NT * kNRad + (pp * kNEta + m) * 4;
+        o[0] = P0 + P1;
+        o[1] = P0 - P1;
+        o[2] = 0.5f * (P0 + 2.0f * P1 + P2);
+        o[3] = 0.5f * (P0 - 2.0f * P1 + P2);
+      }
+}
+
+// dE/d(moments) of each atom from dE/dG and its moments
+template <int NT>
+__global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restrict__ M, const float* __restrict__ dG,
+                                                    float* __restrict__ adj) {
+  constexpr int NP = NT * (NT + 1) / 2;
+  constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
+  constexpr int D = NT * kNRad + NP * kNEta * 4;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* mi = M + (size_t)i * NMT;
+  const float* g = dG + (size_t)i * D;
+  float* ad = adj + (size_t)i * NMT;
+  float dA[NT][kNEta][kNMom];
+#pragma unroll
+  for (int a = 0; a < NT; ++a)
+#pragma unroll
+    for (int m = 0; m < kNEta; ++m)
+#pragma unroll
+      for (int c = 0; c < kNMom; ++c) dA[a][m][c] = 0.0f;
+  int pp = 0;
+#pragma unroll
+  for (int a = 0; a < NT; ++a)
+#pragma unroll
+    for (int b = a; b < NT; ++b, ++pp)
+#pragma unroll
+      for (int m = 0; m < kNEta; ++m) {
+        const float* gg = g + NT * kNRad + (pp * kNEta + m) * 4;
+        const float u0 = gg[0] + gg[1] + 0.5f * (gg[2] + gg[3]);
+        const float u1 = gg[0] - gg[1] + gg[2] - gg[3];
+        const float u2 = 0.5f * (gg[2] + gg[3]);
+        const float* X = mi + (a * kNEta + m) * kNMom;
+        const float* Y = mi + (b * kNEta + m) * kNMom;
+        dA[a][m][0] += u0 * Y[0]; dA[b][m][0] += u0 * X[0];
+#pragma unroll
+        for (int c = 1; c <= 3; ++c) { dA[a][m][c] += u1 * Y[c]; dA[b][m][c] += u1 * X[c]; }
+#pragma unroll
+        for (int c = 4; c <= 6; ++c) { dA[a][m][c] += u2 * Y[c]; dA[b][m][c] += u2 * X[c]; }
+#pragma unroll
+        for (int c = 7; c <= 9; ++c) { dA[a][m][c] += 2.0f * u2 * Y[c]; dA[b][m][c] += 2.0f * u2 * X[c]; }
+        if (a == b) dA[a][m][10] -= u0 + u1 + u2;
+      }
+#pragma unroll
+  for (int a = 0; a < NT; ++a)
+#pragma unroll
+    for (int m = 0; m < kNEta; ++m)
+#pragma unroll
+      for (int c = 0; c < kNMom; ++c) ad[(a * kNEta + m) * kNMom + c] = dA[a][m][c];
+#pragma unroll
+  for (int k = 0; k < NT * kNRad; ++k) ad[NT * kNEta * kNMom + k] = g[k];
+}
+
+// dE_owner/du for one neighbour at unit vector h (u = r h) of the given element slice of an
+// adjoint row: the angular part (vector) and the radial part's scalar folded into `along`
+__device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const float* wv, const float* dwv, float ir,
+                                            float hx, float hy, float hz, float& along, float& gx, float& gy, float& gz) {
+#pragma unroll
+  for (int m = 0; m < kNEta; ++m) {
+    const float* e = d + m * kNMom;
+    const float w = wv[m], dw = dwv[m], wr = w * ir;
+    const float p1 = e[1] * hx + e[2] * hy + e[3] * hz;
+    const float mrx = e[4] * hx + 0.5f * (e[7] * hy + e[8] * hz);
+    const float mry = e[5] * hy + 0.5f * (e[7] * hx + e[9] * hz);
+    const float mrz = e[6] * hz + 0.5f * (e[8] * hx + e[9] * hy);
+    const float qq = mrx * hx + mry * hy + mrz * hz;
+    along += dw * (e[0] + 2.0f * w * e[10] + p1 + qq) - wr * (p1 + 2.0f * qq);
+    gx += wr * (e[1] + 2.0f * mrx);
+    gy += wr * (e[2] + 2.0f * mry);
+    gz += wr * (e[3] + 2.0f * mrz);
+  }
+}
+
+template <int NT, bool EV>
+__global__ void __launch_bounds__(256) k_sf_force(int n, int stride, const uint32_t* __restrict__ nbr,
+                                                  const int* __restrict__ cnt, const float4* __restrict__ xq,
+                                                  const int* __restrict__ typ, float bl, float m_bl, SfParams P,
+                                                  const float* __restrict__ adj, float4* __restrict__ f, double* pe,
+                                                  double* pw) {
+  constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
+  constexpr int RAD0 = NT * kNEta * kNMom;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double wacc = 0;
+  if (i < n) {
+    const float4 xi = xq[i];
+    const int ti = typ[i];
+    const float* ai = adj + (size_t)i * NMT;
+    float fx = 0, fy = 0, fz = 0, vir = 0;
+    const int cn = cnt[i];
+    for (int q = 0; q < cn; ++q) {
+      const uint32_t ent = nbr[(size_t)q * stride + i];
+      const int j = NNPE_ENTRY_J(ent);
+      const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
+      const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
+      const float rsq = ux * ux + uy * uy + uz * uz;
+      if (!(rsq < P.rcsq)) continue;
+      const float r = sqrtf(rsq), ir = 1.0f / r;
+      const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
+      float fc, dfc;
+      sf_cut(r, P.rc, fc, dfc);
+      const int tj = typ[j];
+      float wv[kNEta], dwv[kNEta];
+#pragma unroll
+      for (int m = 0; m < kNEta; ++m) {
+        const float ex = __expf(-P.eta[m] * rsq);
+        wv[m] = ex * fc;
+        dwv[m] = ex * (dfc - 2.0f * P.eta[m] * r * fc);
+      }
+      const float* aj = adj + (size_t)j * NMT;
+      // radial: sum_n g_n'(r) (dR_i[tj][n] + dR_j[ti][n])
+      float rad = 0.0f;
+      const float* ri = ai + RAD0 + tj * kNRad;
+      const float* rj = aj + RAD0 + ti * kNRad;
+#pragma unroll
+      for (int k = 0; k < kNRad; ++k) {
+        const float d = r - P.mu[k];
+        const float ex = __expf(-P.eta_r * d * d);
+        rad += (ri[k] + rj[k]) * ex * (dfc - 2.0f * P.eta_r * d * fc);
+      }
+      // own adjoint, direction +h
+      float al_i = 0, gix = 0, giy = 0, giz = 0;
+      sf_grad_ang(ai + tj * kNEta * kNMom, wv, dwv, ir, hx, hy, hz, al_i, gix, giy, giz);
+      // neighbour's adjoint, direction -h
+      float al_j = 0, gjx = 0, gjy = 0, gjz = 0;
+      sf_grad_ang(aj + ti * kNEta * kNMom, wv, dwv, ir, -hx, -hy, -hz, al_j, gjx, gjy, gjz);
+      // F_i += dE_i/du - dE_j/du' with u' = -u; radial pieces are along h for i and -h for j
+      const float gxi = gix + (al_i)*hx, gyi = giy + (al_i)*hy, gzi = giz + (al_i)*hz;
+      const float gxj = gjx - al_j * hx, gyj = gjy - al_j * hy, gzj = gjz - al_j * hz;
+      fx += gxi - gxj + rad * hx;
+      fy += gyi - gyj + rad * hy;
+      fz += gzi - gzj + rad * hz;
+      if (EV) {
+        // W_i = -(dE_i/du . u), own descriptor only (the radial share of E_i is dR_i part)
+        float radi = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kNRad; ++k) {
+          const float d = r - P.mu[k];
+          radi += ri[k] * __expf(-P.eta_r * d * d) * (dfc - 2.0f * P.eta_r * d * fc);
+        }
+        vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
+      }
+    }
+    float4 fo = f[i];
+    fo.x += fx; fo.y += fy; fo.z += fz;
+    f[i] = fo;
+    wacc = vir;
+  }
+  if (EV) block_sum2_store<256>(0.0, wacc, pe, pw);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// host side
+
+struct NnpState {
+  // tabulated form
+  bool tbl = false;
+  float tbl_cut = 0, tbl_cutsq = 0, tbl_rdr = 0;
+  int tbl_nbin = 0;
+  float *tdesc = nullptr, *tfdesc = nullptr, *tpair = nullptr, *tfpair = nullptr;
+  // symmetry functions
+  bool sf = false;
+  SfParams P{};
+  float* M = nullptr;    // moments [N][NMT]
+  float* adj = nullptr;  // [N][NMT]
+  int nmt = 0;
+};
+
+NnpState* nnp_state_of(nnpe_ctx* ctx) {
+  if (!ctx->nnp) ctx->nnp = new NnpState;
+  return ctx->nnp;
+}
+
+void nnp_state_free(nnpe_ctx* ctx) {
+  NnpState* S = ctx->nnp;
+  if (!S) return;
+  dev_free(S->tdesc); dev_free(S->tfdesc); dev_free(S->tpair); dev_free(S->tfpair);
+  dev_free(S->M); dev_free(S->adj);
+  delete S;
+  ctx->nnp = nullptr;
+}
+
+// descriptor / gradient buffers sized for the largest descriptor in use
+int nnp_desc_reserve(nnpe_ctx* ctx, int D) {
+  if (D > ctx->D_cap) {
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->G, size_t(ctx->natom) * D));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->dG, size_t(ctx->natom) * D));
+    NNPE_CHECK_HIP(ctx, hipMemset(ctx->dG, 0, sizeof(float) * size_t(ctx->natom) * D));
+    ctx->D_cap = D;
+    ctx->D_cur = 0;
+  }
+  if (!ctx->e_atom) NNPE_CHECK_HIP(ctx, dev_alloc(ctx->e_atom, ctx->natom));
+  return NNPE_OK;
+}
+
+static NnpState& nnp_state(nnpe_ctx* ctx) { return *nnp_state_of(ctx); }
+static float* nnp_G(nnpe_ctx* ctx) { return ctx->G; }
+static float* nnp_dG(nnpe_ctx* ctx) { return ctx->dG; }
+static void nnp_set_current_D(nnpe_ctx* ctx, int D) { ctx->D_cur = D; }
+static int nnp_current_D(nnpe_ctx* ctx) { return ctx->D_cur; }
+
+// NNPOTENTIAL_TBL ctor table construction (src/pair_nnp.cpp:521-590), FP32 like the reference
+extern "C" int nnpe_nnp_tbl_set(nnpe_ctx* ctx, float cutoff, int nbin, const float* coef_p, const float* coef_w,
+                                const float* coef_d, const float* coef_r) {
+  if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (!ctx || nbin <= 0 || !(cutoff > 0) || !coef_p || !coef_w || !coef_d || !coef_r)
+    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_nnp_tbl_set: bad argument");
+  if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
+  NnpState& S = nnp_state(ctx);
+  const int npair = ctx->npair, nb1 = nbin + 1, szd = kND * 4;
+  std::vector<float> tdesc(size_t(npair) * 4 * nb1 * kND, 0.0f), tfdesc(tdesc.size(), 0.0f);
+  std::vector<float> tpair(size_t(npair) * 4 * nb1, 0.0f), tfpair(tpair.size(), 0.0f);
+  const float dr = cutoff / float(nbin);
+  std::vector<float> cp(coef_p, coef_p + size_t(npair) * 24);
+  auto sq = [](float v) { return v * v; };
+  for (int p = 0; p < npair; ++p) {
+    for (int k = 0; k < 4; ++k) {
+      float* c = &cp[size_t(p) * 24 + k * 6];
+      float src2 = sq(c[1] / cutoff), src6 = src2 * src2 * src2;
+      float elim = -2 * c[5] / cutoff - c[0] * src6 * (-7 + src2 * (9 * c[2] + src2 * (11 * c[3] + src2 * 13 * c[4])));
+      float flim = (c[5] / cutoff + c[0] * src6 * (-6 + src2 * ((c[2] *= 8) + src2 * ((c[3] *= 10) + src2 * (c[4] *= 12))))) / cutoff;
+      c[1] *= c[1];
+      for (int i = 0; i < nbin; ++i) {
+        float rr = float(double(dr) * (i + 0.5));
+        float rsq = sq(rr);
+        float s2 = c[1] / rsq, s6 = s2 * s2 * s2;
+        tfpair[(size_t(p) * 4 + k) * nb1 + i] = (c[5] / rr + c[0] * s6 * (-6 + s2 * (c[2] + s2 * (c[3] + s2 * c[4])))) / rsq - flim / rr;
+        tpair[(size_t(p) * 4 + k) * nb1 + i] =
+            c[5] / rr + c[0] * s6 * (-1 + s2 * (c[2] * .125F + s2 * (c[3] * .1F + s2 * c[4] * 0.08333333333333333333F))) + flim * rr + elim;
+      }
+    }
+    for (int i = 0; i < nbin; ++i) {
+      float rr = float(double(dr) * (i + 0.5));
+      float cf2 = rr - cutoff, rrcsq_1 = 1 + sq(cf2);
+      float cff = 1 - 1 / rrcsq_1;
+      float cfr = (-2 * cff) / rr;
+      cf2 /= (-.5f * rr * sq(rrcsq_1));
+      for (int k = 0; k < szd; ++k) {
+        float rr0 = rr - coef_r[size_t(p) * szd + k];
+        float drr0 = coef_d[size_t(p) * szd + k] * rr0;
+        float wexp = std::exp(drr0 * rr0) * coef_w[size_t(p) * szd + k];
+        // [pair][special][bin][k]: special = k / 20, descriptor = k % 20
+        size_t o = ((size_t(p) * 4 + k / kND) * nb1 + i) * kND + (k % kND);
+        tfdesc[o] = wexp * (cf2 + drr0 * cfr);
+        tdesc[o] = wexp * cff;
+      }
+    }
+  }
+  auto up = [&](float*& d, const std::vector<float>& h) -> hipError_t {
+    hipError_t e = dev_alloc(d, h.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+  };
+  NNPE_CHECK_HIP(ctx, up(S.tdesc, tdesc)); NNPE_CHECK_HIP(ctx, up(S.tfdesc, tfdesc));
+  NNPE_CHECK_HIP(ctx, up(S.tpair, tpair)); NNPE_CHECK_HIP(ctx, up(S.tfpair, tfpair));
+  S.tbl = true;
+  S.tbl_cut = cutoff; S.tbl_cutsq = cutoff * cutoff; S.tbl_nbin = nbin;
+  S.tbl_rdr = float(nbin) / cutoff;
+  return nnp_desc_reserve(ctx, kND);
+}
+
+int nnp_tbl_desc_enqueue(nnpe_ctx* ctx) {
+  NnpState& S = nnp_state(ctx);
+  hipLaunchKernelGGL(k_tbl_desc, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride,
+                     ctx->nbr_long, ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq,
+                     S.tbl_rdr, S.tbl_nbin + 1, S.tdesc, nnp_G(ctx));
+  NNPE_CHECK_LAUNCH(ctx);
+  nnp_set_current_D(ctx, kND);
+  return NNPE_OK;
+}
+
+int nnp_tbl_force_enqueue(nnpe_ctx* ctx, bool ev) {
+  NnpState& S = nnp_state(ctx);
+  const int nblk = blocks_for(ctx->natom, 256);
+  double* pe = ev_part(ctx, NNPE_TERM_NNP_PAIR, 0);
+  double* pw = ev_part(ctx, NNPE_TERM_NNP_PAIR, 1);
+  if (ev)
+    hipLaunchKernelGGL(k_tbl_force<true>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, ctx->nbr_long,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq, S.tbl_rdr,
+                       S.tbl_nbin + 1, S.tfdesc, S.tpair, S.tfpair, nnp_dG(ctx), ctx->f, pe, pw);
+  else
+    hipLaunchKernelGGL(k_tbl_force<false>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, ctx->nbr_long,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq, S.tbl_rdr,
+                       S.tbl_nbin + 1, S.tfdesc, S.tpair, S.tfpair, nnp_dG(ctx), ctx->f, pe, pw);
+  NNPE_CHECK_LAUNCH(ctx);
+  if (ev) return ev_finish(ctx, NNPE_TERM_NNP_PAIR, nblk);
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_nnp_tbl_descriptor(nnpe_ctx* ctx) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!nnp_state(ctx).tbl || !ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "needs nnpe_nnp_tbl_set and a list");
+  if (nnp_state(ctx).tbl_cut > ctx->pair_cutoff + ctx->skin) return ctx_fail(ctx, NNPE_ERR_ARG, "NNP cutoff beyond the list");
+  return nnp_tbl_desc_enqueue(ctx);
+}
+
+extern "C" int nnpe_nnp_tbl_force(nnpe_ctx* ctx, int ev) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!nnp_state(ctx).tbl || !ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "needs nnpe_nnp_tbl_set and a list");
+  if (nnp_current_D(ctx) != kND) return ctx_fail(ctx, NNPE_ERR_STATE, "dE/dG is not of the tabulated form");
+  return nnp_tbl_force_enqueue(ctx, ev != 0);
+}
+
+extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, float eta_r, int neta, const float* eta) {
+  if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (!ctx || nrad != kNRad || neta != kNEta || !mu || !eta || !(rc > 0))
+    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_sf_set: this build has 16 radial centres and 4 angular exponents");
+  if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
+  if (ctx->ntype < 1 || ctx->ntype > 4) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "symmetry functions support 1-4 elements");
+  NnpState& S = nnp_state(ctx);
+  S.P.rc = rc; S.P.rcsq = rc * rc; S.P.eta_r = eta_r;
+  for (int k = 0; k < kNRad; ++k) S.P.mu[k] = mu[k];
+  for (int k = 0; k < kNEta; ++k) S.P.eta[k] = eta[k];
+  S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
+  NNPE_CHECK_HIP(ctx, dev_alloc(S.M, size_t(ctx->natom) * S.nmt));
+  NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->natom) * S.nmt));
+  S.sf = true;
+  return nnp_desc_reserve(ctx, sf_dim(ctx->ntype));
+}
+
+int sf_dim(int ntype) { return ntype * kNRad + ntype * (ntype + 1) / 2 * kNEta * 4; }
+
+int sf_desc_enqueue(nnpe_ctx* ctx) {
+  NnpState& S = nnp_state(ctx);
+  const int nblk = blocks_for(ctx->natom, 256);
+  const uint32_t* nbr = ctx->cap_short ? ctx->nbr_short : ctx->nbr_long;
+  const int* cnt = ctx->cap_short ? ctx->cnt_short : ctx->cnt_long;
+#define NNPE_SF_DESC(NT)                                                                                          \
+  hipLaunchKernelGGL(k_sf_desc<NT>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, nbr, cnt, ctx->xq, \
+                     ctx->typ, ctx->bl, ctx->m_bl, S.P, S.M, nnp_G(ctx))
+  switch (ctx->ntype) {
+    case 1: NNPE_SF_DESC(1); break;
+    case 2: NNPE_SF_DESC(2); break;
+    case 3: NNPE_SF_DESC(3); break;
+    default: NNPE_SF_DESC(4); break;
+  }
+#undef NNPE_SF_DESC
+  NNPE_CHECK_LAUNCH(ctx);
+  nnp_set_current_D(ctx, sf_dim(ctx->ntype));
+  return NNPE_OK;
+}
+
+int sf_adjoint_enqueue(nnpe_ctx* ctx) {
+  NnpState& S = nnp_state(ctx);
+  const int nblk = blocks_for(ctx->natom, 256);
+#define NNPE_SF_ADJ(NT) \
+  hipLaunchKernelGGL(k_sf_adjoint<NT>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, S.M, nnp_dG(ctx), S.adj)
+  switch (ctx->ntype) {
+    case 1: NNPE_SF_ADJ(1); break;
+    case 2: NNPE_SF_ADJ(2); break;
+    case 3: NNPE_SF_ADJ(3); break;
+    default: NNPE_SF_ADJ(4); break;
+  }
+#undef NNPE_SF_ADJ
+  NNPE_CHECK_LAUNCH(ctx);
+  return NNPE_OK;
+}
+
+// the pair-force kernel alone (uses the adjoints of the last sf_adjoint_enqueue)
+int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
+  NnpState& S = nnp_state(ctx);
+  const int nblk = blocks_for(ctx->natom, 256);
+  const uint32_t* nbr = ctx->cap_short ? ctx->nbr_short : ctx->nbr_long;
+  const int* cnt = ctx->cap_short ? ctx->cnt_short : ctx->cnt_long;
+  double* pe = ev_part(ctx, NNPE_TERM_SF, 0);
+  double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
+#define NNPE_SF_FORCE(NT, EV)                                                                                        \
+  hipLaunchKernelGGL((k_sf_force<NT, EV>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, nbr, cnt, \
+                     ctx->xq, ctx->typ, ctx->bl, ctx->m_bl, S.P, S.adj, ctx->f, pe, pw)
+  switch (ctx->ntype) {
+    case 1: if (ev) NNPE_SF_FORCE(1, true); else NNPE_SF_FORCE(1, false); break;
+    case 2: if (ev) NNPE_SF_FORCE(2, true); else NNPE_SF_FORCE(2, false); break;
+    case 3: if (ev) NNPE_SF_FORCE(3, true); else NNPE_SF_FORCE(3, false); break;
+    default: if (ev) NNPE_SF_FORCE(4, true); else NNPE_SF_FORCE(4, false); break;
+  }
+#undef NNPE_SF_FORCE
+  NNPE_CHECK_LAUNCH(ctx);
+  if (ev) return ev_finish(ctx, NNPE_TERM_SF, nblk);
+  return NNPE_OK;
+}
+
+int sf_force_enqueue(nnpe_ctx* ctx, bool ev) {
+  int rc = sf_adjoint_enqueue(ctx);
+  if (rc) return rc;
+  return sf_force_only_enqueue(ctx, ev);
+}
+
+extern "C" int nnpe_sf_descriptor(nnpe_ctx* ctx) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!nnp_state(ctx).sf || !ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "needs nnpe_sf_set and a list");
+  const float reach = ctx->cap_short ? ctx->short_cutoff + ctx->skin : ctx->pair_cutoff + ctx->skin;
+  if (nnp_state(ctx).P.rc > reach) return ctx_fail(ctx, NNPE_ERR_ARG, "symmetry-function cutoff beyond the list");
+  return sf_desc_enqueue(ctx);
+}
+
+extern "C" int nnpe_sf_force(nnpe_ctx* ctx, int ev) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!nnp_state(ctx).sf || !ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "needs nnpe_sf_set and a list");
+  if (nnp_current_D(ctx) != sf_dim(ctx->ntype)) return ctx_fail(ctx, NNPE_ERR_STATE, "dE/dG is not of the symmetry-function form");
+  return sf_force_enqueue(ctx, ev != 0);
+}

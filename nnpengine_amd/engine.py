@@ -1,0 +1,246 @@
+"""Python handle on one nnpe context (a system resident in HBM), calling the C ABI.
+
+The method names follow the reference classes they stand in for (ENVIRON, NEIGHLIST,
+LJTableCoulDSF, BOND/ANGLE/DIHEDRAL, VERLET, GR, SQ); every call goes to the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import NnpeError
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Engine:
+    def __init__(self, sysm, device=0):
+        self.L = _lib.load()
+        self.sys = sysm
+        h = C.c_void_p()
+        self._check(self.L.nnpe_create(C.byref(h), device), ctx=None)
+        self.h = h
+        self.natom = sysm.natom
+        self.npair = sysm.npair
+        offs, flat = sysm.special_flat()
+        k = [_f32(sysm.x), _f32(sysm.v), _f32(sysm.q), _f32(sysm.mass), _i32(sysm.typ), _i32(sysm.mol_id),
+             _i32(sysm.intra_id), _i32(sysm.mol_type), _i32([m.n_atom for m in sysm.moltypes]), _i32(offs), _i32(flat)]
+        self._check(self.L.nnpe_system_set(self.h, sysm.natom, sysm.ntype, C.c_float(sysm.bl), _p(k[0], C.c_float),
+                                           _p(k[1], C.c_float), _p(k[2], C.c_float), _p(k[3], C.c_float),
+                                           _p(k[4], C.c_int), _p(k[5], C.c_int), _p(k[6], C.c_int), sysm.nmol,
+                                           _p(k[7], C.c_int), len(sysm.moltypes), _p(k[8], C.c_int),
+                                           _p(k[9], C.c_int), _p(k[10], C.c_int)))
+        self.gr_nbin = 0
+        self.sq_nq = 0
+
+    def _check(self, rc, ctx=True):
+        if rc != 0:
+            msg = self.L.nnpe_last_error(self.h).decode() if ctx else ""
+            raise NnpeError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.nnpe_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self._check(self.L.nnpe_synchronize(self.h))
+
+    # ENVIRON state -----------------------------------------------------------------------
+    def get(self, what):
+        out = np.zeros((self.natom, 3), np.float64)
+        self._check(self.L.nnpe_state_get(self.h, {"x": 0, "v": 1, "f": 2, "f_prev": 3}[what], _p(out, C.c_double)))
+        return out
+
+    def set(self, what, arr):
+        a = np.ascontiguousarray(arr, np.float64)
+        self._check(self.L.nnpe_state_set(self.h, {"x": 0, "v": 1, "f": 2, "f_prev": 3}[what], _p(a, C.c_double)))
+
+    # NEIGHLIST ---------------------------------------------------------------------------
+    def neigh_init(self, pair_cutoff, skin, gr_cutoff, short_cutoff=0.0):
+        self._check(self.L.nnpe_neigh_init(self.h, C.c_float(pair_cutoff), C.c_float(skin), C.c_float(gr_cutoff),
+                                           C.c_float(short_cutoff)))
+
+    def build(self):
+        self._check(self.L.nnpe_neigh_build(self.h))
+
+    def isvalid(self):
+        v = C.c_int(0)
+        self._check(self.L.nnpe_neigh_isvalid(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def list_size(self, which=0):
+        n, g = C.c_longlong(0), C.c_int(0)
+        self._check(self.L.nnpe_neigh_size(self.h, which, C.byref(n), C.byref(g)))
+        return n.value, g.value
+
+    def list_entries(self, which=0):
+        n, _ = self.list_size(which)
+        arrs = [np.zeros(max(n, 1), np.int32) for _ in range(4)]
+        got = C.c_longlong(0)
+        self._check(self.L.nnpe_neigh_export(self.h, which, *[_p(a, C.c_int) for a in arrs], C.c_longlong(n), C.byref(got)))
+        return np.stack([a[:got.value] for a in arrs], 1)
+
+    # forces ------------------------------------------------------------------------------
+    def pair_table(self, cutoff, force_tbl, pot_tbl):
+        self._tbl = (_f32(force_tbl), _f32(pot_tbl))
+        self._check(self.L.nnpe_pair_table(self.h, C.c_float(cutoff), force_tbl.shape[1] - 1,
+                                           _p(self._tbl[0], C.c_float), _p(self._tbl[1], C.c_float)))
+
+    def pair_compute(self, ev=True):
+        self._check(self.L.nnpe_pair_compute(self.h, int(ev)))
+
+    def bonded_set_from_system(self):
+        s = self.sys
+        for kind, name, coef in ((0, "bond", s.bond_coef), (1, "angle", s.angle_coef), (2, "dihedral", s.dihedral_coef)):
+            terms = s.expand(name)
+            coef = np.zeros((0, 4), np.float32) if coef is None else _f32(coef)
+            if len(terms):
+                self._check(self.L.nnpe_bonded_set(self.h, kind, len(coef), _p(coef, C.c_float), len(terms),
+                                                   _p(_i32(terms), C.c_int)))
+
+    def bonded_compute(self, ev=True):
+        self._check(self.L.nnpe_bonded_compute(self.h, int(ev)))
+
+    def clear_force(self):
+        self._check(self.L.nnpe_clear_force(self.h))
+
+    def energy(self):
+        out = np.zeros(2 * _lib.NTERM, np.float64)
+        self._check(self.L.nnpe_energy_get(self.h, _p(out, C.c_double)))
+        return {t: (out[2 * k], out[2 * k + 1]) for k, t in enumerate(_lib.TERMS)}
+
+    def energy_clear(self):
+        self._check(self.L.nnpe_energy_clear(self.h))
+
+    # VERLET ------------------------------------------------------------------------------
+    def verlet_set(self, dt, fp64=False):
+        self._check(self.L.nnpe_verlet_set(self.h, C.c_double(dt), int(fp64)))
+
+    def update_x(self):
+        self._check(self.L.nnpe_verlet_update_x(self.h))
+
+    def update_v(self):
+        self._check(self.L.nnpe_verlet_update_v(self.h))
+
+    def compute_ke(self):
+        ke = C.c_double(0)
+        self._check(self.L.nnpe_verlet_compute_ke(self.h, C.byref(ke)))
+        return ke.value
+
+    def scale_v(self, s):
+        self._check(self.L.nnpe_verlet_scale_v(self.h, C.c_double(s)))
+
+    # GR / SQ -----------------------------------------------------------------------------
+    def gr_init(self, nbin_inner, r_max, r_max_inner):
+        nb = C.c_int(0)
+        self._check(self.L.nnpe_gr_init(self.h, nbin_inner, C.c_float(r_max), C.c_float(r_max_inner), C.byref(nb)))
+        self.gr_nbin = nb.value
+        return nb.value
+
+    def gr_tally(self, full=False):
+        self._check(self.L.nnpe_gr_tally(self.h, int(full)))
+
+    def gr_get(self):
+        counts = np.zeros((self.npair, self.gr_nbin), np.int64)
+        h = np.zeros((self.npair, self.gr_nbin), np.float64)
+        self._check(self.L.nnpe_gr_get(self.h, _p(counts, C.c_longlong), _p(h, C.c_double)))
+        return counts, h
+
+    def sq_init(self, nq, q_min, q_max):
+        self.sq_nq = nq
+        self._check(self.L.nnpe_sq_init(self.h, nq, C.c_double(q_min), C.c_double(q_max)))
+
+    def sq_compute(self):
+        out = np.zeros((self.npair, self.sq_nq), np.float64)
+        self._check(self.L.nnpe_sq_compute(self.h, _p(out, C.c_double)))
+        return out
+
+    # NNPOTENTIAL_TBL ---------------------------------------------------------------------
+    def nnp_tbl_set(self, cutoff, nbin, coef_p, coef_w, coef_d, coef_r):
+        self._nnp = [_f32(coef_p), _f32(coef_w), _f32(coef_d), _f32(coef_r)]
+        self._check(self.L.nnpe_nnp_tbl_set(self.h, C.c_float(cutoff), nbin, *[_p(a, C.c_float) for a in self._nnp]))
+
+    def nnp_tbl_descriptor(self):
+        self._check(self.L.nnpe_nnp_tbl_descriptor(self.h))
+
+    def nnp_tbl_force(self, ev=True):
+        self._check(self.L.nnpe_nnp_tbl_force(self.h, int(ev)))
+
+    # symmetry functions ------------------------------------------------------------------
+    def sf_set(self, rc, mu, eta_r, eta_a):
+        self._sf = (_f32(mu), _f32(eta_a))
+        self._check(self.L.nnpe_sf_set(self.h, C.c_float(rc), len(mu), _p(self._sf[0], C.c_float), C.c_float(eta_r),
+                                       len(eta_a), _p(self._sf[1], C.c_float)))
+
+    def sf_descriptor(self):
+        self._check(self.L.nnpe_sf_descriptor(self.h))
+
+    def sf_force(self, ev=True):
+        self._check(self.L.nnpe_sf_force(self.h, int(ev)))
+
+    # TFCALL buffers / MLP ----------------------------------------------------------------
+    def nnp_get(self, what):
+        D = C.c_int(0)
+        k = {"G": 0, "dG": 1, "e_atom": 2}[what]
+        self._check(self.L.nnpe_nnp_get(self.h, k, None, C.byref(D)))
+        out = np.zeros((self.natom, D.value), np.float32)
+        self._check(self.L.nnpe_nnp_get(self.h, k, _p(out, C.c_float), C.byref(D)))
+        return out[:, 0] if what == "e_atom" else out
+
+    def nnp_set_dedg(self, dG):
+        d = _f32(dG)
+        self._check(self.L.nnpe_nnp_set_dedg(self.h, _p(d, C.c_float), d.shape[1]))
+
+    def mlp_set(self, widths, params, out_scale=1.0, precision="fp32", in_absmax=0.0):
+        prec = {"fp32": 0, "bf16": 1, "fp16": 2}[precision]
+        self._mlp = (_i32(widths), _f32(params))
+        self._check(self.L.nnpe_mlp_set(self.h, len(widths) - 1, _p(self._mlp[0], C.c_int), _p(self._mlp[1], C.c_float),
+                                        C.c_float(out_scale), prec, C.c_float(in_absmax)))
+
+    def mlp_eval(self, ev=True):
+        self._check(self.L.nnpe_mlp_eval(self.h, int(ev)))
+
+    # MD loop -----------------------------------------------------------------------------
+    def md_run(self, nsteps, dt=0.5, fp64=False, rebuild_every=10, pair=True, bonded=True, nnp=0, gr_every=0,
+               gr_long_every=0, sq=False, log_every=0, use_graph=True):
+        p = _lib.MdParams(dt, int(fp64), rebuild_every, int(pair), int(bonded), int(nnp), gr_every, gr_long_every,
+                          int(sq), log_every, int(use_graph))
+        cap = nsteps // log_every + 2 if log_every > 0 else 1
+        log = np.zeros((cap, 2 * _lib.NTERM), np.float64)
+        n = C.c_int(0)
+        self._check(self.L.nnpe_md_run(self.h, C.byref(p), nsteps, _p(log, C.c_double), cap, C.byref(n)))
+        return log[:n.value]
+
+    def md_stats(self):
+        out = np.zeros(8, np.int64)
+        self._check(self.L.nnpe_md_stats(self.h, _p(out, C.c_longlong)))
+        keys = ("steps", "nbuild", "skin_violations", "max_long", "max_short", "nghost", "cap_long", "cap_short")
+        return dict(zip(keys, out.tolist()))
+
+    def md_reset_step(self):
+        self._check(self.L.nnpe_md_reset_step(self.h))
+
+    def profile(self, stage, reps=20):
+        """Average ms of one launch of a stage (HIP events on the engine's stream)."""
+        ms = C.c_double(0)
+        self._check(self.L.nnpe_profile_stage(self.h, _lib.STAGES[stage], reps, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,82 @@
+"""NNP model parameters: symmetry-function settings, seeded per-element MLP weights, the
+descriptor standardisation folded into the first layer, and the reference's NNPOTENTIAL_TBL
+coefficient arrays (the layout its weight file holds, src/pair_nnp.cpp:505-517)."""
+from __future__ import annotations
+
+import numpy as np
+
+NRAD = 16
+NETA = 4
+
+
+def sf_params(rc=6.0):
+    """16 Gaussian radial centres from 0.8 A to rc - 0.2 A with width equal to their spacing, and
+    4 angular exponents spanning short to long range."""
+    mu = np.linspace(0.8, rc - 0.2, NRAD).astype(np.float32)
+    sp = float(mu[1] - mu[0])
+    eta_r = np.float32(0.5 / sp ** 2)
+    eta_a = np.array([0.02, 0.06, 0.15, 0.4], np.float32)
+    return dict(rc=float(rc), mu=mu, eta_r=float(eta_r), eta_a=eta_a)
+
+
+def sf_dim(ntype):
+    return NRAD * ntype + NETA * 4 * ntype * (ntype + 1) // 2
+
+
+def mlp_init(widths, ntype, seed):
+    """Weights N(0, 1/fan_in), zero biases; flat float32 array, per element then per layer
+    W [in][out] row-major followed by b [out]."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for _ in range(ntype):
+        for l in range(len(widths) - 1):
+            fi, fo = widths[l], widths[l + 1]
+            parts.append(rng.standard_normal((fi, fo)) / np.sqrt(fi))
+            parts.append(np.zeros(fo))
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
+def fold_standardisation(params, widths, ntype, mean, std):
+    """Network input x = (G - mean) / std folded into layer 1: W1' = W1 / std, b1' = b1 - (mean/std) W1.
+    mean, std: [ntype, D]."""
+    out = params.astype(np.float64).copy()
+    o = 0
+    for t in range(ntype):
+        for l in range(len(widths) - 1):
+            fi, fo = widths[l], widths[l + 1]
+            W = out[o:o + fi * fo].reshape(fi, fo)
+            b = out[o + fi * fo:o + fi * fo + fo]
+            if l == 0:
+                s = np.where(std[t] > 1e-6, std[t], 1.0)
+                b -= (mean[t] / s) @ W
+                W /= s[:, None]
+            o += fi * fo + fo
+    return out.astype(np.float32)
+
+
+def standardisation(G, typ, ntype):
+    mean = np.zeros((ntype, G.shape[1]))
+    std = np.ones((ntype, G.shape[1]))
+    for t in range(ntype):
+        g = G[typ == t].astype(np.float64)
+        if len(g):
+            mean[t] = g.mean(0)
+            std[t] = g.std(0)
+    return mean, std
+
+
+def tbl_coefficients(npair, seed, cutoff):
+    """Seeded NNPOTENTIAL_TBL coefficients in the weight-file layout: coef_p [npair][4][6]
+    (eps, sigma, c8, c10, c12, qij for each special class) and coef_w/d/r [npair][80]."""
+    rng = np.random.default_rng(seed)
+    cp = np.zeros((npair, 4, 6), np.float32)
+    for p in range(npair):
+        for s in range(4):
+            c = rng.standard_normal(3)
+            c /= np.linalg.norm(c)
+            cp[p, s] = [0.1 + 0.1 * rng.random(), 2.8 + 0.6 * rng.random(), abs(c[0]) * 0.5, abs(c[1]) * 0.2,
+                        abs(c[2]) * 0.1, 0.2 * rng.standard_normal()]
+    cw = (0.2 + 0.8 * rng.random((npair, 80))).astype(np.float32)
+    cd = (-(0.5 + 3.0 * rng.random((npair, 80)))).astype(np.float32)
+    cr = (0.8 + (cutoff - 1.0) * rng.random((npair, 80))).astype(np.float32)
+    return cp, cw, cd, cr

@@ -152,6 +152,35 @@ def random_rotations(rng, n):
     return qm
 
 
+def _place_rigid(rng, centres, tmpl, bl, sites, n_side, min_dist=1.7, tries=400):
+    """Place one rigid template per lattice site at a uniformly random orientation, re-drawing an
+    orientation (up to `tries` times) while any of its atoms lies within `min_dist` of an atom of
+    an already placed molecule on a neighbouring site (minimum image).  Returns [n, k, 3]."""
+    n, k = centres.shape[0], tmpl.shape[0]
+    x = np.zeros((n, k, 3))
+    occ = -np.ones((n_side,) * 3, np.int64)
+    for m, s in enumerate(sites):
+        occ[tuple(s)] = m
+    offs = np.array(list(itertools.product((-1, 0, 1), repeat=3)))
+    placed = np.zeros(n, bool)
+    for m in range(n):
+        nb = occ[tuple(((sites[m] + offs) % n_side).T)]
+        nb = nb[(nb >= 0) & placed[np.maximum(nb, 0)] & (nb != m)]
+        other = x[nb].reshape(-1, 3)
+        for t in range(tries):
+            rot = random_rotations(rng, 1)[0]
+            cand = centres[m] + tmpl @ rot.T
+            if len(other) == 0:
+                break
+            d = cand[:, None, :] - other[None, :, :]
+            d -= bl * np.round(d / bl)
+            if np.sqrt((d * d).sum(-1)).min() >= min_dist:
+                break
+        x[m] = cand
+        placed[m] = True
+    return x
+
+
 def maxwell_boltzmann(rng, mass, temperature):
     """v ~ N(0, sqrt(kT/m)) per component, in A/fs, as ``ENVIRON::initV`` draws them
     (src/environ.cpp:190-205); the net momentum is then removed."""
@@ -193,11 +222,7 @@ def water_box(nmol: int, bl: float, seed: int, temperature: float = 300.0, r0: f
     th = math.radians(theta0_deg)
     h1 = np.array([math.sin(th / 2), 0.0, math.cos(th / 2)]) * r0
     h2 = np.array([-math.sin(th / 2), 0.0, math.cos(th / 2)]) * r0
-    rot = random_rotations(rng, nmol)
-    x = np.empty((nmol, 3, 3))
-    x[:, 0] = o
-    x[:, 1] = o + rot @ h1
-    x[:, 2] = o + rot @ h2
+    x = _place_rigid(rng, o, np.stack([np.zeros(3), h1, h2]), bl, sites.astype(np.int64), n_side)
     x = x.reshape(-1, 3).astype(np.float32)
     n = 3 * nmol
     typ = np.tile(np.array([0, 1, 1], np.int32), nmol)
@@ -248,8 +273,8 @@ def butane_box(nmol: int, bl: float, seed: int, temperature: float = 300.0) -> S
     dx, dy = b * math.sin(th / 2), b * math.cos(th / 2)
     tmpl = np.array([[-1.5 * dx, 0, 0], [-0.5 * dx, dy, 0], [0.5 * dx, 0, 0], [1.5 * dx, dy, 0]])
     tmpl -= tmpl.mean(0)
-    rot = random_rotations(rng, nmol)
-    x = (c[:, None, :] + np.einsum("nij,aj->nai", rot, tmpl)).reshape(-1, 3).astype(np.float32)
+    x = _place_rigid(rng, c, tmpl, bl, sites.astype(np.int64), n_side, min_dist=3.2)
+    x = x.reshape(-1, 3).astype(np.float32)
     typ = np.tile(np.array([0, 1, 1, 0], np.int32), nmol)
     mass = np.tile(np.array([15.035, 14.027, 14.027, 15.035], np.float32), nmol)
     q = np.zeros(4 * nmol, np.float32)

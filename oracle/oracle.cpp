@@ -289,6 +289,7 @@ struct Eng {
 
   // NEIGHLIST::isvalid (src/neighlist.cpp:178-187)
   bool isvalid() const {
+    if (x_prev.size() < size_t(natom) * 3) return false;  // no build yet (src/neighlist.cpp:29)
     for (int i = 0; i < natom; ++i)
       if (sqr(x[3 * i] - x_prev[3 * i]) + sqr(x[3 * i + 1] - x_prev[3 * i + 1]) + sqr(x[3 * i + 2] - x_prev[3 * i + 2]) > dx2) return false;
     return true;
@@ -524,16 +525,21 @@ struct Eng {
   std::vector<double> sq;
   void sq_compute(int nq, double q_min, double q_max) {
     sq.assign(size_t(npair) * nq, 0);
-    double dq = (q_max - q_min) / nq, rho4pidr = 4 * 3.141592653589793 * natom / cub(double(bl)) * double(gr_dr);
+    const double rho = natom / cub(double(bl));
     for (int p = 0; p < npair; ++p)
-      for (int iq = 0; iq < nq; ++iq) {
-        double qi = q_min + (iq + 0.5) * dq, acc = 0;
-        for (int ir = 0; ir < gr_nbin; ++ir) {
-          double r = (ir + 0.5) * double(gr_dr);
-          acc += std::sin(qi * r) * rho4pidr * r / qi * gr_h[size_t(p) * gr_nbin + ir];
-        }
-        sq[size_t(p) * nq + iq] = acc;
+      sq_transform(gr_nbin, double(gr_dr), rho, &gr_h[size_t(p) * gr_nbin], nq, q_min, q_max, &sq[size_t(p) * nq]);
+  }
+  // S(q_i) = sum_r 4 pi rho dr r sin(q_i r)/q_i h(r), r = (k+1/2) dr (src/grsq.cpp:270-279, :305-320)
+  static void sq_transform(int nbin, double dr, double rho, const double* h, int nq, double q_min, double q_max, double* out) {
+    const double dq = (q_max - q_min) / nq, rho4pidr = 4 * 3.141592653589793 * rho * dr;
+    for (int iq = 0; iq < nq; ++iq) {
+      double qi = q_min + (iq + 0.5) * dq, acc = 0;
+      for (int ir = 0; ir < nbin; ++ir) {
+        double r = (ir + 0.5) * dr;
+        acc += std::sin(qi * r) * rho4pidr * r / qi * h[ir];
       }
+      out[iq] = acc;
+    }
   }
 
   // ---- NNPOTENTIAL_TBL (src/pair_nnp.cpp:433-793) -------------------------------------------
@@ -1001,6 +1007,11 @@ int orc_gr_get(OrcHandle* h, long long* counts, double* hr) {
 int orc_sq(OrcHandle* h, int nq, double q_min, double q_max, double* out) {
   auto run = [&](auto* e) { e->gr_reduce(); e->sq_compute(nq, q_min, q_max); std::copy(e->sq.begin(), e->sq.end(), out); };
   if (h->prec) run(static_cast<Eng<double>*>(h->e)); else run(static_cast<Eng<float>*>(h->e));
+  return 0;
+}
+
+int orc_sq_transform(int nbin, double dr, double rho, const double* hr, int nq, double q_min, double q_max, double* out) {
+  Eng<double>::sq_transform(nbin, dr, rho, hr, nq, q_min, q_max, out);
   return 0;
 }
 

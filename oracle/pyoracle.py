@@ -256,6 +256,15 @@ class Oracle:
         return w.value
 
 
+def sq_transform(h, dr, rho, nq, q_min, q_max):
+    """The S(q) sum of SQ::compute on an arbitrary h(r) sampled at bin centres."""
+    h = _f64(h)
+    out = np.zeros(nq, np.float64)
+    lib().orc_sq_transform(len(h), C.c_double(dr), C.c_double(rho), _p(h, C.c_double), nq, C.c_double(q_min),
+                           C.c_double(q_max), _p(out, C.c_double))
+    return out
+
+
 def canonical(entries):
     """Directed list entries (i, j, code, special) -> sorted unique int64 keys.  Real-real entries
     (code 13) are expanded to both directions so a half list and a full list with the same

@@ -1,0 +1,179 @@
+"""GPU parity of the NNP path: symmetry-function descriptors and forces, the MFMA MLP in FP32 /
+BF16 / FP16, the reference's tabulated descriptor form, and the graph-captured MD loop.
+Tolerances (relative RMS vs the FP64 oracle): FP32 forms 1e-5; BF16 NNP forces 1e-2; FP16 5e-3."""
+import numpy as np
+import pytest
+
+from nnpengine_amd import model as M
+from nnpengine_amd import system as S
+from oracle import pyoracle
+from tests.conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+RC_PAIR, SKIN, RC_NNP = 9.0, 1.0, 6.0
+
+
+def _setup(sysm, widths, seed=11, out_scale=0.5):
+    """Engine + FP64 oracle on the same system with the same (standardised) network."""
+    from nnpengine_amd.engine import Engine
+    sfp = M.sf_params(RC_NNP)
+    o = pyoracle.Oracle(sysm, 1)
+    o.neigh_init(RC_PAIR, SKIN, RC_PAIR)
+    o.rebuild(True)
+    D = o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    G64 = o.sf_descriptor()
+    mean, std = M.standardisation(G64, sysm.typ, sysm.ntype)
+    par = M.fold_standardisation(M.mlp_init(widths, sysm.ntype, seed), widths, sysm.ntype, mean, std)
+    o.mlp_set(widths, par, out_scale)
+    e = Engine(sysm)
+    e.neigh_init(RC_PAIR, SKIN, RC_PAIR, RC_NNP)
+    e.build()
+    e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    assert D == M.sf_dim(sysm.ntype)
+    return e, o, par, G64, float(np.abs(G64).max())
+
+
+@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3)])
+def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
+    widths = [80, 64, 64, 1]
+    e, o, par, G64, gmax = _setup(water_c0, widths)
+    e.sf_descriptor()
+    G = e.nnp_get("G").astype(np.float64)
+    print(f"[mlp] prec={prec} descriptor rel-rms vs fp64 = {rel_rms(G, G64):.2e}", flush=True)
+    assert rel_rms(G, G64) < 1e-5
+    e.mlp_set(widths, par, 0.5, prec, gmax)
+    e.energy_clear()
+    e.mlp_eval(True)
+    ea = e.nnp_get("e_atom").astype(np.float64)
+    dG = e.nnp_get("dG").astype(np.float64)
+    et, ea64, dG64 = o.mlp_eval(G)
+    print(f"[mlp] prec={prec} E rel-rms {rel_rms(ea, ea64):.2e} dG rel-rms {rel_rms(dG, dG64):.2e}")
+    assert rel_rms(ea, ea64) < tol_e
+    assert rel_rms(dG, dG64) < tol_g
+    en = e.energy()
+    assert abs(en["nn"][0] - et) < max(tol_e, 1e-6) * (np.abs(ea64).sum() + 1e-9) * 10
+
+
+def test_sf_forces_fp32(water_c0):
+    widths = [80, 64, 64, 1]
+    e, o, par, G64, gmax = _setup(water_c0, widths)
+    e.mlp_set(widths, par, 0.5, "fp32", gmax)
+    e.sf_descriptor()
+    e.clear_force()
+    e.energy_clear()
+    e.mlp_eval(True)
+    e.sf_force(True)
+    fg = e.get("f")
+    en = e.energy()
+    et, _, dG64 = o.mlp_eval(G64)
+    o.clear_f()
+    w64 = o.sf_force(dG64)
+    f64 = o.get("f")
+    print(f"[sf] force rel-rms {rel_rms(fg, f64):.2e} virial {en['sf'][1]:.6f} vs {w64:.6f}")
+    assert rel_rms(fg, f64) < 1e-5
+    assert abs(en["sf"][1] - w64) < 1e-4 * abs(w64) + 1e-6
+    assert abs(en["nn"][0] - et) < 1e-5 * abs(et) + 1e-5
+    # Newton's third law: the NNP forces sum to zero
+    assert np.abs(fg.sum(0)).max() < 1e-3 * np.abs(fg).max()
+
+
+def test_sf_forces_bf16_error(water_c1):
+    """BF16 MFMA network: NNP force error vs FP64 within 1e-2 relative RMS (24k atoms, 3x128)."""
+    widths = [80, 128, 128, 128, 1]
+    e, o, par, G64, gmax = _setup(water_c1, widths)
+    _, _, dG64 = o.mlp_eval(G64)
+    o.clear_f()
+    o.sf_force(dG64)
+    f64 = o.get("f")
+    errs = {}
+    for prec in ("fp32", "bf16", "fp16"):
+        e.mlp_set(widths, par, 0.5, prec, gmax)
+        e.sf_descriptor()
+        e.clear_force()
+        e.mlp_eval(False)
+        e.sf_force(False)
+        errs[prec] = rel_rms(e.get("f"), f64)
+    print("[sf 24k] NNP force rel-rms vs fp64:", errs)
+    assert errs["fp32"] < 1e-5
+    assert errs["bf16"] < 1e-2
+    assert errs["fp16"] < 5e-3
+
+
+def test_tbl_descriptor_and_force(water_c0):
+    sysm = water_c0
+    from nnpengine_amd.engine import Engine
+    cp, cw, cd, cr = M.tbl_coefficients(sysm.npair, 5, 6.0)
+    res = {}
+    for prec in (0, 1):
+        o = pyoracle.Oracle(sysm, prec)
+        o.neigh_init(6.0, 1.0, 6.0)
+        o.rebuild(False)
+        o.nnp_tbl_init(6.0, 600, cp, cw, cd, cr)
+        res[prec] = o
+    e = Engine(sysm)
+    e.neigh_init(6.0, 1.0, 6.0)
+    e.build()
+    e.nnp_tbl_set(6.0, 600, cp, cw, cd, cr)
+    e.nnp_tbl_descriptor()
+    G = e.nnp_get("G").astype(np.float64)
+    G32 = res[0].nnp_tbl_descriptor()
+    G64 = res[1].nnp_tbl_descriptor()
+    print(f"[tbl] desc rel-rms vs fp32 {rel_rms(G, G32):.2e} vs fp64 {rel_rms(G, G64):.2e}")
+    assert rel_rms(G, G32) < 2e-6
+    assert rel_rms(G, G64) < 1e-5
+    rng = np.random.default_rng(3)
+    dedg = (0.01 * rng.standard_normal(G.shape)).astype(np.float32)
+    e.nnp_set_dedg(dedg)
+    e.clear_force()
+    e.energy_clear()
+    e.nnp_tbl_force(True)
+    fg = e.get("f")
+    en = e.energy()
+    res[1].clear_f()
+    eo, wo = res[1].nnp_tbl_force(dedg.astype(np.float64))
+    f64 = res[1].get("f")
+    print(f"[tbl] force rel-rms vs fp64 {rel_rms(fg, f64):.2e}")
+    assert rel_rms(fg, f64) < 1e-5
+    assert abs(en["nnp_pair"][0] - eo) < 1e-5 * abs(eo) + 1e-6
+
+
+def test_md_graph_equals_eager(water_c0):
+    """Replayed hipGraph steps give bit-identical trajectories to eagerly launched steps."""
+    from nnpengine_amd.engine import Engine
+    sysm = water_c0
+    ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
+    sfp = M.sf_params(RC_NNP)
+    widths = [80, 64, 64, 1]
+    par = M.mlp_init(widths, sysm.ntype, 2) * 0.1
+    xs = []
+    for graph in (False, True):
+        e = Engine(sysm)
+        e.neigh_init(RC_PAIR, 2.0, RC_PAIR, RC_NNP)
+        e.pair_table(RC_PAIR, ft, ut)
+        e.bonded_set_from_system()
+        e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+        e.mlp_set(widths, par, 0.5, "bf16", 50.0)
+        e.gr_init(450, RC_PAIR, RC_PAIR)
+        log = e.md_run(40, dt=0.5, rebuild_every=10, nnp=2, gr_every=10, log_every=10, use_graph=graph)
+        assert log.shape[0] == 4
+        xs.append((e.get("x"), log))
+    assert np.array_equal(xs[0][0], xs[1][0])
+    assert np.array_equal(xs[0][1], xs[1][1])
+
+
+def test_md_nve_energy_conservation(water_c0):
+    """FP64 integration, dt = 0.5 fs, 400 steps: total energy drift small against its fluctuation."""
+    from nnpengine_amd.engine import Engine
+    sysm = water_c0
+    ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
+    e = Engine(sysm)
+    e.neigh_init(RC_PAIR, 2.0, RC_PAIR)
+    e.pair_table(RC_PAIR, ft, ut)
+    e.bonded_set_from_system()
+    log = e.md_run(400, dt=0.5, fp64=True, rebuild_every=10, log_every=20)
+    et = log[:, 0] + log[:, 2] + log[:, 4] + log[:, 6] + log[:, 14]
+    print("[nve] E_tot first/last", et[1], et[-1], "ke", log[1, 14], log[-1, 14])
+    st = e.md_stats()
+    assert st["skin_violations"] == 0
+    assert abs(et[-1] - et[1]) < 0.02 * abs(log[1, 14])
