@@ -66,7 +66,26 @@ def build(force=False, verbose=False):
         if res.returncode != 0:
             raise RuntimeError(f"link failed:\n{res.stderr[-4000:]}")
         os.replace(tmp, LIB)
+    build_host(force)
     return LIB
+
+
+HOST_EXE = os.path.join(HERE, "nnpe_run")
+
+
+def build_host(force=False):
+    """The reference-style C++ driver (host/nnpe_run.cpp over host/nnpe_host.cpp), linked
+    against libnnpe.so (rpath $ORIGIN)."""
+    srcs = [os.path.join(HERE, "host", f) for f in ("nnpe_host.cpp", "nnpe_run.cpp")]
+    deps = srcs + [os.path.join(HERE, "host", "nnpe_host.h"), LIB]
+    if not force and os.path.exists(HOST_EXE) and os.path.getmtime(HOST_EXE) >= max(os.path.getmtime(d) for d in deps):
+        return HOST_EXE
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-o", HOST_EXE] + srcs + \
+          ["-L" + HERE, "-lnnpe", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host driver build failed:\n{res.stderr[-4000:]}")
+    return HOST_EXE
 
 
 if __name__ == "__main__":

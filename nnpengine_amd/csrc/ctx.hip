@@ -103,6 +103,8 @@ extern "C" int nnpe_system_set(nnpe_ctx* ctx, int natom, int ntype, float bl, co
   NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   free_all(ctx);
   ctx->have_grid = ctx->have_list = ctx->have_pair = false;
+  ctx->fp64 = false;
+  ctx->md_step = 0;
   ctx->nbond = ctx->nangle = ctx->ndih = 0;
   ctx->natom = natom; ctx->ntype = ntype; ctx->npair = ntype * (ntype + 1) / 2;
   ctx->nmol = nmol; ctx->nmoltype = nmoltype;

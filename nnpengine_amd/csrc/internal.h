@@ -160,6 +160,7 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev);
 int mlp_enqueue(nnpe_ctx* ctx, bool ev);
 void md_state_free(nnpe_ctx* ctx);
 int neigh_check_and_grow(nnpe_ctx* ctx, bool* rebuilt);
+int verlet_switch_precision(nnpe_ctx* ctx, bool fp64);
 
 // NNP helpers shared by nnp.hip / mlp.hip / md.hip
 NnpState* nnp_state_of(nnpe_ctx* ctx);

@@ -18,6 +18,7 @@
 namespace {
 
 enum StepBits { kRebuild = 1, kGr = 2, kGrFull = 4, kLog = 8 };
+constexpr int kCopyProbeBytes = 64 << 20;  // each direction
 
 struct MdGraphs {
   std::map<int, hipGraphExec_t> exec;
@@ -59,6 +60,11 @@ __global__ void k_md_reset(DevCounters* cnt) {
 __global__ void k_clear_force(int n, float4* f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) f[i] = make_float4(0, 0, 0, 0);
+}
+
+// plain streaming copy of a known number of bytes (calibrates memory-traffic counters)
+__global__ void __launch_bounds__(256) k_copy_probe(int n, const float4* __restrict__ a, float4* __restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
 }
 
 MdGraphs* graphs_of(nnpe_ctx* ctx) {
@@ -119,6 +125,7 @@ void md_state_free(nnpe_ctx* ctx) {
 extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, double* log_out, int log_cap, int* nlog) {
   if (!ctx || !p || nsteps < 0) return NNPE_ERR_ARG;
   const nnpe_md_params P = *p;
+  int rc0;
   if (!ctx->have_grid) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_md_run before nnpe_neigh_init");
   if (P.pair && !ctx->have_pair) return ctx_fail(ctx, NNPE_ERR_STATE, "pair term without a table");
   if (P.nnp && !ctx->mlp) return ctx_fail(ctx, NNPE_ERR_STATE, "NNP term without nnpe_mlp_set");
@@ -128,9 +135,9 @@ extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, d
   if (P.gr_long_every > 0 && (P.gr_every <= 0 || P.gr_long_every % P.gr_every))
     return ctx_fail(ctx, NNPE_ERR_ARG, "gr_long_every must be a multiple of gr_every");
   if (ctx->dt != P.dt || ctx->fp64 != (P.fp64 != 0)) {
-    ctx->dt = P.dt;
-    ctx->fp64 = P.fp64 != 0;
     md_state_free(ctx);
+    ctx->dt = P.dt;
+    if ((rc0 = verlet_switch_precision(ctx, P.fp64 != 0))) return rc0;
   }
   // log buffer
   const int nrec = P.log_every > 0 ? nsteps / P.log_every + 2 : 1;
@@ -226,7 +233,8 @@ extern "C" int nnpe_md_stats(nnpe_ctx* ctx, long long* out) {
 // accumulate (forces += ...) are timed as launched, their results are not meant to be used.
 extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* avg_ms) {
   if (!ctx || reps <= 0 || !avg_ms) return NNPE_ERR_ARG;
-  if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "profile needs a built list");
+  if (!ctx->have_list && stage != NNPE_STAGE_COPY) return ctx_fail(ctx, NNPE_ERR_STATE, "profile needs a built list");
+  float4* probe = nullptr;
   auto launch = [&](int st) -> int {
     switch (st) {
       case NNPE_STAGE_PAIR: return ctx->have_pair ? pair_enqueue(ctx, false) : NNPE_ERR_STATE;
@@ -245,6 +253,15 @@ extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* av
       case NNPE_STAGE_SQ: return ctx->sq_ftmx ? sq_enqueue(ctx) : NNPE_ERR_STATE;
       case NNPE_STAGE_TBL_DESC: return nnp_tbl_desc_enqueue(ctx);
       case NNPE_STAGE_TBL_FORCE: return nnp_tbl_force_enqueue(ctx, false);
+      case NNPE_STAGE_COPY: {
+        const int n = kCopyProbeBytes / 16;
+        if (!probe) {
+          if (hipMalloc(reinterpret_cast<void**>(&probe), 2 * size_t(kCopyProbeBytes)) != hipSuccess) return NNPE_ERR_HIP;
+          if (hipMemsetAsync(probe, 0, 2 * size_t(kCopyProbeBytes), ctx->stream) != hipSuccess) return NNPE_ERR_HIP;
+        }
+        hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, ctx->stream, n, probe, probe + n);
+        return hipGetLastError() == hipSuccess ? NNPE_OK : NNPE_ERR_HIP;
+      }
       default: return NNPE_ERR_ARG;
     }
   };
@@ -261,6 +278,7 @@ extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* av
   NNPE_CHECK_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (probe) (void)hipFree(probe);
   if (rc) return rc;
   *avg_ms = double(ms) / reps;
   return NNPE_OK;

@@ -101,14 +101,51 @@ __global__ void k_clear_f(int n, float4* f) {
   if (i < n) f[i] = make_float4(0, 0, 0, 0);
 }
 
+// the integration state changes representation: copy it across so a run continues seamlessly
+__global__ void k_to_fp64(int n, const float4* xq, const float4* v, const float4* fprev, double4* xd, double4* vd,
+                          double4* ad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 x = xq[i], w = v[i], a = fprev[i];
+  xd[i] = make_double4(x.x, x.y, x.z, 0);
+  vd[i] = make_double4(w.x, w.y, w.z, 0);
+  ad[i] = make_double4(a.x, a.y, a.z, 0);
+}
+
+__global__ void k_to_fp32(int n, float4* xq, float4* v, float4* fprev, const double4* xd, const double4* vd,
+                          const double4* ad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 x = xd[i], w = vd[i], a = ad[i];
+  float4 q = xq[i];
+  q.x = float(x.x); q.y = float(x.y); q.z = float(x.z);
+  xq[i] = q;
+  v[i] = make_float4(float(w.x), float(w.y), float(w.z), 0);
+  fprev[i] = make_float4(float(a.x), float(a.y), float(a.z), 0);
+}
+
 }  // namespace
+
+int verlet_switch_precision(nnpe_ctx* ctx, bool fp64) {
+  if (fp64 == ctx->fp64) return NNPE_OK;
+  const int N = ctx->natom;
+  if (fp64)
+    hipLaunchKernelGGL(k_to_fp64, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->xq, ctx->v, ctx->fprev, ctx->xd,
+                       ctx->vd, ctx->ad);
+  else
+    hipLaunchKernelGGL(k_to_fp32, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->xq, ctx->v, ctx->fprev, ctx->xd,
+                       ctx->vd, ctx->ad);
+  NNPE_CHECK_LAUNCH(ctx);
+  ctx->fp64 = fp64;
+  return NNPE_OK;
+}
 
 extern "C" int nnpe_verlet_set(nnpe_ctx* ctx, double dt, int fp64) {
   if (!ctx) return NNPE_ERR_ARG;
   if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
+  md_state_free(ctx);
   ctx->dt = dt;
-  ctx->fp64 = fp64 != 0;
-  return NNPE_OK;
+  return verlet_switch_precision(ctx, fp64 != 0);
 }
 
 int verlet_enqueue_x(nnpe_ctx* ctx) {

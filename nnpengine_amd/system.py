@@ -386,6 +386,38 @@ def write_config(path, sys: System, mol_files):
                      f"{sys.x[i, 0]:.9g} {sys.x[i, 1]:.9g} {sys.x[i, 2]:.9g}\n")
 
 
+def write_classical_input(d, sysm, pair_cutoff, skin, gr_cutoff, force_tbl, pot_tbl, nbin_gr, nq, q_max,
+                          temp=(300.0, 300.0), steps=(100, 10, 10, 10, 10), outputs=None):
+    """The positional input file of ``classical`` (src/run.cpp:25-50) plus the configuration,
+    molecule and table files it names, written into directory d.  Returns the input path."""
+    import os
+    mols = []
+    for k, mt in enumerate(sysm.moltypes):
+        p = os.path.join(d, f"mol{k}.txt")
+        write_molecule(p, mt)
+        mols.append(p)
+    cfg = os.path.join(d, "config.txt")
+    write_config(cfg, sysm, mols)
+    tbl = os.path.join(d, "table.txt")
+    write_table(tbl, sysm, pair_cutoff, force_tbl, pot_tbl)
+    lines = [cfg, f"{pair_cutoff:.9g} {skin:.9g} {gr_cutoff:.9g}", f"{force_tbl.shape[1] - 1} {tbl}",
+             str(len(sysm.bond_coef))]
+    lines += [f"{r[0]:.9g} {r[3] / 2:.9g}" for r in sysm.bond_coef]
+    ap = sysm.angle_params or []
+    lines += [str(len(ap))] + [" ".join(f"{v:.9g}" for v in r) for r in ap]
+    dp = sysm.dihedral_params or []
+    lines += [str(len(dp))] + [" ".join(f"{v:.9g}" for v in r) for r in dp]
+    lines += [f"{nbin_gr} {nq} {q_max:.9g}", "0 1", "0", "*", f"{temp[0]:.9g} {temp[1]:.9g}",
+              " ".join(str(s) for s in steps)]
+    o = outputs or {}
+    lines += [f"{o.get('xyz', '*')} 0", f"{o.get('gr', '*')} 0", f"{o.get('sq', '*')} 0", "* 0 10 10", "* 0",
+              f"{o.get('force', '*')} 0"]
+    path = os.path.join(d, "in.classical")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
 WATER_LJ = {(0, 0): (0.155, 3.166)}
 BUTANE_LJ_BASE = {0: (0.195, 3.75), 1: (0.091, 3.95)}
 
