@@ -36,7 +36,7 @@ enum nnpe_state_kind { NNPE_X = 0, NNPE_V = 1, NNPE_F = 2, NNPE_FPREV = 3 };
 /* energy/virial terms accumulated by the compute calls (nnpe_energy_get) */
 enum nnpe_term { NNPE_TERM_PAIR = 0, NNPE_TERM_BOND = 1, NNPE_TERM_ANGLE = 2, NNPE_TERM_DIHEDRAL = 3,
                  NNPE_TERM_NNP_PAIR = 4, NNPE_TERM_NN = 5, NNPE_TERM_SF = 6, NNPE_TERM_KE = 7,
-                 NNPE_NTERM = 8 };
+                 NNPE_TERM_RMDF = 8, NNPE_NTERM = 9 };
 
 const char* nnpe_status_string(int status);
 const char* nnpe_last_error(const nnpe_ctx* ctx);
@@ -120,6 +120,23 @@ int nnpe_gr_get(nnpe_ctx* ctx, long long* counts, double* h);
 int nnpe_sq_init(nnpe_ctx* ctx, int nq, double q_min, double q_max);
 int nnpe_sq_compute(nnpe_ctx* ctx, double* out);
 
+/* RMDF ctor (src/grsq.cpp:899-1060): refine the pair tables against nsq measured S(q) sets.
+ * data [nsq][nq][3 + ntype] rows (q, S_exp(q), weight, form factor per element) on the S(q)
+ * grid of nnpe_sq_init, which must have q_min = 0; the refined table's bins must be the inner
+ * g(r) bins.  gamma = 1 is direct refinement (DA/RMD), 0 <= gamma < 1 FMIRL.  target 0: the
+ * LJTableCoulDSF table (nnpe_pair_table); target 1: the NNPOTENTIAL_TBL pair tables, special
+ * classes 0 and 3 (the latter scaled by 0.5).  The tables as they are now are kept as the base. */
+int nnpe_rmdf_init(nnpe_ctx* ctx, int nsq, const float* data, float strength, float gamma, int target);
+/* RMDF::loadDeltaSQ (src/grsq.cpp:1419-1443): relaxed residual [nsq][nq] to resume from. */
+int nnpe_rmdf_set_delta(nnpe_ctx* ctx, const float* deltasq);
+/* compute_Qspace then compute_Rspace[_NNP] (+ compute_Potential[_NNP] when potential) on the
+ * current S(q) (src/grsq.cpp:1079-1377); residual energy into NNPE_TERM_RMDF (stored). */
+int nnpe_rmdf_step(nnpe_ctx* ctx, int potential);
+/* Last step's S_n(q) and relaxed residual [nsq][nq] and force_q [npair][nq]; any may be NULL. */
+int nnpe_rmdf_get(nnpe_ctx* ctx, float* sq, float* deltasq, float* force_q);
+/* The refined tables, force then potential, in the target's layout. */
+int nnpe_rmdf_tables(nnpe_ctx* ctx, float* force, float* potential);
+
 /* NNPOTENTIAL_TBL ctor (src/pair_nnp.cpp:433-598): tables built from the weight-file arrays
  * coef_p [npair][4][6] (eps, sigma, c8, c10, c12, qij per special class) and
  * coef_w / coef_d / coef_r [npair][80] (20 descriptors x 4 special classes). */
@@ -169,6 +186,7 @@ typedef struct nnpe_md_params {
   int sq;             /* S(q) after each tally */
   int log_every;      /* energy/virial record every n steps (0 = never) */
   int use_graph;      /* replay captured step graphs (fixed-interval rebuilds only) */
+  int rmdf;           /* RMDF refinement after each S(q) (needs nnpe_rmdf_init, sq, gr_every) */
 } nnpe_md_params;
 /* Run nsteps; log_out receives [nlog][2*NNPE_NTERM] records (energy, virial per term; the
  * NNPE_TERM_KE slot holds the kinetic energy) for steps s with s % log_every == 0. */

@@ -13,8 +13,8 @@ STATUS = {0: "ok", 1: "bad argument", 2: "HIP runtime error", 3: "call out of or
           7: "no HIP device", 8: "unsupported configuration"}
 
 NNPE_X, NNPE_V, NNPE_F, NNPE_FPREV = 0, 1, 2, 3
-TERMS = ("pair", "bond", "angle", "dihedral", "nnp_pair", "nn", "sf", "ke")
-NTERM = 8
+TERMS = ("pair", "bond", "angle", "dihedral", "nnp_pair", "nn", "sf", "ke", "rmdf")
+NTERM = 9
 
 # (name, restype, argtypes) of every entry point declared in include/nnpe.h
 _P, _I, _F, _D = C.c_void_p, C.c_int, C.c_float, C.c_double
@@ -50,6 +50,11 @@ SIGNATURES = [
     ("nnpe_gr_get", _I, [_P, _PLL, _PD]),
     ("nnpe_sq_init", _I, [_P, _I, _D, _D]),
     ("nnpe_sq_compute", _I, [_P, _PD]),
+    ("nnpe_rmdf_init", _I, [_P, _I, _PF, _F, _F, _I]),
+    ("nnpe_rmdf_set_delta", _I, [_P, _PF]),
+    ("nnpe_rmdf_step", _I, [_P, _I]),
+    ("nnpe_rmdf_get", _I, [_P, _PF, _PF, _PF]),
+    ("nnpe_rmdf_tables", _I, [_P, _PF, _PF]),
     ("nnpe_nnp_tbl_set", _I, [_P, _F, _I, _PF, _PF, _PF, _PF]),
     ("nnpe_nnp_tbl_descriptor", _I, [_P]),
     ("nnpe_nnp_tbl_force", _I, [_P, _I]),
@@ -75,7 +80,7 @@ class MdParams(C.Structure):
     """struct nnpe_md_params (include/nnpe.h)."""
     _fields_ = [("dt", C.c_double), ("fp64", C.c_int), ("rebuild_every", C.c_int), ("pair", C.c_int),
                 ("bonded", C.c_int), ("nnp", C.c_int), ("gr_every", C.c_int), ("gr_long_every", C.c_int),
-                ("sq", C.c_int), ("log_every", C.c_int), ("use_graph", C.c_int)]
+                ("sq", C.c_int), ("log_every", C.c_int), ("use_graph", C.c_int), ("rmdf", C.c_int)]
 
 _lib = None
 

@@ -177,6 +177,7 @@ __global__ void __launch_bounds__(256) k_bonded(int nmol, const int* __restrict_
 
 extern "C" int nnpe_pair_table(nnpe_ctx* ctx, float cutoff, int nbin, const float* force, const float* potential) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (ctx) rmdf_state_free(ctx);  // a refinement is tied to the tables and grids it was set up on
   if (!ctx || !force || !potential || nbin <= 0 || !(cutoff > 0)) return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_pair_table: bad argument");
   if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
   const int nb1 = nbin + 1;

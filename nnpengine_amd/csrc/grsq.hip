@@ -154,6 +154,7 @@ __global__ void __launch_bounds__(256) k_sq(int nq, int nbin, int npair, const d
 
 extern "C" int nnpe_gr_init(nnpe_ctx* ctx, int nbi, float r_max, float r_max_inner, int* nbin_total) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (ctx) rmdf_state_free(ctx);  // a refinement is tied to the tables and grids it was set up on
   if (!ctx || nbi <= 0) return NNPE_ERR_ARG;
   if (!ctx->have_grid) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_gr_init before nnpe_neigh_init");
   // GR ctor (src/grsq.cpp:28-47)
@@ -247,6 +248,7 @@ extern "C" int nnpe_gr_get(nnpe_ctx* ctx, long long* counts, double* h) {
 
 extern "C" int nnpe_sq_init(nnpe_ctx* ctx, int nq, double q_min, double q_max) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (ctx) rmdf_state_free(ctx);  // a refinement is tied to the tables and grids it was set up on
   if (!ctx || nq <= 0 || !(q_max > q_min) || q_min < 0) return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_sq_init: bad argument");
   if (!ctx->gr_count) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_sq_init before nnpe_gr_init");
   ctx->sq_nq = nq; ctx->sq_qmin = q_min; ctx->sq_qmax = q_max;

@@ -41,6 +41,7 @@ struct DevCounters {
 
 struct NnpState;  // descriptor tables and moment buffers (nnp.hip)
 struct MlpState;  // per-element network weights (mlp.hip)
+struct RmdfState; // S(q) refinement of the pair tables (rmdf.hip)
 
 struct nnpe_ctx {
   int device = 0;
@@ -127,6 +128,7 @@ struct nnpe_ctx {
   // ---- NNP: descriptors G and dE/dG by row, per-atom network energies ---------------------
   NnpState* nnp = nullptr;
   MlpState* mlp = nullptr;
+  RmdfState* rmdf = nullptr;
   float* G = nullptr;
   float* dG = nullptr;
   int D_cur = 0, D_cap = 0;
@@ -169,6 +171,14 @@ void mlp_state_free(nnpe_ctx* ctx);
 int nnp_desc_reserve(nnpe_ctx* ctx, int D);
 int sf_dim(int ntype);
 int type_index_enqueue(nnpe_ctx* ctx);
+// NNP pair tables being refined: force / potential [npair][4][nbin+1]; nonzero if not set
+int nnp_tbl_tables(nnpe_ctx* ctx, float** ftab, float** utab, int* nbin, float* dr);
+
+// RMDF (rmdf.hip)
+void rmdf_state_free(nnpe_ctx* ctx);
+int rmdf_enqueue(nnpe_ctx* ctx, bool potential);
+bool rmdf_active(nnpe_ctx* ctx);
+float rmdf_gamma(nnpe_ctx* ctx);
 
 // ---- helpers -------------------------------------------------------------------------------
 int ctx_fail(nnpe_ctx* ctx, int code, const std::string& msg);

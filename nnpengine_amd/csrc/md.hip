@@ -41,11 +41,12 @@ __global__ void k_skin_accum(DevCounters* cnt) {
   cnt->invalid = 0;
 }
 
+// the RMDF slot is not cleared: it holds the residual of the last refinement step
 __global__ void k_log_append(double* ev_sum, double* log, int cap, int nval, DevCounters* cnt) {
   const int slot = cnt->log_count;
   if (threadIdx.x < nval) {
     if (slot < cap) log[(size_t)slot * nval + threadIdx.x] = ev_sum[threadIdx.x];
-    ev_sum[threadIdx.x] = 0.0;
+    if (threadIdx.x / 2 != NNPE_TERM_RMDF) ev_sum[threadIdx.x] = 0.0;
   }
   __syncthreads();
   if (threadIdx.x == 0) cnt->log_count = slot + 1;
@@ -87,6 +88,9 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   if (key & kGr) {
     if ((rc = gr_enqueue(ctx, (key & kGrFull) != 0))) return rc;
     if (P.sq && (rc = sq_enqueue(ctx))) return rc;
+    // RMDF compute_Qspace / compute_Rspace, and compute_Potential on logged steps when
+    // gamma != 1 (src/run.cpp:294-298, :1206-1210)
+    if (P.rmdf && (rc = rmdf_enqueue(ctx, (key & kLog) && rmdf_gamma(ctx) != 1.0f))) return rc;
   }
   const bool ev = (key & kLog) != 0;
   if (P.pair) {
@@ -132,6 +136,8 @@ extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, d
   if (P.nnp == 1 && (!ctx->nnp || !nnp_state_of(ctx) || ctx->mlp == nullptr)) return ctx_fail(ctx, NNPE_ERR_STATE, "NNP tables missing");
   if ((P.gr_every > 0) && !ctx->gr_count) return ctx_fail(ctx, NNPE_ERR_STATE, "g(r) requested without nnpe_gr_init");
   if (P.sq && !ctx->sq_ftmx) return ctx_fail(ctx, NNPE_ERR_STATE, "S(q) requested without nnpe_sq_init");
+  if (P.rmdf && (!rmdf_active(ctx) || !P.sq || P.gr_every <= 0))
+    return ctx_fail(ctx, NNPE_ERR_STATE, "RMDF requested without nnpe_rmdf_init, S(q) or g(r) tallies");
   if (P.gr_long_every > 0 && (P.gr_every <= 0 || P.gr_long_every % P.gr_every))
     return ctx_fail(ctx, NNPE_ERR_ARG, "gr_long_every must be a multiple of gr_every");
   if (ctx->dt != P.dt || ctx->fp64 != (P.fp64 != 0)) {

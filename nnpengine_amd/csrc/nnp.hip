@@ -437,6 +437,13 @@ int nnp_desc_reserve(nnpe_ctx* ctx, int D) {
 }
 
 static NnpState& nnp_state(nnpe_ctx* ctx) { return *nnp_state_of(ctx); }
+
+int nnp_tbl_tables(nnpe_ctx* ctx, float** ftab, float** utab, int* nbin, float* dr) {
+  if (!ctx->nnp || !ctx->nnp->tbl) return 1;
+  *ftab = ctx->nnp->tfpair; *utab = ctx->nnp->tpair;
+  *nbin = ctx->nnp->tbl_nbin; *dr = ctx->nnp->tbl_cut / float(ctx->nnp->tbl_nbin);
+  return 0;
+}
 static float* nnp_G(nnpe_ctx* ctx) { return ctx->G; }
 static float* nnp_dG(nnpe_ctx* ctx) { return ctx->dG; }
 static void nnp_set_current_D(nnpe_ctx* ctx, int D) { ctx->D_cur = D; }
@@ -446,6 +453,7 @@ static int nnp_current_D(nnpe_ctx* ctx) { return ctx->D_cur; }
 extern "C" int nnpe_nnp_tbl_set(nnpe_ctx* ctx, float cutoff, int nbin, const float* coef_p, const float* coef_w,
                                 const float* coef_d, const float* coef_r) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
+  if (ctx) rmdf_state_free(ctx);  // a refinement holds the old tables as its base
   if (!ctx || nbin <= 0 || !(cutoff > 0) || !coef_p || !coef_w || !coef_d || !coef_r)
     return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_nnp_tbl_set: bad argument");
   if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");

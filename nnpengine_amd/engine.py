@@ -33,6 +33,7 @@ class Engine:
         self._check(self.L.nnpe_create(C.byref(h), device), ctx=None)
         self.h = h
         self.natom = sysm.natom
+        self.ntype = sysm.ntype
         self.npair = sysm.npair
         offs, flat = sysm.special_flat()
         k = [_f32(sysm.x), _f32(sysm.v), _f32(sysm.q), _f32(sysm.mass), _i32(sysm.typ), _i32(sysm.mol_id),
@@ -174,6 +175,39 @@ class Engine:
         self._check(self.L.nnpe_sq_compute(self.h, _p(out, C.c_double)))
         return out
 
+    # RMDF ----------------------------------------------------------------------------------
+    def rmdf_init(self, data, strength, gamma, target=0):
+        """data [nsq][nq][3 + ntype] (q, S_exp, weight, form factors) on the S(q) grid."""
+        d = _f32(data)
+        if d.ndim != 3 or d.shape[1] != self.sq_nq or d.shape[2] != 3 + self.ntype:
+            raise ValueError(f"RMDF data must be [nsq][{self.sq_nq}][{3 + self.ntype}], got {d.shape}")
+        self._rmdf = dict(nsq=d.shape[0], target=target)
+        self._check(self.L.nnpe_rmdf_init(self.h, d.shape[0], _p(d, C.c_float), C.c_float(strength),
+                                          C.c_float(gamma), int(target)))
+
+    def rmdf_set_delta(self, deltasq):
+        d = _f32(deltasq).reshape(self._rmdf["nsq"], self.sq_nq)
+        self._check(self.L.nnpe_rmdf_set_delta(self.h, _p(d, C.c_float)))
+
+    def rmdf_step(self, potential=False):
+        self._check(self.L.nnpe_rmdf_step(self.h, int(potential)))
+
+    def rmdf_get(self):
+        nsq = self._rmdf["nsq"]
+        sq = np.zeros((nsq, self.sq_nq), np.float32)
+        ds = np.zeros((nsq, self.sq_nq), np.float32)
+        fq = np.zeros((self.npair, self.sq_nq), np.float32)
+        self._check(self.L.nnpe_rmdf_get(self.h, _p(sq, C.c_float), _p(ds, C.c_float), _p(fq, C.c_float)))
+        return sq, ds, fq
+
+    def rmdf_tables(self, nbin):
+        """Refined (force, potential) tables; nbin = the refined table's bin count."""
+        shape = (self.npair, nbin + 1) if self._rmdf["target"] == 0 else (self.npair, 4, nbin + 1)
+        f = np.zeros(shape, np.float32)
+        u = np.zeros(shape, np.float32)
+        self._check(self.L.nnpe_rmdf_tables(self.h, _p(f, C.c_float), _p(u, C.c_float)))
+        return f, u
+
     # NNPOTENTIAL_TBL ---------------------------------------------------------------------
     def nnp_tbl_set(self, cutoff, nbin, coef_p, coef_w, coef_d, coef_r):
         self._nnp = [_f32(coef_p), _f32(coef_w), _f32(coef_d), _f32(coef_r)]
@@ -221,9 +255,9 @@ class Engine:
 
     # MD loop -----------------------------------------------------------------------------
     def md_run(self, nsteps, dt=0.5, fp64=False, rebuild_every=10, pair=True, bonded=True, nnp=0, gr_every=0,
-               gr_long_every=0, sq=False, log_every=0, use_graph=True):
+               gr_long_every=0, sq=False, log_every=0, use_graph=True, rmdf=False):
         p = _lib.MdParams(dt, int(fp64), rebuild_every, int(pair), int(bonded), int(nnp), gr_every, gr_long_every,
-                          int(sq), log_every, int(use_graph))
+                          int(sq), log_every, int(use_graph), int(rmdf))
         cap = nsteps // log_every + 2 if log_every > 0 else 1
         log = np.zeros((cap, 2 * _lib.NTERM), np.float64)
         n = C.c_int(0)

@@ -542,6 +542,98 @@ struct Eng {
     }
   }
 
+  // ---- RMDF (src/grsq.cpp:899-1377) ----------------------------------------------------------
+  // Refines the pair table against experimental S(q): datasets n with rows
+  // (q, S_exp, weight, form factor per element) on the S(q) grid q_i = (i+1/2) q_max / nq.
+  int rm_nsq = 0, rm_nq = 0, rm_nr = 0, rm_target = 0;
+  T rm_gamma = 0;
+  std::vector<T> rm_sqex, rm_wt, rm_sqwt, rm_fwt, rm_ftmx, rm_potmx, rm_deltasq, rm_sq, rm_forceq;
+  std::vector<T> rm_fbase, rm_ubase;  // base tables (pair: [npair][nb1]; NNP: [npair][4][nb1])
+  std::vector<T> sq_T;                // partial S(q) of SQ::compute in T: [npair][nq]
+  void rmdf_init(int nsq, int nq, double q_max, const float* data, T strength, T gamma, int target) {
+    rm_nsq = nsq; rm_nq = nq; rm_gamma = gamma; rm_target = target;
+    rm_nr = gr_nbin_inner;
+    const int nt = ntype;
+    rm_sqex.assign(size_t(nsq) * nq, 0); rm_wt.assign(size_t(nsq) * nq, 0);
+    rm_sqwt.assign(size_t(nsq) * npair * nq, 0); rm_fwt.assign(size_t(nsq) * npair * nq, 0);
+    rm_deltasq.assign(size_t(nsq) * nq, 0); rm_sq.assign(size_t(nsq) * nq, 0);
+    rm_forceq.assign(size_t(npair) * nq, 0);
+    const T natsq = sqr(T(natom));
+    std::vector<T> ff(nt);
+    for (int n = 0; n < nsq; ++n)
+      for (int i = 0; i < nq; ++i) {
+        const float* row = data + (size_t(n) * nq + i) * (3 + nt);
+        rm_sqex[size_t(n) * nq + i] = T(row[1]);
+        T wti = T(row[2]) * strength;
+        rm_wt[size_t(n) * nq + i] = wti;
+        T cifi2 = 0;
+        for (int j = 0; j < nt; ++j) { ff[j] = T(row[3 + j]); cifi2 += ff[j] * T(typecount[j]); }
+        cifi2 = sqr(cifi2 / T(natom));
+        int ipr = 0;
+        for (int j = 0; j < nt; ++j) {
+          T fi = ff[j], ni = T(typecount[j]) / natsq;
+          for (int k = j; k < nt; ++k, ++ipr) {
+            T fw = fi * ff[k] / cifi2;
+            rm_sqwt[(size_t(n) * npair + ipr) * nq + i] = fw * T(typecount[k]) * ni * T(k == j ? 1 : 2);
+            rm_fwt[(size_t(n) * npair + ipr) * nq + i] = fw * wti;
+          }
+        }
+      }
+    rm_ftmx.assign(size_t(rm_nr) * nq, 0); rm_potmx.assign(size_t(rm_nr) * nq, 0);
+    const T dq = T(q_max) / T(nq);
+    for (int r = 0; r < rm_nr; ++r) {
+      T rri = (T(r) + T(0.5F)) * gr_dr, fi = T(1) / sqr(rri);
+      for (int j = 0; j < nq; ++j) {
+        T qi = (T(j) + T(.5F)) * dq * rri;
+        T j0 = std::sin(qi) / qi;
+        rm_potmx[size_t(r) * nq + j] = j0;
+        rm_ftmx[size_t(r) * nq + j] = fi * (j0 - std::cos(qi));
+      }
+    }
+    if (target == 0) { rm_fbase = tbl_f; rm_ubase = tbl_u; }
+    else { rm_fbase = tfpair; rm_ubase = tpair; }
+  }
+  T rmdf_qspace() {  // compute_Qspace (src/grsq.cpp:1079-1118) on the current partial S(q)
+    const int nq = rm_nq;
+    T erg = 0;
+    std::fill(rm_forceq.begin(), rm_forceq.end(), T(0));
+    for (int n = 0; n < rm_nsq; ++n) {
+      T* sqi = &rm_sq[size_t(n) * nq];
+      for (int i = 0; i < nq; ++i) sqi[i] = 0;
+      for (int p = 0; p < npair; ++p)
+        for (int i = 0; i < nq; ++i) sqi[i] += sq_T[size_t(p) * nq + i] * rm_sqwt[(size_t(n) * npair + p) * nq + i];
+      for (int i = 0; i < nq; ++i) {
+        T d = sqi[i] - rm_sqex[size_t(n) * nq + i];
+        erg += sqr(d) * rm_wt[size_t(n) * nq + i];
+        T& ds = rm_deltasq[size_t(n) * nq + i];
+        ds += rm_gamma * (d - ds);
+      }
+      for (int p = 0; p < npair; ++p)
+        for (int i = 0; i < nq; ++i)
+          rm_forceq[size_t(p) * nq + i] += rm_deltasq[size_t(n) * nq + i] * rm_fwt[(size_t(n) * npair + p) * nq + i];
+    }
+    return erg * T(natom) * T(.25F);
+  }
+  // compute_Rspace / compute_Potential (src/grsq.cpp:1120-1210) and their NNP-table twins
+  // (src/grsq.cpp:1213-1377): table = base + force_q . kernel, special class 3 scaled by 0.5
+  void rmdf_tables(bool potential) {
+    const int nb1 = (rm_target == 0 ? pair_nbin : nnp_nbin) + 1, nq = rm_nq;
+    const std::vector<T>& K = potential ? rm_potmx : rm_ftmx;
+    const std::vector<T>& base = potential ? rm_ubase : rm_fbase;
+    std::vector<T>& tab = rm_target == 0 ? (potential ? tbl_u : tbl_f) : (potential ? tpair : tfpair);
+    for (int p = 0; p < npair; ++p)
+      for (int r = 0; r < rm_nr; ++r) {
+        T temp = 0;
+        for (int q = 0; q < nq; ++q) temp += rm_forceq[size_t(p) * nq + q] * K[size_t(r) * nq + q];
+        if (rm_target == 0) {
+          tab[size_t(p) * nb1 + r] = base[size_t(p) * nb1 + r] + temp;
+        } else {
+          tab[(size_t(p) * 4 + 0) * nb1 + r] = temp + base[(size_t(p) * 4 + 0) * nb1 + r];
+          tab[(size_t(p) * 4 + 3) * nb1 + r] = temp * T(0.5) + base[(size_t(p) * 4 + 3) * nb1 + r];
+        }
+      }
+  }
+
   // ---- NNPOTENTIAL_TBL (src/pair_nnp.cpp:433-793) -------------------------------------------
   static constexpr int ND = 20;
   int nnp_nbin = 0;
@@ -1006,6 +1098,41 @@ int orc_gr_get(OrcHandle* h, long long* counts, double* hr) {
 }
 int orc_sq(OrcHandle* h, int nq, double q_min, double q_max, double* out) {
   auto run = [&](auto* e) { e->gr_reduce(); e->sq_compute(nq, q_min, q_max); std::copy(e->sq.begin(), e->sq.end(), out); };
+  if (h->prec) run(static_cast<Eng<double>*>(h->e)); else run(static_cast<Eng<float>*>(h->e));
+  return 0;
+}
+
+int orc_rmdf_init(OrcHandle* h, int nsq, int nq, double q_max, const float* data, float strength, float gamma, int target) {
+  if (h->prec) static_cast<Eng<double>*>(h->e)->rmdf_init(nsq, nq, q_max, data, strength, gamma, target);
+  else static_cast<Eng<float>*>(h->e)->rmdf_init(nsq, nq, q_max, data, strength, gamma, target);
+  return 0;
+}
+// one refinement step on the current g(r): S(q) (q_min = 0), compute_Qspace, compute_Rspace and,
+// when potential != 0, compute_Potential; returns the Q-space energy
+double orc_rmdf_step(OrcHandle* h, double q_max, int potential, double* sq_out, double* deltasq_out, double* forceq_out) {
+  double erg = 0;
+  auto run = [&](auto* e) {
+    using TT = typename std::remove_reference<decltype(e->x[0])>::type;
+    e->gr_reduce();
+    e->sq_compute(e->rm_nq, 0.0, q_max);
+    e->sq_T.assign(e->sq.begin(), e->sq.end());
+    erg = double(e->rmdf_qspace());
+    e->rmdf_tables(false);
+    if (potential) e->rmdf_tables(true);
+    for (size_t k = 0; k < e->rm_sq.size(); ++k) { if (sq_out) sq_out[k] = double(e->rm_sq[k]); if (deltasq_out) deltasq_out[k] = double(e->rm_deltasq[k]); }
+    if (forceq_out) for (size_t k = 0; k < e->rm_forceq.size(); ++k) forceq_out[k] = double(e->rm_forceq[k]);
+    (void)sizeof(TT);
+  };
+  if (h->prec) run(static_cast<Eng<double>*>(h->e)); else run(static_cast<Eng<float>*>(h->e));
+  return erg;
+}
+// current pair tables (target 0: LJTableCoulDSF force/potential [npair][nb1]; 1: NNP pair tables [npair][4][nb1])
+int orc_rmdf_tables(OrcHandle* h, double* f_out, double* u_out) {
+  auto run = [&](auto* e) {
+    const auto& F = e->rm_target == 0 ? e->tbl_f : e->tfpair;
+    const auto& U = e->rm_target == 0 ? e->tbl_u : e->tpair;
+    for (size_t k = 0; k < F.size(); ++k) { f_out[k] = double(F[k]); u_out[k] = double(U[k]); }
+  };
   if (h->prec) run(static_cast<Eng<double>*>(h->e)); else run(static_cast<Eng<float>*>(h->e));
   return 0;
 }

@@ -39,6 +39,7 @@ def lib():
         L.orc_list_export.restype = C.c_longlong
         L.orc_compute_ke.restype = C.c_double
         L.orc_mlp_eval.restype = C.c_double
+        L.orc_rmdf_step.restype = C.c_double
         for fn in ("orc_init", "orc_neigh_init", "orc_num_gx", "orc_rebuild", "orc_update_ghost_pos",
                    "orc_isvalid", "orc_list_size", "orc_list_export", "orc_get", "orc_set",
                    "orc_get_cell_ids", "orc_get_xmol_codes", "orc_clear_f", "orc_pair_set",
@@ -196,6 +197,26 @@ class Oracle:
         out = np.zeros((self.sys.npair, nq), np.float64)
         self.L.orc_sq(self.h, nq, C.c_double(q_min), C.c_double(q_max), _p(out, C.c_double))
         return out
+
+    # RMDF ---------------------------------------------------------------------------------
+    def rmdf_init(self, data, q_max, strength, gamma, target=0):
+        """data: [nsq][nq][3 + ntype] rows (q, S_exp, weight, form factors)."""
+        self._rm = _f32(data)
+        self.rm_shape = self._rm.shape
+        self.L.orc_rmdf_init(self.h, self._rm.shape[0], self._rm.shape[1], C.c_double(q_max), _p(self._rm, C.c_float),
+                             C.c_float(strength), C.c_float(gamma), target)
+
+    def rmdf_step(self, q_max, potential=True):
+        nsq, nq = self.rm_shape[0], self.rm_shape[1]
+        sq = np.zeros((nsq, nq)); ds = np.zeros((nsq, nq)); fq = np.zeros((self.sys.npair, nq))
+        e = self.L.orc_rmdf_step(self.h, C.c_double(q_max), int(potential), _p(sq, C.c_double), _p(ds, C.c_double),
+                                 _p(fq, C.c_double))
+        return e, sq, ds, fq
+
+    def rmdf_tables(self, shape):
+        f = np.zeros(shape); u = np.zeros(shape)
+        self.L.orc_rmdf_tables(self.h, _p(f, C.c_double), _p(u, C.c_double))
+        return f, u
 
     # NNP (reference tabulated form) -------------------------------------------------------
     def nnp_tbl_init(self, cutoff, nbin, coef_p, coef_w, coef_d, coef_r):

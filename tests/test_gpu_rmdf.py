@@ -1,0 +1,168 @@
+"""GPU RMDF (src/grsq.cpp:899-1377): one refinement step on the device against the FP64 oracle
+on the same g(r) (bit-exact counts, so the same partial S(q)), for the LJTableCoulDSF table and
+for the NNPOTENTIAL_TBL pair tables; then RMDF inside the MD loop (graph replays equal eager
+launches bit for bit, and the eager loop follows the FP32 oracle's trajectory).
+Bars: S(q), residuals and force_q within 1e-5 of the oracle's scale (FP32 arithmetic as the
+reference's), tables within 1e-5 relative RMS."""
+import numpy as np
+import pytest
+
+from nnpengine_amd import model as M
+from nnpengine_amd import system as S
+from oracle import pyoracle
+from tests.conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+RC, SKIN, NBIN, QMAX, NQ = 9.0, 0.5, 450, 12.0, 240
+
+
+def _data(sysm, seed, nsq=1):
+    rng = np.random.default_rng(seed)
+    q = (np.arange(NQ) + 0.5) * QMAX / NQ
+    out = []
+    for n in range(nsq):
+        ff = np.stack([(1.0 + 0.5 * t + 0.3 * n) * np.exp(-(0.02 + 0.01 * t) * q * q) + 0.1 * rng.random(NQ)
+                       for t in range(sysm.ntype)], 1)
+        sexp = 1.0 + 0.2 * np.sin(q) * np.exp(-0.1 * q)
+        w = 0.5 + rng.random(NQ)
+        out.append(np.concatenate([q[:, None], sexp[:, None], w[:, None], ff], 1))
+    return np.array(out, np.float32)
+
+
+def _scale_err(a, b):
+    return np.max(np.abs(a - b)) / max(1e-30, np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("gamma,nsq", [(1.0, 1), (0.4, 2)])
+def test_rmdf_step_pair_table(water_c0, gamma, nsq):
+    from nnpengine_amd.engine import Engine
+    sysm = water_c0
+    ft, ut = S.lj_table(sysm, RC, NBIN, S.WATER_LJ)
+    data = _data(sysm, 11, nsq)
+    print(f"[rmdf] pair table gamma={gamma} nsq={nsq} natom={sysm.natom}", flush=True)
+    e = Engine(sysm)
+    e.neigh_init(RC, SKIN, RC, 0.0)
+    e.build()
+    e.pair_table(RC, ft, ut)
+    e.gr_init(NBIN, RC, RC)
+    e.gr_tally(False)
+    e.sq_init(NQ, 0.0, QMAX)
+    e.sq_compute()
+    e.energy_clear()
+    e.rmdf_init(data, 2.0, gamma, target=0)
+    o = pyoracle.Oracle(sysm, 1)
+    o.neigh_init(RC, SKIN, RC)
+    o.rebuild(False)
+    o.pair_set(RC, ft, ut)
+    o.gr_init(NBIN, RC, RC)
+    o.gr_tally(False)
+    o.rmdf_init(data, QMAX, 2.0, gamma, target=0)
+    for it in range(2):
+        e.rmdf_step(potential=True)
+        erg, sq_o, ds_o, fq_o = o.rmdf_step(QMAX, potential=True)
+        sq_g, ds_g, fq_g = e.rmdf_get()
+        assert _scale_err(sq_g, sq_o) < 1e-5, it
+        assert _scale_err(ds_g, ds_o) < 1e-5, it
+        assert _scale_err(fq_g, fq_o) < 1e-5, it
+        assert e.energy()["rmdf"][0] == pytest.approx(erg, rel=1e-5)
+    f_g, u_g = e.rmdf_tables(NBIN)
+    f_o, u_o = o.rmdf_tables((sysm.npair, NBIN + 1))
+    assert rel_rms(f_g[:, :NBIN], f_o[:, :NBIN]) < 1e-5
+    assert rel_rms(u_g[:, :NBIN], u_o[:, :NBIN]) < 1e-5
+    base_f = np.asarray(ft, np.float64).reshape(sysm.npair, -1)
+    assert _scale_err(f_g[:, 100:NBIN] - base_f[:, 100:NBIN], f_o[:, 100:NBIN] - base_f[:, 100:NBIN]) < 1e-4
+
+
+def test_rmdf_step_nnp_tables(water_c0):
+    from nnpengine_amd.engine import Engine
+    sysm = water_c0
+    rc, nbin = 6.0, 300
+    coef = M.tbl_coefficients(sysm.npair, 3, rc)
+    data = _data(sysm, 12)
+    print(f"[rmdf] NNP tables natom={sysm.natom}", flush=True)
+    e = Engine(sysm)
+    e.neigh_init(rc, SKIN, rc, 0.0)
+    e.build()
+    e.nnp_tbl_set(rc, nbin, *coef)
+    e.gr_init(nbin, rc, rc)
+    e.gr_tally(False)
+    e.sq_init(NQ, 0.0, QMAX)
+    e.sq_compute()
+    e.rmdf_init(data, 1.0, 0.7, target=1)
+    e.rmdf_step(potential=True)
+    o = pyoracle.Oracle(sysm, 1)
+    o.neigh_init(rc, SKIN, rc)
+    o.rebuild(False)
+    o.nnp_tbl_init(rc, nbin, *coef)
+    o.gr_init(nbin, rc, rc)
+    o.gr_tally(False)
+    o.rmdf_init(data, QMAX, 1.0, 0.7, target=1)
+    _, _, _, fq_o = o.rmdf_step(QMAX, potential=True)
+    _, _, fq_g = e.rmdf_get()
+    assert _scale_err(fq_g, fq_o) < 1e-5
+    f_g, u_g = e.rmdf_tables(nbin)
+    f_o, u_o = o.rmdf_tables((sysm.npair, 4, nbin + 1))
+    for k in range(4):
+        assert rel_rms(f_g[:, k, 20:nbin], f_o[:, k, 20:nbin]) < 1e-5, k
+        assert rel_rms(u_g[:, k, 20:nbin], u_o[:, k, 20:nbin]) < 1e-5, k
+
+
+def _md_engine(sysm, ft, ut, data):
+    from nnpengine_amd.engine import Engine
+    e = Engine(sysm)
+    e.neigh_init(RC, SKIN, RC, 0.0)
+    e.pair_table(RC, ft, ut)
+    e.bonded_set_from_system()
+    e.build()
+    e.gr_init(NBIN, RC, RC)
+    e.sq_init(NQ, 0.0, QMAX)
+    e.rmdf_init(data, 1.0, 0.5, target=0)
+    return e
+
+
+def test_rmdf_in_md_loop(water_c0):
+    """20 steps with a refinement every 5 steps (src/run.cpp:285-298): captured graphs equal eager
+    launches bit for bit; the skin-triggered eager loop follows the FP32 oracle."""
+    sysm = water_c0
+    ft, ut = S.lj_table(sysm, RC, NBIN, S.WATER_LJ)
+    data = _data(sysm, 13)
+    print("[rmdf] md loop graph vs eager", flush=True)
+    runs = []
+    for use_graph in (True, False):
+        e = _md_engine(sysm, ft, ut, data)
+        log = e.md_run(20, dt=0.5, rebuild_every=5, gr_every=5, sq=True, rmdf=True, log_every=5, use_graph=use_graph)
+        runs.append((e.get("x"), e.get("v"), log, e.rmdf_tables(NBIN)))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][2], runs[1][2])
+    assert np.array_equal(runs[0][3][0], runs[1][3][0])
+
+    print("[rmdf] md loop vs FP32 oracle", flush=True)
+    e = _md_engine(sysm, ft, ut, data)
+    log = e.md_run(21, dt=0.5, rebuild_every=0, gr_every=5, sq=True, rmdf=True, log_every=5, use_graph=False)
+    o = pyoracle.Oracle(sysm, 0)
+    o.neigh_init(RC, SKIN, RC)
+    o.pair_set(RC, ft, ut)
+    o.gr_init(NBIN, RC, RC)
+    o.set_dt(0.5)
+    ergs = []
+    for step in range(21):
+        if step > 0:
+            o.update_x()
+        if step == 0 or not o.isvalid():
+            o.rebuild(False)
+        else:
+            o.update_ghost_pos()
+        if step % 5 == 0:
+            if step == 0:
+                o.rmdf_init(data, QMAX, 1.0, 0.5, target=0)
+            o.gr_tally(False)
+            erg, _, _, _ = o.rmdf_step(QMAX, potential=True)
+            ergs.append(erg)
+        o.clear_f(); o.bonded_compute(False); o.pair_compute(False)
+        o.update_v()
+    assert rel_rms(e.get("x"), o.get("x")) < 1e-6
+    assert rel_rms(e.get("v"), o.get("v")) < 1e-4
+    from nnpengine_amd import _lib
+    k = _lib.TERMS.index("rmdf")
+    assert np.allclose(log[:, 2 * k], ergs, rtol=1e-4)
