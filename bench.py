@@ -34,7 +34,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_engine(sysm, par, gmax, prec):
+def make_engine(sysm, par, norm, prec):
     from nnpengine_amd import model as M
     from nnpengine_amd import system as S
     from nnpengine_amd.engine import Engine
@@ -45,7 +45,7 @@ def make_engine(sysm, par, gmax, prec):
     e.pair_table(RC_PAIR, ft, ut)
     e.bonded_set_from_system()
     e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
-    e.mlp_set(WIDTHS, par, 0.5, prec, gmax)
+    e.mlp_set(WIDTHS, par, 0.5, prec, *norm)
     e.gr_init(450, R_GR, RC_PAIR)
     e.sq_init(1000, 0.3, 20.0)
     return e
@@ -64,7 +64,7 @@ def time_steps(e, steps, warmup):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(sysm, par):
+def cpu_baseline(sysm, par, norm):
     """The oracle's float32 restatement of the same step on the host (one thread), on a bounded
     sample: 2 steps of the 24k-atom workload."""
     from nnpengine_amd import model as M
@@ -76,7 +76,7 @@ def cpu_baseline(sysm, par):
     o.neigh_init(RC_PAIR, SKIN, R_GR)
     o.pair_set(RC_PAIR, ft, ut)
     o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
-    o.mlp_set(WIDTHS, par, 0.5)
+    o.mlp_set(WIDTHS, par, 0.5, *norm)
     o.gr_init(450, R_GR, RC_PAIR)
     o.set_dt(DT)
     nsteps = 2
@@ -110,7 +110,7 @@ def cpu_baseline(sysm, par):
                       f"charged at 1/10 per step; oracle float32 restatement, 1 thread"}
 
 
-def force_errors(sysm, par, gmax, e32, e16):
+def force_errors(sysm, par, norm):
     """NNP and total force relative RMS vs the FP64 oracle on the starting snapshot."""
     from nnpengine_amd import model as M
     from nnpengine_amd import system as S
@@ -123,7 +123,7 @@ def force_errors(sysm, par, gmax, e32, e16):
     o.rebuild(True)
     o.pair_set(RC_PAIR, ft, ut)
     o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
-    o.mlp_set(WIDTHS, par, 0.5)
+    o.mlp_set(WIDTHS, par, 0.5, *norm)
     o.clear_f()
     G = o.sf_descriptor()
     _, _, dG = o.mlp_eval(G)
@@ -137,8 +137,8 @@ def force_errors(sysm, par, gmax, e32, e16):
         return float(np.sqrt(((a - b) ** 2).mean() / (b ** 2).mean()))
 
     out = {}
-    for name in ("fp32", "bf16"):
-        e = make_engine(sysm, par, gmax, name)
+    for name in ("fp32", "bf16", "fp16"):
+        e = make_engine(sysm, par, norm, name)
         e.build()
         e.sf_descriptor()
         e.clear_force()
@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-err", action="store_true")
+    ap.add_argument("--lat-samples", type=int, default=200, help="single-step replays for the latency percentiles")
+    ap.add_argument("--stage-reps", type=int, default=20, help="launches per stage in the stage profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is "
@@ -180,16 +182,15 @@ def main():
     o.rebuild(True)
     o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
     G0 = o.sf_descriptor()
-    mean, std = M.standardisation(G0, sysm.typ, sysm.ntype)
-    par = M.fold_standardisation(M.mlp_init(WIDTHS, sysm.ntype, 2024), WIDTHS, sysm.ntype, mean, std)
-    gmax = float(np.abs(G0).max())
+    norm = M.standardisation(G0, sysm.typ, sysm.ntype)
+    par = M.mlp_init(WIDTHS, sysm.ntype, 2024)
     del o
-    log(f"[bench] system {sysm.natom} atoms, descriptor max {gmax:.3g}")
+    log(f"[bench] system {sysm.natom} atoms")
 
     res = {}
     engines = {}
     for prec in ("bf16", "fp32"):
-        e = make_engine(sysm, par, gmax, prec)
+        e = make_engine(sysm, par, norm, prec)
         t = time_steps(e, args.steps, args.warmup)
         res[prec] = t
         engines[prec] = e
@@ -199,7 +200,7 @@ def main():
     # per-step latency distribution of the BF16 form (events around single graph replays)
     eb = engines["bf16"]
     lat = []
-    for _ in range(200):
+    for _ in range(args.lat_samples):
         t0 = time.perf_counter()
         eb.md_run(1, log_every=0, **MD_KW)
         lat.append((time.perf_counter() - t0) * 1e3)
@@ -207,9 +208,9 @@ def main():
 
     # per-stage profile (HIP events on the engine stream) of the BF16 form
     stages = {}
-    for st in ("pair", "bonded", "sf_desc", "mlp", "sf_adjoint", "sf_force", "rebuild", "gr_full", "verlet", "sq"):
-        stages[st] = eb.profile(st, reps=20)
-    ms32_mlp = engines["fp32"].profile("mlp", reps=20)
+    for st in ("pair", "bonded", "sf_desc", "mlp", "sf_adjoint", "sf_force", "rebuild", "gr_full", "verlet", "sq", "copy"):
+        stages[st] = eb.profile(st, reps=args.stage_reps)
+    ms32_mlp = engines["fp32"].profile("mlp", reps=args.stage_reps)
     log("[bench] stage ms:", {k: round(v, 4) for k, v in stages.items()}, "fp32 mlp", round(ms32_mlp, 4))
 
     # roofline of the dominant kernel
@@ -237,8 +238,8 @@ def main():
                 "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
                 "flop_per_atom": flop_atom}
 
-    errs = None if args.no_err else force_errors(sysm, par, gmax, None, None)
-    cpu = None if args.no_cpu else cpu_baseline(sysm, par)
+    errs = None if args.no_err else force_errors(sysm, par, norm)
+    cpu = None if args.no_cpu else cpu_baseline(sysm, par, norm)
 
     t_b, t_f = res["bf16"], res["fp32"]
     value = N * args.steps / t_b

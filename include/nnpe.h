@@ -163,11 +163,12 @@ int nnpe_nnp_set_dedg(nnpe_ctx* ctx, const float* dG, int D);
 
 /* Per-element MLP replacing TFCALL::evaluate (src/pair_nnp.cpp:92-97).  widths[nlayer+1]
  * (input D, hidden widths multiples of 32, output 1); params per element, per layer:
- * W [in][out] row-major then b [out].  E_i = out_scale * network(G_i).
- * precision 0 = FP32, 1 = BF16, 2 = FP16 (MFMA, FP32 accumulation, per-layer scaling);
- * in_absmax = largest |G| expected (FP16 input scaling; <= 0 for none). */
+ * W [in][out] row-major then b [out].  E_i = out_scale * network((G_i - in_shift) * in_scale)
+ * with the input standardisation [ntype][D] applied in FP32 before any 16-bit rounding
+ * (NULL = identity).  precision 0 = FP32, 1 = BF16, 2 = FP16 (MFMA, FP32 accumulation,
+ * per-layer power-of-two scaling in the FP16 form). */
 int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const float* params, float out_scale,
-                 int precision, float in_absmax);
+                 int precision, const float* in_shift, const float* in_scale);
 /* nnEval: descriptors -> per-atom energy and dE/dG; energy sum in NNPE_TERM_NN when ev. */
 int nnpe_mlp_eval(nnpe_ctx* ctx, int ev);
 

@@ -63,7 +63,7 @@ SIGNATURES = [
     ("nnpe_sf_force", _I, [_P, _I]),
     ("nnpe_nnp_get", _I, [_P, _I, _PF, _PI]),
     ("nnpe_nnp_set_dedg", _I, [_P, _PF, _I]),
-    ("nnpe_mlp_set", _I, [_P, _I, _PI, _PF, _F, _I, _F]),
+    ("nnpe_mlp_set", _I, [_P, _I, _PI, _PF, _F, _I, _PF, _PF]),
     ("nnpe_mlp_eval", _I, [_P, _I]),
     ("nnpe_md_run", _I, [_P, _P, _I, _PD, _I, _PI]),
     ("nnpe_md_stats", _I, [_P, _PLL]),

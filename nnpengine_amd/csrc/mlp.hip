@@ -15,9 +15,12 @@
 // layouts: [out][in] for the forward products, [in][out] for the backward ones, so each lane's
 // B fragment is one contiguous 16-byte load.
 // Precision forms: FP32 (fp32 storage, fp32 MFMA), BF16 and FP16 (16-bit storage of
-// activations and weights, fp32 accumulation, bias/tanh/epilogues in fp32).  Per-layer
-// power-of-two scales keep FP16 operands in range: input scale s_g, weight scale s_w[l],
-// gradient scale s_d[l]; each accumulator is unscaled in fp32 before use.
+// activations and weights, fp32 accumulation, bias/tanh/epilogues in fp32).  The descriptor
+// standardisation x = (G - shift) * scale runs in fp32 while the rows are staged, so 16-bit
+// rounding applies to O(1) standardised inputs and never to raw descriptors whose spread is
+// small against their mean (folding the standardisation into layer-1 weights would cancel
+// catastrophically there).  Per-layer power-of-two scales keep FP16 operands in range:
+// weight scale s_w[l], gradient scale s_d[l]; each accumulator is unscaled in fp32 before use.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -46,9 +49,11 @@ struct MlpDev {
   const void* W[kMaxType][kMaxLayers];   // [in][out] storage type
   const float* b[kMaxType][kMaxLayers];
   const float* wout[kMaxType];
+  const float* shift[kMaxType];  // input standardisation [D] per element
+  const float* scale[kMaxType];
   float bout[kMaxType];
   float out_scale;
-  float s_g, s_w[kMaxLayers], s_d[kMaxLayers];
+  float s_w[kMaxLayers], s_d[kMaxLayers];
 };
 
 template <int KIND> struct Prec;
@@ -117,11 +122,13 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
   T* Gs = lds + P.slab_off[0];
   const int ldg = P.ld[0];
 
-  // stage the descriptor rows (scaled), zero padding rows/cols
+  // stage the descriptor rows standardised in fp32, zero padding rows/cols
+  const float* shv = P.shift[e];
+  const float* scv = P.scale[e];
   for (int idx = lane; idx < 32 * Dp; idx += 64) {
     const int r = idx / Dp, c = idx - r * Dp;
     float v = 0.0f;
-    if (r < nrows && c < D) v = G[(size_t)rows[r] * D + c] * P.s_g;
+    if (r < nrows && c < D) v = (G[(size_t)rows[r] * D + c] - shv[c]) * scv[c];
     Gs[r * ldg + c] = Prec<KIND>::cvt(v);
   }
   __syncthreads();
@@ -135,7 +142,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     const int ldd = P.ld[l + 1];
     const T* wt = reinterpret_cast<const T*>(P.Wt[e][l]);
     const float* bias = P.b[e][l];
-    const float un = 1.0f / ((l == 0 ? P.s_g : 1.0f) * P.s_w[l]);
+    const float un = 1.0f / P.s_w[l];
     for (int cb = 0; cb < P.H[l] / 32; ++cb) {
       f32x16 acc = gemm_block<KIND>(src, lds_src, wt, K, cb, lane);
       const int col = cb * 32 + (lane & 31);
@@ -193,7 +200,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     }
     __syncthreads();
   }
-  // dE/dG = d_1 W_1^T: A = d_1 [32 x H_0], B = W_0 [Dp][H_0]
+  // dE/dG = (d_1 W_1^T) * scale: A = d_1 [32 x H_0], B = W_0 [Dp][H_0]
   {
     const T* d1 = lds + P.slab_off[1];
     const T* w = reinterpret_cast<const T*>(P.W[e][0]);
@@ -202,10 +209,11 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
       f32x16 acc = gemm_block<KIND>(d1, P.ld[1], w, P.H[0], cb, lane);
       const int col = cb * 32 + (lane & 31);
       if (col < D) {
+        const float sc = scv[col] * un;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int r = crow(lane, j);
-          if (r < nrows) dG[(size_t)rows[r] * D + col] = acc[j] * un;
+          if (r < nrows) dG[(size_t)rows[r] * D + col] = acc[j] * sc;
         }
       }
     }
@@ -267,7 +275,7 @@ int type_index_enqueue(nnpe_ctx* ctx) {
 }
 
 extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const float* params, float out_scale,
-                            int precision, float in_absmax) {
+                            int precision, const float* in_shift, const float* in_scale) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
   if (!ctx || nlayer < 2 || nlayer > kMaxLayers || !widths || !params || precision < 0 || precision > 2)
     return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_mlp_set: bad argument (2..6 weight layers, precision 0/1/2)");
@@ -328,10 +336,8 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
     pofs = o;
   }
   // per-layer scales (FP16 form only; BF16 and FP32 have the FP32 exponent range)
-  d.s_g = 1.0f;
   for (int l = 0; l < kMaxLayers; ++l) { d.s_w[l] = 1.0f; d.s_d[l] = 1.0f; }
   if (precision == 2) {
-    d.s_g = in_absmax > 0 ? pow2_scale(in_absmax) : 1.0f;
     for (int l = 0; l < d.nh; ++l) d.s_w[l] = pow2_scale(maxw[l]);
     // gradient bounds: |d_L| <= s_out max|w_out|, |d_{l-1}| <= |d_l| * max_i sum_j |W_l[i][j]|
     double bnd = std::fabs(out_scale) * maxw[nlayer - 1];
@@ -395,6 +401,21 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
       NNPE_CHECK_HIP(ctx, hipMemcpy(db, b, sizeof(float) * out, hipMemcpyHostToDevice));
       d.b[t][l] = db;
     }
+  }
+  // input standardisation, identity where not given
+  for (int t = 0; t < ctx->ntype; ++t) {
+    std::vector<float> sh(d.D, 0.0f), sc(d.D, 1.0f);
+    if (in_shift) std::copy(in_shift + size_t(t) * d.D, in_shift + size_t(t + 1) * d.D, sh.begin());
+    if (in_scale) std::copy(in_scale + size_t(t) * d.D, in_scale + size_t(t + 1) * d.D, sc.begin());
+    for (int k = 0; k < d.D; ++k)
+      if (!std::isfinite(sh[k]) || !std::isfinite(sc[k])) return ctx_fail(ctx, NNPE_ERR_ARG, "non-finite input standardisation");
+    float* p = nullptr;
+    NNPE_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&p), sizeof(float) * 2 * d.D));
+    S->bufs.push_back(p);
+    NNPE_CHECK_HIP(ctx, hipMemcpy(p, sh.data(), sizeof(float) * d.D, hipMemcpyHostToDevice));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(p + d.D, sc.data(), sizeof(float) * d.D, hipMemcpyHostToDevice));
+    d.shift[t] = p;
+    d.scale[t] = p + d.D;
   }
   NNPE_CHECK_HIP(ctx, dev_alloc(S->d_type_off, ctx->ntype));
   NNPE_CHECK_HIP(ctx, hipMemcpy(S->d_type_off, ctx->h_type_off.data(), sizeof(int) * ctx->ntype, hipMemcpyHostToDevice));

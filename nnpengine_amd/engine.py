@@ -244,11 +244,15 @@ class Engine:
         d = _f32(dG)
         self._check(self.L.nnpe_nnp_set_dedg(self.h, _p(d, C.c_float), d.shape[1]))
 
-    def mlp_set(self, widths, params, out_scale=1.0, precision="fp32", in_absmax=0.0):
+    def mlp_set(self, widths, params, out_scale=1.0, precision="fp32", shift=None, scale=None):
+        """shift, scale: [ntype][D] input standardisation x = (G - shift) * scale (None: identity)."""
         prec = {"fp32": 0, "bf16": 1, "fp16": 2}[precision]
-        self._mlp = (_i32(widths), _f32(params))
+        sh = None if shift is None else _f32(shift)
+        sc = None if scale is None else _f32(scale)
+        self._mlp = (_i32(widths), _f32(params), sh, sc)
         self._check(self.L.nnpe_mlp_set(self.h, len(widths) - 1, _p(self._mlp[0], C.c_int), _p(self._mlp[1], C.c_float),
-                                        C.c_float(out_scale), prec, C.c_float(in_absmax)))
+                                        C.c_float(out_scale), prec, None if sh is None else _p(sh, C.c_float),
+                                        None if sc is None else _p(sc, C.c_float)))
 
     def mlp_eval(self, ev=True):
         self._check(self.L.nnpe_mlp_eval(self.h, int(ev)))

@@ -407,7 +407,7 @@ NNPOTENTIAL_TBL::NNPOTENTIAL_TBL(numtype cf, int nbr, const char* paramfile, con
   std::fclose(fp);
   bnd->autoGenerate(); agl->autoGenerate(); dih->autoGenerate();
   check(nnpe_nnp_tbl_set(nnpe_host::ctx(), cf, nbr, cp.data(), cw.data(), cd.data(), cr.data()), "NNPOTENTIAL_TBL");
-  check(nnpe_mlp_set(nnpe_host::ctx(), nl, widths.data(), par.data(), out_scale, p, 0.0f), "NNPOTENTIAL_TBL");
+  check(nnpe_mlp_set(nnpe_host::ctx(), nl, widths.data(), par.data(), out_scale, p, nullptr, nullptr), "NNPOTENTIAL_TBL");
 }
 
 void NNPOTENTIAL_TBL::computeDescriptor(NEIGHLIST*) { check(nnpe_nnp_tbl_descriptor(nnpe_host::ctx()), "computeDescriptor"); }

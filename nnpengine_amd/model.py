@@ -1,5 +1,5 @@
 """NNP model parameters: symmetry-function settings, seeded per-element MLP weights, the
-descriptor standardisation folded into the first layer, and the reference's NNPOTENTIAL_TBL
+per-element descriptor standardisation of the network input, and the reference's NNPOTENTIAL_TBL
 coefficient arrays (the layout its weight file holds, src/pair_nnp.cpp:505-517)."""
 from __future__ import annotations
 
@@ -36,33 +36,23 @@ def mlp_init(widths, ntype, seed):
     return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
 
 
-def fold_standardisation(params, widths, ntype, mean, std):
-    """Network input x = (G - mean) / std folded into layer 1: W1' = W1 / std, b1' = b1 - (mean/std) W1.
-    mean, std: [ntype, D]."""
-    out = params.astype(np.float64).copy()
-    o = 0
-    for t in range(ntype):
-        for l in range(len(widths) - 1):
-            fi, fo = widths[l], widths[l + 1]
-            W = out[o:o + fi * fo].reshape(fi, fo)
-            b = out[o + fi * fo:o + fi * fo + fo]
-            if l == 0:
-                s = np.where(std[t] > 1e-6, std[t], 1.0)
-                b -= (mean[t] / s) @ W
-                W /= s[:, None]
-            o += fi * fo + fo
-    return out.astype(np.float32)
-
-
-def standardisation(G, typ, ntype):
-    mean = np.zeros((ntype, G.shape[1]))
-    std = np.ones((ntype, G.shape[1]))
+def standardisation(G, typ, ntype, floor=0.1):
+    """Per-element input standardisation x = (G - shift) * scale of the network: shift = mean,
+    scale = 1 / max(std, floor * rms).  The floor keeps components whose spread is tiny against
+    their magnitude (e.g. on a lattice snapshot) from amplifying descriptor rounding.
+    Returns shift, scale as [ntype, D] float32."""
+    D = G.shape[1]
+    shift = np.zeros((ntype, D))
+    scale = np.ones((ntype, D))
     for t in range(ntype):
         g = G[typ == t].astype(np.float64)
         if len(g):
-            mean[t] = g.mean(0)
-            std[t] = g.std(0)
-    return mean, std
+            m = g.mean(0)
+            sd = g.std(0)
+            rms = np.sqrt(m * m + sd * sd)
+            shift[t] = m
+            scale[t] = 1.0 / np.maximum(np.maximum(sd, floor * rms), 1e-12)
+    return shift.astype(np.float32), scale.astype(np.float32)
 
 
 def tbl_coefficients(npair, seed, cutoff):

@@ -747,16 +747,23 @@ struct Eng {
   std::vector<double> mlp_par;
   std::vector<size_t> mlp_off;  // per (type, layer): offset of W; b follows W
   double mlp_scale = 1;
-  void mlp_set(int nl, const int* widths, const float* par, float out_scale) {
+  std::vector<double> mlp_shift, mlp_inscale;  // input standardisation x = (G - shift) * scale, [ntype][D]
+  void mlp_set(int nl, const int* widths, const float* par, float out_scale, const float* shift, const float* scale) {
     mlp_nl = nl; mlp_w.assign(widths, widths + nl + 1); mlp_scale = out_scale;
     size_t o = 0; mlp_off.clear();
     for (int t = 0; t < ntype; ++t) for (int l = 0; l < nl; ++l) { mlp_off.push_back(o); o += size_t(mlp_w[l]) * mlp_w[l + 1] + mlp_w[l + 1]; }
     mlp_par.assign(par, par + o);
+    const size_t nd = size_t(ntype) * widths[0];
+    mlp_shift.assign(nd, 0.0); mlp_inscale.assign(nd, 1.0);
+    if (shift) for (size_t k = 0; k < nd; ++k) mlp_shift[k] = shift[k];
+    if (scale) for (size_t k = 0; k < nd; ++k) mlp_inscale[k] = scale[k];
   }
   // forward + input-gradient backward in double for one descriptor row
   double mlp_eval_row(int t, const double* g, double* dg) const {
     std::vector<std::vector<double>> act(mlp_nl + 1);
-    act[0].assign(g, g + mlp_w[0]);
+    const int D0 = mlp_w[0];
+    act[0].resize(D0);
+    for (int k = 0; k < D0; ++k) act[0][k] = (g[k] - mlp_shift[size_t(t) * D0 + k]) * mlp_inscale[size_t(t) * D0 + k];
     for (int l = 0; l < mlp_nl; ++l) {
       const double* W = &mlp_par[mlp_off[size_t(t) * mlp_nl + l]];
       const double* b = W + size_t(mlp_w[l]) * mlp_w[l + 1];
@@ -779,7 +786,7 @@ struct Eng {
       }
       delta.swap(prev);
     }
-    for (int k = 0; k < mlp_w[0]; ++k) dg[k] = delta[k];
+    for (int k = 0; k < D0; ++k) dg[k] = delta[k] * mlp_inscale[size_t(t) * D0 + k];
     return e;
   }
   // descriptor rows are indexed by atom id; returns total NN energy
@@ -1171,7 +1178,11 @@ int orc_nnp_tbl_set_dedg(OrcHandle* h, const double* dedg) {
 }
 int orc_nnp_tbl_force(OrcHandle* h, int ev, double* erg, double* vir) { ORC_DISPATCH(h, nnp_tbl_force(ev != 0, erg, vir)); return 0; }
 
-int orc_mlp_set(OrcHandle* h, int nl, const int* widths, const float* par, float out_scale) { ORC_DISPATCH(h, mlp_set(nl, widths, par, out_scale)); return 0; }
+int orc_mlp_set(OrcHandle* h, int nl, const int* widths, const float* par, float out_scale, const float* shift,
+                const float* scale) {
+  ORC_DISPATCH(h, mlp_set(nl, widths, par, out_scale, shift, scale));
+  return 0;
+}
 // evaluate the MLP on descriptor rows G[natom][D] (by atom id); writes dE/dG and per-atom E
 double orc_mlp_eval(OrcHandle* h, int D, const double* G, double* dG, double* e_atom) {
   double etot = 0;

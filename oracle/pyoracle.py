@@ -244,11 +244,15 @@ class Oracle:
         return e.value, w.value
 
     # MLP ---------------------------------------------------------------------------------
-    def mlp_set(self, widths, params, out_scale=1.0):
+    def mlp_set(self, widths, params, out_scale=1.0, shift=None, scale=None):
+        """shift, scale: [ntype][D] input standardisation x = (G - shift) * scale (None: identity)."""
         self._w = _i32(widths)
         self._par = _f32(params)
+        self._sh = None if shift is None else _f32(shift)
+        self._sc = None if scale is None else _f32(scale)
         self.L.orc_mlp_set(self.h, len(widths) - 1, _p(self._w, C.c_int), _p(self._par, C.c_float),
-                           C.c_float(out_scale))
+                           C.c_float(out_scale), None if self._sh is None else _p(self._sh, C.c_float),
+                           None if self._sc is None else _p(self._sc, C.c_float))
 
     def mlp_eval(self, G):
         G = _f64(G)

@@ -11,7 +11,9 @@ from tests.conftest import rel_rms
 
 pytestmark = pytest.mark.gpu
 
-RC_PAIR, SKIN, RC_NNP = 9.0, 1.0, 6.0
+# the 768-atom box (L = 19.73 A) needs skin + rc < L/2 = 9.865 A
+RC_PAIR, SKIN, RC_NNP = 9.0, 0.5, 6.0
+MD_SKIN = 0.8
 
 
 def _setup(sysm, widths, seed=11, out_scale=0.5):
@@ -23,26 +25,26 @@ def _setup(sysm, widths, seed=11, out_scale=0.5):
     o.rebuild(True)
     D = o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
     G64 = o.sf_descriptor()
-    mean, std = M.standardisation(G64, sysm.typ, sysm.ntype)
-    par = M.fold_standardisation(M.mlp_init(widths, sysm.ntype, seed), widths, sysm.ntype, mean, std)
-    o.mlp_set(widths, par, out_scale)
+    norm = M.standardisation(G64, sysm.typ, sysm.ntype)
+    par = M.mlp_init(widths, sysm.ntype, seed)
+    o.mlp_set(widths, par, out_scale, *norm)
     e = Engine(sysm)
     e.neigh_init(RC_PAIR, SKIN, RC_PAIR, RC_NNP)
     e.build()
     e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
     assert D == M.sf_dim(sysm.ntype)
-    return e, o, par, G64, float(np.abs(G64).max())
+    return e, o, par, G64, norm
 
 
 @pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3)])
 def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
     widths = [80, 64, 64, 1]
-    e, o, par, G64, gmax = _setup(water_c0, widths)
+    e, o, par, G64, norm = _setup(water_c0, widths)
     e.sf_descriptor()
     G = e.nnp_get("G").astype(np.float64)
     print(f"[mlp] prec={prec} descriptor rel-rms vs fp64 = {rel_rms(G, G64):.2e}", flush=True)
     assert rel_rms(G, G64) < 1e-5
-    e.mlp_set(widths, par, 0.5, prec, gmax)
+    e.mlp_set(widths, par, 0.5, prec, *norm)
     e.energy_clear()
     e.mlp_eval(True)
     ea = e.nnp_get("e_atom").astype(np.float64)
@@ -57,8 +59,8 @@ def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
 
 def test_sf_forces_fp32(water_c0):
     widths = [80, 64, 64, 1]
-    e, o, par, G64, gmax = _setup(water_c0, widths)
-    e.mlp_set(widths, par, 0.5, "fp32", gmax)
+    e, o, par, G64, norm = _setup(water_c0, widths)
+    e.mlp_set(widths, par, 0.5, "fp32", *norm)
     e.sf_descriptor()
     e.clear_force()
     e.energy_clear()
@@ -81,19 +83,30 @@ def test_sf_forces_fp32(water_c0):
 def test_sf_forces_bf16_error(water_c1):
     """BF16 MFMA network: NNP force error vs FP64 within 1e-2 relative RMS (24k atoms, 3x128)."""
     widths = [80, 128, 128, 128, 1]
-    e, o, par, G64, gmax = _setup(water_c1, widths)
+    e, o, par, G64, norm = _setup(water_c1, widths)
     _, _, dG64 = o.mlp_eval(G64)
     o.clear_f()
     o.sf_force(dG64)
     f64 = o.get("f")
+    # force kernel alone on the oracle's dE/dG
+    e.mlp_set(widths, par, 0.5, "fp32", *norm)
+    e.sf_descriptor()
+    G = e.nnp_get("G").astype(np.float64)
+    print(f"[sf 24k] descriptor rel-rms vs fp64 {rel_rms(G, G64):.2e}", flush=True)
+    e.nnp_set_dedg(dG64.astype(np.float32))
+    e.clear_force()
+    e.sf_force(False)
+    print(f"[sf 24k] force kernel on fp64 dE/dG: rel-rms {rel_rms(e.get('f'), f64):.2e}", flush=True)
     errs = {}
     for prec in ("fp32", "bf16", "fp16"):
-        e.mlp_set(widths, par, 0.5, prec, gmax)
+        e.mlp_set(widths, par, 0.5, prec, *norm)
         e.sf_descriptor()
         e.clear_force()
         e.mlp_eval(False)
+        dG = e.nnp_get("dG").astype(np.float64)
         e.sf_force(False)
         errs[prec] = rel_rms(e.get("f"), f64)
+        print(f"[sf 24k] {prec}: dE/dG rel-rms {rel_rms(dG, dG64):.2e} force rel-rms {errs[prec]:.2e}", flush=True)
     print("[sf 24k] NNP force rel-rms vs fp64:", errs)
     assert errs["fp32"] < 1e-5
     assert errs["bf16"] < 1e-2
@@ -149,11 +162,11 @@ def test_md_graph_equals_eager(water_c0):
     xs = []
     for graph in (False, True):
         e = Engine(sysm)
-        e.neigh_init(RC_PAIR, 2.0, RC_PAIR, RC_NNP)
+        e.neigh_init(RC_PAIR, MD_SKIN, RC_PAIR, RC_NNP)
         e.pair_table(RC_PAIR, ft, ut)
         e.bonded_set_from_system()
         e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
-        e.mlp_set(widths, par, 0.5, "bf16", 50.0)
+        e.mlp_set(widths, par, 0.5, "bf16")
         e.gr_init(450, RC_PAIR, RC_PAIR)
         log = e.md_run(40, dt=0.5, rebuild_every=10, nnp=2, gr_every=10, log_every=10, use_graph=graph)
         assert log.shape[0] == 4
@@ -168,10 +181,10 @@ def test_md_nve_energy_conservation(water_c0):
     sysm = water_c0
     ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
     e = Engine(sysm)
-    e.neigh_init(RC_PAIR, 2.0, RC_PAIR)
+    e.neigh_init(RC_PAIR, MD_SKIN, RC_PAIR)
     e.pair_table(RC_PAIR, ft, ut)
     e.bonded_set_from_system()
-    log = e.md_run(400, dt=0.5, fp64=True, rebuild_every=10, log_every=20)
+    log = e.md_run(400, dt=0.5, fp64=True, rebuild_every=4, log_every=20)
     et = log[:, 0] + log[:, 2] + log[:, 4] + log[:, 6] + log[:, 14]
     print("[nve] E_tot first/last", et[1], et[-1], "ke", log[1, 14], log[-1, 14])
     st = e.md_stats()

@@ -282,8 +282,7 @@ def test_sf_force_finite_difference():
     widths = [80, 32, 32, 1]
     o = _sf_oracle(sysm)
     G0 = o.sf_descriptor()
-    mean, std = M.standardisation(G0, sysm.typ, sysm.ntype)
-    o.mlp_set(widths, M.fold_standardisation(M.mlp_init(widths, 2, 3), widths, 2, mean, std), 0.7)
+    o.mlp_set(widths, M.mlp_init(widths, 2, 3), 0.7, *M.standardisation(G0, sysm.typ, sysm.ntype))
     x0 = o.get("x")
 
     def energy(xx):
@@ -329,9 +328,11 @@ def test_mlp_backward_matches_torch_autograd():
     sysm = make_system("water_small")
     widths = [80, 64, 64, 64, 1]
     par = M.mlp_init(widths, 2, 5)
-    o = pyoracle.Oracle(sysm, 1)
-    o.mlp_set(widths, par, 0.3)
     rng = np.random.default_rng(0)
+    shift = rng.standard_normal((2, 80)).astype(np.float32)
+    scale = (0.5 + rng.random((2, 80))).astype(np.float32)
+    o = pyoracle.Oracle(sysm, 1)
+    o.mlp_set(widths, par, 0.3, shift, scale)
     G = rng.standard_normal((sysm.natom, 80))
     et, ea, dG = o.mlp_eval(G)
     # independent torch FP64 evaluation
@@ -348,9 +349,11 @@ def test_mlp_backward_matches_torch_autograd():
     E = torch.zeros(sysm.natom, dtype=torch.float64)
     typ = torch.tensor(sysm.typ)
     out = []
+    sh = torch.tensor(shift, dtype=torch.float64)
+    sc = torch.tensor(scale, dtype=torch.float64)
     for i in range(sysm.natom):
-        a = g[i]
         t = int(typ[i])
+        a = (g[i] - sh[t]) * sc[t]
         for l in range(len(widths) - 1):
             W, b = layers[(t, l)]
             a = a @ W + b
