@@ -120,8 +120,14 @@ def force_errors(sysm, par, norm):
     sfp = M.sf_params(RC_NNP)
     o = pyoracle.Oracle(sysm, 1)
     o.neigh_init(RC_PAIR, SKIN, R_GR)
-    o.rebuild(True)
+    # classical terms on the reference's half list, NNP terms on the full list
+    o.rebuild(False)
     o.pair_set(RC_PAIR, ft, ut)
+    o.clear_f()
+    o.bonded_compute()
+    o.pair_compute()
+    f_cls64 = o.get("f")
+    o.rebuild(True)
     o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
     o.mlp_set(WIDTHS, par, 0.5, *norm)
     o.clear_f()
@@ -129,9 +135,7 @@ def force_errors(sysm, par, norm):
     _, _, dG = o.mlp_eval(G)
     o.sf_force(dG)
     f_nnp64 = o.get("f")
-    o.bonded_compute()
-    o.pair_compute()
-    f_tot64 = o.get("f")
+    f_tot64 = f_cls64 + f_nnp64
 
     def rr(a, b):
         return float(np.sqrt(((a - b) ** 2).mean() / (b ** 2).mean()))
@@ -213,27 +217,38 @@ def main():
     ms32_mlp = engines["fp32"].profile("mlp", reps=args.stage_reps)
     log("[bench] stage ms:", {k: round(v, 4) for k, v in stages.items()}, "fp32 mlp", round(ms32_mlp, 4))
 
-    # roofline of the dominant kernel
+    # roofline of the dominant kernel: the largest share of the step (stage time x launches per
+    # step; rebuild and the full g(r) tally run every 10th step)
     n_long, n_ghost = eb.list_size(0)
     n_short, _ = eb.list_size(1)
     N = sysm.natom
     flop_atom = 4 * sum(WIDTHS[i] * WIDTHS[i + 1] for i in range(len(WIDTHS) - 1))
-    dom = max(("pair", "sf_desc", "mlp", "sf_adjoint", "sf_force", "gr_full"), key=lambda k: stages[k])
+    per_step = {"pair": 1.0, "bonded": 1.0, "sf_desc": 1.0, "mlp": 1.0, "sf_adjoint": 1.0, "sf_force": 1.0,
+                "verlet": 1.0, "rebuild": 0.1, "gr_full": 0.1, "sq": 0.1}
+    share = {k: stages[k] * f for k, f in per_step.items()}
+    dom = max(share, key=share.get)
     nmt = sysm.ntype * 4 * 11 + sysm.ntype * 16
+    D = WIDTHS[0]
+    # algorithmic HBM bytes per launch: each input read once, each output written once
+    # (list entries 4 B; xq 16 B, typ 4 B, f 16 B per atom; moments/adjoints 4*nmt B; G/dG 4*D B)
+    alg_bytes = {"pair": 4 * n_long + N * (16 + 4 + 16),
+                 "sf_desc": 4 * n_short + N * (16 + 4 + 4 * nmt + 4 * D),
+                 "sf_adjoint": N * 4 * (nmt + D + nmt),
+                 "sf_force": 4 * n_short + N * (16 + 4 + 4 * nmt + 16 + 16),
+                 "bonded": N * (16 + 16 + 16),
+                 "verlet": N * 164,
+                 "rebuild": 4 * (n_long + n_short) + N * 2 * (16 + 16 + 16 + 4 + 4 + 16 + 3 * 32),
+                 "gr_full": N * (16 + 4),
+                 "sq": 8 * 1000 * 600}
     if dom == "mlp":
         roof = {"kernel": "k_mlp<bf16>", "bound": "mfma", "achieved": flop_atom * N / (stages["mlp"] * 1e-3) / 1e12,
                 "peak": MFMA_PEAK["bf16"], "unit": "TFLOP/s"}
     else:
-        # algorithmic bytes per launch: list entries (4 B) + per-atom rows read/written once
-        bytes_per = {"pair": 4 * n_long + N * (16 + 4 + 4 + 16),
-                     "sf_desc": 4 * n_short + N * (16 + 4 + 4 + 4 * nmt + 4 * 80),
-                     "sf_adjoint": N * 4 * (nmt + 80 + nmt),
-                     "sf_force": 4 * n_short + N * (16 + 4 + 4 + 4 * nmt + 16),
-                     "gr_full": N * (16 + 4 + 4)}[dom]
-        roof = {"kernel": dom, "bound": "hbm", "achieved": bytes_per / (stages[dom] * 1e-3) / 1e9,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+        roof = {"kernel": dom, "bound": "hbm", "achieved": alg_bytes[dom] / (stages[dom] * 1e-3) / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": alg_bytes[dom]}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
+    roof["step_share"] = {k: round(v / sum(share.values()), 4) for k, v in share.items()}
     mlp_roof = {"achieved_tflops_bf16": flop_atom * N / (stages["mlp"] * 1e-3) / 1e12,
                 "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
                 "flop_per_atom": flop_atom}

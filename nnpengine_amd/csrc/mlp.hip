@@ -21,6 +21,12 @@
 // small against their mean (folding the standardisation into layer-1 weights would cancel
 // catastrophically there).  Per-layer power-of-two scales keep FP16 operands in range:
 // weight scale s_w[l], gradient scale s_d[l]; each accumulator is unscaled in fp32 before use.
+// The BF16 form keeps bf16 operands on every MFMA but carries the weights and the backward
+// gradients as hi + lo bf16 pairs (x = hi + lo, lo = bf16(x - hi), a 16-bit significand): forward
+// products take two MFMAs (A.W_hi + A.W_lo), backward products three (d_hi.W_hi + d_lo.W_hi +
+// d_hi.W_lo).  Activations and the standardised input stay single bf16.  Weight and gradient
+// rounding dominate the error of a plain bf16 network; the pairs remove both for 2-3x the MFMA
+// work of a layer, which the MFMA rate (16x the FP32 MFMA rate) absorbs.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -47,6 +53,9 @@ struct MlpDev {
   int type_off[kMaxType];
   const void* Wt[kMaxType][kMaxLayers];  // [out][in] storage type
   const void* W[kMaxType][kMaxLayers];   // [in][out] storage type
+  const void* Wt_lo[kMaxType][kMaxLayers];  // BF16 form: low parts of the weight pairs
+  const void* W_lo[kMaxType][kMaxLayers];
+  int lo_off[kMaxLayers + 1];    // BF16 form: LDS slabs of the gradients' low parts (layers 1..nh)
   const float* b[kMaxType][kMaxLayers];
   const float* wout[kMaxType];
   const float* shift[kMaxType];  // input standardisation [D] per element
@@ -105,6 +114,24 @@ __device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, in
   return acc;
 }
 
+// BF16 pair products: sum of A.B over the hi/lo parts given (nullptr = part absent)
+__device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf16* Alo, int lda, const __bf16* Bhi,
+                                                   const __bf16* Blo, int K, int cb, int lane) {
+  typedef Prec<1> P;
+  constexpr int KS = P::KS, HALF = KS / 2;
+  f32x16 acc;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+  const size_t ao = (size_t)(lane & 31) * lda + HALF * (lane >> 5);
+  const size_t bo = (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
+  for (int k0 = 0; k0 < K; k0 += KS) {
+    acc = P::mma(Ahi + ao + k0, Bhi + bo + k0, acc);
+    if (Blo) acc = P::mma(Ahi + ao + k0, Blo + bo + k0, acc);
+    if (Alo) acc = P::mma(Alo + ao + k0, Bhi + bo + k0, acc);
+  }
+  return acc;
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
                                            float* __restrict__ e_atom, const int* __restrict__ type_idx) {
@@ -144,7 +171,11 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     const float* bias = P.b[e][l];
     const float un = 1.0f / P.s_w[l];
     for (int cb = 0; cb < P.H[l] / 32; ++cb) {
-      f32x16 acc = gemm_block<KIND>(src, lds_src, wt, K, cb, lane);
+      f32x16 acc;
+      if constexpr (KIND == 1)
+        acc = gemm_block_pairs(src, nullptr, lds_src, wt, reinterpret_cast<const T*>(P.Wt_lo[e][l]), K, cb, lane);
+      else
+        acc = gemm_block<KIND>(src, lds_src, wt, K, cb, lane);
       const int col = cb * 32 + (lane & 31);
       const float bc = bias[col];
 #pragma unroll
@@ -175,7 +206,10 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     for (int idx = lane; idx < 32 * HL; idx += 64) {
       const int r = idx / HL, c = idx - r * HL;
       const float a = Prec<KIND>::back(dL[r * ldl + c]);
-      dL[r * ldl + c] = Prec<KIND>::cvt(sc * wo[c] * (1.0f - a * a));
+      const float v = sc * wo[c] * (1.0f - a * a);
+      const T hi = Prec<KIND>::cvt(v);
+      dL[r * ldl + c] = hi;
+      if constexpr (KIND == 1) lds[P.lo_off[nh] + r * ldl + c] = Prec<KIND>::cvt(v - Prec<KIND>::back(hi));
     }
   }
   __syncthreads();
@@ -189,13 +223,21 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     const float un = 1.0f / (P.s_d[l] * P.s_w[l]);
     const float sd = P.s_d[l - 1];
     for (int cb = 0; cb < P.H[l - 1] / 32; ++cb) {
-      f32x16 acc = gemm_block<KIND>(dl, ldl2, w, P.H[l], cb, lane);
+      f32x16 acc;
+      if constexpr (KIND == 1)
+        acc = gemm_block_pairs(dl, lds + P.lo_off[l + 1], ldl2, w, reinterpret_cast<const T*>(P.W_lo[e][l]), P.H[l], cb,
+                               lane);
+      else
+        acc = gemm_block<KIND>(dl, ldl2, w, P.H[l], cb, lane);
       const int col = cb * 32 + (lane & 31);
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         T* p = prev + crow(lane, j) * ldp + col;
         const float a = Prec<KIND>::back(*p);
-        *p = Prec<KIND>::cvt(acc[j] * un * (1.0f - a * a) * sd);
+        const float v = acc[j] * un * (1.0f - a * a) * sd;
+        const T hi = Prec<KIND>::cvt(v);
+        *p = hi;
+        if constexpr (KIND == 1) lds[P.lo_off[l] + crow(lane, j) * ldp + col] = Prec<KIND>::cvt(v - Prec<KIND>::back(hi));
       }
     }
     __syncthreads();
@@ -206,7 +248,12 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     const T* w = reinterpret_cast<const T*>(P.W[e][0]);
     const float un = 1.0f / (P.s_d[0] * P.s_w[0]);
     for (int cb = 0; cb < Dp / 32; ++cb) {
-      f32x16 acc = gemm_block<KIND>(d1, P.ld[1], w, P.H[0], cb, lane);
+      f32x16 acc;
+      if constexpr (KIND == 1)
+        acc = gemm_block_pairs(d1, lds + P.lo_off[1], P.ld[1], w, reinterpret_cast<const T*>(P.W_lo[e][0]), P.H[0], cb,
+                               lane);
+      else
+        acc = gemm_block<KIND>(d1, P.ld[1], w, P.H[0], cb, lane);
       const int col = cb * 32 + (lane & 31);
       if (col < D) {
         const float sc = scv[col] * un;
@@ -258,6 +305,19 @@ void mlp_state_free(nnpe_ctx* ctx) {
   ctx->mlp = nullptr;
 }
 
+// float -> bf16 bits, round to nearest even (finite inputs)
+static uint16_t bf16_rne(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+static float bf16_value(uint16_t b) {
+  const uint32_t u = uint32_t(b) << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
 static float pow2_scale(double maxabs) {
   if (!(maxabs > 0) || !std::isfinite(maxabs)) return 1.0f;
   return float(std::ldexp(1.0, -int(std::floor(std::log2(maxabs)))));
@@ -300,6 +360,9 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
   for (int l = 0; l < d.nh; ++l) d.ld[l + 1] = d.H[l] + pad;
   int off = 0;
   for (int s = 0; s <= d.nh; ++s) { d.slab_off[s] = off; off += 32 * d.ld[s]; }
+  for (int s = 0; s <= kMaxLayers; ++s) d.lo_off[s] = 0;
+  if (precision == 1)
+    for (int s = 1; s <= d.nh; ++s) { d.lo_off[s] = off; off += 32 * d.ld[s]; }
   S->lds_bytes = size_t(off) * esz;
   if (S->lds_bytes > 64 * 1024) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "MLP tile does not fit the 64 KiB LDS budget");
   // tiles of 32 atoms per element
@@ -368,7 +431,9 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
           wt[size_t(k) * inp + i] = W[size_t(i) * out + k] * sw;
           wn[size_t(i) * out + k] = W[size_t(i) * out + k] * sw;
         }
-      auto upload = [&](const std::vector<float>& h, const void*& dst) -> int {
+      // lo = false: the value rounded to the storage type; lo = true (BF16 form): the rounding
+      // remainder x - hi rounded to bf16
+      auto upload = [&](const std::vector<float>& h, const void*& dst, bool lo) -> int {
         void* p = nullptr;
         NNPE_CHECK_HIP(ctx, hipMalloc(&p, h.size() * esz));
         S->bufs.push_back(p);
@@ -378,9 +443,8 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
           std::vector<uint16_t> hb(h.size());
           for (size_t k = 0; k < h.size(); ++k) {
             if (precision == 1) {
-              uint32_t u; std::memcpy(&u, &h[k], 4);
-              uint32_t lsb = (u >> 16) & 1u;  // round to nearest even
-              hb[k] = uint16_t((u + 0x7FFFu + lsb) >> 16);
+              const uint16_t hi = bf16_rne(h[k]);
+              hb[k] = lo ? bf16_rne(h[k] - bf16_value(hi)) : hi;
             } else {
               _Float16 hv = (_Float16)h[k];
               std::memcpy(&hb[k], &hv, 2);
@@ -391,10 +455,16 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
         dst = p;
         return NNPE_OK;
       };
-      int rc = upload(wt, d.Wt[t][l]);
+      int rc = upload(wt, d.Wt[t][l], false);
       if (rc) return rc;
-      rc = upload(wn, d.W[t][l]);
+      rc = upload(wn, d.W[t][l], false);
       if (rc) return rc;
+      if (precision == 1) {
+        rc = upload(wt, d.Wt_lo[t][l], true);
+        if (rc) return rc;
+        rc = upload(wn, d.W_lo[t][l], true);
+        if (rc) return rc;
+      }
       float* db = nullptr;
       NNPE_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&db), sizeof(float) * out));
       S->bufs.push_back(db);

@@ -36,7 +36,7 @@ def mlp_init(widths, ntype, seed):
     return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
 
 
-def standardisation(G, typ, ntype, floor=0.1):
+def standardisation(G, typ, ntype, floor=0.5):
     """Per-element input standardisation x = (G - shift) * scale of the network: shift = mean,
     scale = 1 / max(std, floor * rms).  The floor keeps components whose spread is tiny against
     their magnitude (e.g. on a lattice snapshot) from amplifying descriptor rounding.
