@@ -171,7 +171,8 @@ extern "C" int nnpe_system_set(nnpe_ctx* ctx, int natom, int ntype, float bl, co
   NNPE_CHECK_HIP(ctx, up(ctx->ad, had.data(), N));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_xd, N)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_vd, N));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_ad, N));
-  NNPE_CHECK_HIP(ctx, ev_reserve(ctx, blocks_for(N, 64) + 64));
+  // partial sums: one per 256-thread block at up to 64 lanes per atom
+  NNPE_CHECK_HIP(ctx, ev_reserve(ctx, blocks_for((long long)N * 64, 256) + 64));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->ev_sum, 0, sizeof(double) * 2 * NNPE_NTERM));
   ctx->have_system = true;
   return NNPE_OK;

@@ -1,8 +1,8 @@
 // forces.hip -- tabulated LJ + shifted-force Coulomb pair forces and the bonded terms.
 //
-// Pair (LJTableCoulDSF::compute, src/pair.cpp:289-419): one thread per atom walks its full
-// list (ELL layout, coalesced), so each atom's force is a private register sum written once:
-// no atomics, deterministic.  The F/r table sits in LDS; each directed entry adds half of the
+// Pair (LJTableCoulDSF::compute, src/pair.cpp:289-419): a group of kPairG lanes per atom walks
+// its full list row (G consecutive entries per step), and the group's butterfly sum is written
+// once per atom: no atomics, deterministic.  The F/r table sits in LDS; each directed entry adds half of the
 // pair energy and virial.  The arithmetic per pair is the reference's, in the same order:
 //   fpair = lerp(F/r)*s_lj + qi*qj*K*(1/(r*rsq) + fc/r)*s_coul
 // Bonded (src/bond.cpp:108-137, src/angle.cpp:87-136, src/dihedral.cpp:109-166): one thread
@@ -18,14 +18,15 @@ namespace {
 __constant__ float c_special_lj[4] = {1.0f, 0.0f, 0.0f, 0.5f};               // src/molecule.h:11
 __constant__ float c_special_coul[4] = {1.0f, 0.0f, 0.0f, float(5 / 6.)};
 constexpr float kElectricK = 332.06371f;                                      // src/environ.h:77
+constexpr int kPairG = 16;  // lanes per atom
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   int i = min(a, b), j = max(a, b);
   return i * ntype - (i * (i - 1)) / 2 + (j - i);
 }
 
-template <bool EV>
-__global__ void __launch_bounds__(256) k_pair(int n, int stride, const uint32_t* __restrict__ nbr,
+template <bool EV, int G>
+__global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __restrict__ nbr,
                                               const int* __restrict__ cnt, const float4* __restrict__ xq,
                                               const int* __restrict__ typ, int ntype, int npair, float bl, float m_bl,
                                               float cutsq, float cut, float rdr, float fc, float vc, int nb1,
@@ -34,15 +35,17 @@ __global__ void __launch_bounds__(256) k_pair(int n, int stride, const uint32_t*
   extern __shared__ float s_tab[];  // [npair][nb1] F/r
   for (int k = threadIdx.x; k < npair * nb1; k += blockDim.x) s_tab[k] = tab[k];
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
   double e_acc = 0, w_acc = 0;
   if (i < n) {
     const float4 xi = xq[i];
     const int ti = typ[i];
     float fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     const int m = cnt[i];
-    for (int k = 0; k < m; ++k) {
-      const uint32_t ent = nbr[(size_t)k * stride + i];
+    const uint32_t* row = list_row(nbr, cap, i);
+    for (int k = lane; k < m; k += G) {
+      const uint32_t ent = row[k];
       const int j = NNPE_ENTRY_J(ent), code = NNPE_ENTRY_CODE(ent), sp = NNPE_ENTRY_SP(ent);
       const float4 xj = image_pos(xq[j], code, bl, m_bl);
       float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -66,7 +69,8 @@ __global__ void __launch_bounds__(256) k_pair(int n, int stride, const uint32_t*
         w += 0.5f * fpair * rsq;
       }
     }
-    f[i] = make_float4(fx, fy, fz, 0.0f);
+    fx = group_sum<G>(fx); fy = group_sum<G>(fy); fz = group_sum<G>(fz);
+    if (lane == 0) f[i] = make_float4(fx, fy, fz, 0.0f);
     e_acc = e; w_acc = w;
   }
   if (EV) block_sum2_store<256>(e_acc, w_acc, pe, pw);
@@ -197,18 +201,18 @@ extern "C" int nnpe_pair_table(nnpe_ctx* ctx, float cutoff, int nbin, const floa
 }
 
 int pair_enqueue(nnpe_ctx* ctx, bool ev) {
-  const int N = ctx->natom, nb1 = ctx->pt_nbin + 1, nblk = blocks_for(N, 256);
+  const int N = ctx->natom, nb1 = ctx->pt_nbin + 1, nblk = blocks_for((long long)N * kPairG, 256);
   const size_t lds = size_t(ctx->npair) * nb1 * sizeof(float);
   double* pe = ev_part(ctx, NNPE_TERM_PAIR, 0);
   double* pw = ev_part(ctx, NNPE_TERM_PAIR, 1);
   if (ev)
-    hipLaunchKernelGGL(k_pair<true>, dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->stride, ctx->nbr_long, ctx->cnt_long,
-                       ctx->xq, ctx->typ, ctx->ntype, ctx->npair, ctx->bl, ctx->m_bl, ctx->pt_cutsq, ctx->pt_cut,
-                       ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
+    hipLaunchKernelGGL((k_pair<true, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->npair, ctx->bl, ctx->m_bl, ctx->pt_cutsq,
+                       ctx->pt_cut, ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
   else
-    hipLaunchKernelGGL(k_pair<false>, dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->stride, ctx->nbr_long, ctx->cnt_long,
-                       ctx->xq, ctx->typ, ctx->ntype, ctx->npair, ctx->bl, ctx->m_bl, ctx->pt_cutsq, ctx->pt_cut,
-                       ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
+    hipLaunchKernelGGL((k_pair<false, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->npair, ctx->bl, ctx->m_bl, ctx->pt_cutsq,
+                       ctx->pt_cut, ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_PAIR, nblk);
   return NNPE_OK;

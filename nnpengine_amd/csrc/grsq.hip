@@ -8,7 +8,7 @@
 //                r_max_inner adds 1 (2 per unordered pair, as the reference's +2/+1 rule);
 //                bins >= nbin_inner keep their previous counts, as the reference's do.
 //   full tally   (GR::tally(bin_list), :151-205): cell stencil to r_max; real j > i add 2,
-//                ghost images add 1.
+//                ghost images add 1; one workgroup per home cell, candidates staged in LDS.
 // SQ (src/grsq.cpp:258-322): S_ab(q) = sum_r 4 pi rho dr r sin(q r)/q h_ab(r) in FP64, one
 // workgroup per q point with a fixed-order reduction.
 #include <algorithm>
@@ -19,6 +19,7 @@
 namespace {
 
 constexpr int kHistLdsMax = 48 * 1024;
+constexpr int kGrG = 16;  // lanes per atom, inner tally
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   int i = min(a, b), j = max(a, b);
@@ -33,8 +34,10 @@ __global__ void k_gr_zero(unsigned long long* count, int npair, int nbin, int nz
   count[(k / nzero) * nbin + (k % nzero)] = 0ull;
 }
 
-template <bool kLds>
-__global__ void __launch_bounds__(256) k_gr_inner(int n, int stride, const uint32_t* __restrict__ nbr,
+// inner tally: groups of G lanes per atom walk the list rows; each workgroup owns a contiguous
+// range of atoms (so its LDS histogram is flushed once for many atoms)
+template <bool kLds, int G>
+__global__ void __launch_bounds__(256) k_gr_inner(int n, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                   const int* __restrict__ typ, int ntype, int npair, float bl, float m_bl,
                                                   float rmaxsq, float rdr, int nbin_inner, int nbin,
@@ -44,13 +47,17 @@ __global__ void __launch_bounds__(256) k_gr_inner(int n, int stride, const uint3
     for (int k = threadIdx.x; k < npair * nbin_inner; k += blockDim.x) s_hist[k] = 0u;
     __syncthreads();
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+  constexpr int kGroups = 256 / G;
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int a0 = xcd_block(gridDim.x) * per, a1 = min(n, a0 + per);
+  const int lane = threadIdx.x & (G - 1);
+  for (int i = a0 + (int)threadIdx.x / G; i < a1; i += kGroups) {
     const float4 xi = xq[i];
     const int ti = typ[i];
     const int m = cnt[i];
-    for (int k = 0; k < m; ++k) {
-      const uint32_t e = nbr[(size_t)k * stride + i];
+    const uint32_t* row = list_row(nbr, cap, i);
+    for (int k = lane; k < m; k += G) {
+      const uint32_t e = row[k];
       const int j = NNPE_ENTRY_J(e);
       const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(e), bl, m_bl);
       const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
@@ -71,57 +78,105 @@ __global__ void __launch_bounds__(256) k_gr_inner(int n, int stride, const uint3
   }
 }
 
+// Full tally by home cell: a persistent workgroup takes a contiguous run of home cells; for each,
+// the candidates of its stencil cells (real rows, then ghost images, per stencil cell) are
+// staged in LDS in batches and each wave tests 64 of them per step against one home atom,
+// adding 2 for a real j with a larger row (each unordered real pair once) and 1 for a ghost
+// image, into the workgroup's LDS histogram (flushed once per workgroup).
+constexpr int kGrBatch = 1024;
+constexpr int kGrMaxRanges = 2 * 9 * 9 * 9;
+
 template <bool kLds>
-__global__ void __launch_bounds__(256) k_gr_full(int n, int num_gx, const int4* __restrict__ stencil, int nst,
-                                                 const float4* __restrict__ xq, const int* __restrict__ typ,
-                                                 const int* __restrict__ cell, const int* __restrict__ cstart,
-                                                 const int* __restrict__ gstart, const int* __restrict__ gs_src,
-                                                 const int* __restrict__ gs_code, int ntype, int npair, float bl,
-                                                 float m_bl, float rmaxsq, float rdr, int nbin,
-                                                 unsigned long long* __restrict__ count) {
+__global__ void __launch_bounds__(256) k_gr_cells(int ncell, int num_gx, const int4* __restrict__ stencil, int nst,
+                                                  const float4* __restrict__ xq, const int* __restrict__ typ,
+                                                  const int* __restrict__ cstart, const int* __restrict__ gstart,
+                                                  const int* __restrict__ gs_src, const int* __restrict__ gs_code,
+                                                  int ntype, int npair, float bl, float m_bl, float rmaxsq, float rdr,
+                                                  int nbin, unsigned long long* __restrict__ count) {
   extern __shared__ unsigned int s_hist[];
+  __shared__ int r_begin[kGrMaxRanges], r_pref[kGrMaxRanges + 1];
+  __shared__ unsigned char r_ghost[kGrMaxRanges];
+  __shared__ float4 c_pos[kGrBatch];
+  __shared__ int c_row[kGrBatch];  // real row, or -1 - type for a ghost image
+  __shared__ int s_tot;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (kLds) {
-    for (int k = threadIdx.x; k < npair * nbin; k += blockDim.x) s_hist[k] = 0u;
-    __syncthreads();
+    for (int k = tid; k < npair * nbin; k += blockDim.x) s_hist[k] = 0u;
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const float4 xi = xq[i];
-    const int ti = typ[i];
-    const int ci = cell[i];
-    const int cz = ci % num_gx, cy = (ci / num_gx) % num_gx, cx = ci / (num_gx * num_gx);
-    for (int s = 0; s < nst; ++s) {
+  const int per = (ncell + gridDim.x - 1) / gridDim.x;
+  const int c0 = xcd_block(gridDim.x) * per, c1 = min(ncell, c0 + per);
+  const int nr2 = 2 * nst;
+  for (int c = c0; c < c1; ++c) {
+    const int hb = cstart[c], he = cstart[c + 1];
+    if (hb == he) continue;  // uniform across the workgroup
+    const int cz = c % num_gx, cy = (c / num_gx) % num_gx, cx = c / (num_gx * num_gx);
+    __syncthreads();  // previous cell's ranges and batch are consumed
+    for (int s = tid; s < nst; s += blockDim.x) {
       const int4 st = stencil[s];
       const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
-      if (ax < 0 || ay < 0 || az < 0 || ax >= num_gx || ay >= num_gx || az >= num_gx) continue;
-      const int c = (ax * num_gx + ay) * num_gx + az;
-      for (int j = max(i + 1, cstart[c]), je = cstart[c + 1]; j < je; ++j) {
-        const float4 xj = xq[j];
-        const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
-        if (!(rsq < rmaxsq)) continue;
-        const int b = gr_bin(rsq, rdr);
-        if (b >= nbin) continue;
-        const int p = pair_index(ti, typ[j], ntype);
-        if (kLds) atomicAdd(&s_hist[p * nbin + b], 2u);
-        else atomicAdd(&count[(size_t)p * nbin + b], 2ull);
+      int nr = 0, ng = 0, br = 0, bg = 0;
+      if (ax >= 0 && ay >= 0 && az >= 0 && ax < num_gx && ay < num_gx && az < num_gx) {
+        const int nc = (ax * num_gx + ay) * num_gx + az;
+        br = cstart[nc]; nr = cstart[nc + 1] - br;
+        bg = gstart[nc]; ng = gstart[nc + 1] - bg;
       }
-      for (int g = gstart[c], ge = gstart[c + 1]; g < ge; ++g) {
-        const int src = gs_src[g];
-        const float4 xj = image_pos(xq[src], gs_code[g], bl, m_bl);
-        const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
-        if (!(rsq < rmaxsq)) continue;
-        const int b = gr_bin(rsq, rdr);
-        if (b >= nbin) continue;
-        const int p = pair_index(ti, typ[src], ntype);
-        if (kLds) atomicAdd(&s_hist[p * nbin + b], 1u);
-        else atomicAdd(&count[(size_t)p * nbin + b], 1ull);
+      r_begin[2 * s] = br; r_pref[2 * s] = nr; r_ghost[2 * s] = 0;
+      r_begin[2 * s + 1] = bg; r_pref[2 * s + 1] = ng; r_ghost[2 * s + 1] = 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int run = 0;
+      for (int k = 0; k < nr2; ++k) { const int v = r_pref[k]; r_pref[k] = run; run += v; }
+      r_pref[nr2] = run;
+      s_tot = run;
+    }
+    __syncthreads();
+    const int total = s_tot;
+    for (int t0 = 0; t0 < total; t0 += kGrBatch) {
+      const int tn = min(kGrBatch, total - t0);
+      __syncthreads();
+      for (int u = tid; u < tn; u += blockDim.x) {
+        const int t = t0 + u;
+        int lo = 0, hi = nr2 - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (r_pref[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const int pos = r_begin[lo] + (t - r_pref[lo]);
+        if (r_ghost[lo]) {
+          const int src = gs_src[pos];
+          c_pos[u] = image_pos(xq[src], gs_code[pos], bl, m_bl);
+          c_row[u] = -1 - typ[src];
+        } else {
+          c_pos[u] = xq[pos];
+          c_row[u] = pos;
+        }
+      }
+      __syncthreads();
+      for (int h = hb + wave; h < he; h += blockDim.x / 64) {
+        const float4 xi = xq[h];
+        const int ti = typ[h];
+        for (int u = lane; u < tn; u += 64) {
+          const int jr = c_row[u];
+          if (jr >= 0 && jr <= h) continue;
+          const float4 xj = c_pos[u];
+          const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+          if (!(rsq < rmaxsq)) continue;
+          const int b = gr_bin(rsq, rdr);
+          if (b >= nbin) continue;
+          const int tj = jr >= 0 ? typ[jr] : -1 - jr;
+          const int p = pair_index(ti, tj, ntype);
+          const unsigned int wgt = jr >= 0 ? 2u : 1u;
+          if (kLds) atomicAdd(&s_hist[p * nbin + b], wgt);
+          else atomicAdd(&count[(size_t)p * nbin + b], (unsigned long long)wgt);
+        }
       }
     }
   }
   if (kLds) {
     __syncthreads();
-    for (int k = threadIdx.x; k < npair * nbin; k += blockDim.x) {
-      unsigned int v = s_hist[k];
+    for (int k = tid; k < npair * nbin; k += blockDim.x) {
+      const unsigned int v = s_hist[k];
       if (v) atomicAdd(&count[k], (unsigned long long)v);
     }
   }
@@ -203,24 +258,29 @@ int gr_enqueue(nnpe_ctx* ctx, bool full) {
     const int nz = ctx->gr_nbin_inner;
     hipLaunchKernelGGL(k_gr_zero, dim3(blocks_for(npair * nz, 256)), dim3(256), 0, s, ctx->gr_count, npair, nb, nz);
     const size_t lds = size_t(npair) * nz * sizeof(unsigned int);
+    // about 64 atoms per workgroup, at most 2048 workgroups
+    const int nblk = std::max(1, std::min(2048, blocks_for(N, 64)));
     if (lds <= kHistLdsMax)
-      hipLaunchKernelGGL(k_gr_inner<true>, dim3(blocks_for(N, 256)), dim3(256), lds, s, N, ctx->stride, ctx->nbr_long,
+      hipLaunchKernelGGL((k_gr_inner<true, kGrG>), dim3(nblk), dim3(256), lds, s, N, ctx->cap_long, ctx->nbr_long,
                          ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq_inner,
                          ctx->gr_rdr, nz, nb, ctx->gr_count);
     else
-      hipLaunchKernelGGL(k_gr_inner<false>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->stride, ctx->nbr_long,
+      hipLaunchKernelGGL((k_gr_inner<false, kGrG>), dim3(nblk), dim3(256), 0, s, N, ctx->cap_long, ctx->nbr_long,
                          ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq_inner,
                          ctx->gr_rdr, nz, nb, ctx->gr_count);
   } else {
     hipLaunchKernelGGL(k_gr_zero, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, s, ctx->gr_count, npair, nb, nb);
+    if (ctx->n_stencil_gr > kGrMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "g(r) stencil too large");
     const size_t lds = size_t(npair) * nb * sizeof(unsigned int);
+    const int G = ctx->num_grid;
+    const int nblk = std::max(1, std::min(1024, G));  // persistent workgroups, contiguous runs of cells
     if (lds <= kHistLdsMax)
-      hipLaunchKernelGGL(k_gr_full<true>, dim3(blocks_for(N, 256)), dim3(256), lds, s, N, ctx->num_gx, ctx->stencil_gr,
-                         ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
+      hipLaunchKernelGGL(k_gr_cells<true>, dim3(nblk), dim3(256), lds, s, G, ctx->num_gx, ctx->stencil_gr,
+                         ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
                          ctx->gs_code, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
     else
-      hipLaunchKernelGGL(k_gr_full<false>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->num_gx, ctx->stencil_gr,
-                         ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
+      hipLaunchKernelGGL(k_gr_cells<false>, dim3(nblk), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_gr,
+                         ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
                          ctx->gs_code, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
   }
   NNPE_CHECK_LAUNCH(ctx);

@@ -8,8 +8,8 @@
 //  * ghosts are not materialised: a ghost is (source row, image code) sorted by ghost cell,
 //    its position is xq[src] + image shift computed in FP32 exactly as the reference's
 //    x_ghost = x + (+-L) (src/environ.cpp:431-433, :505-507);
-//  * neighbour lists are ELL-padded: entry k of atom i at nbr[k * stride + i], so lane i of a
-//    wave reads consecutive words; an entry packs (j row : 25 bits, image code : 5 bits,
+//  * neighbour lists are padded rows: entry k of atom i at nbr[i * cap + k], read by a group of
+//    lanes per atom (consecutive words); an entry packs (j row : 25 bits, image code : 5 bits,
 //    special class : 2 bits).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -218,6 +218,31 @@ __device__ __forceinline__ float4 image_pos(float4 xs, int code, float bl, float
 // reference's sqr(dx)+sqr(dy)+sqr(dz) (src/neighlist.cpp:260)
 __device__ __forceinline__ float dist2_rn(float dx, float dy, float dz) {
   return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+// ---- lanes-per-atom groups ------------------------------------------------------------------
+// The pair-loop kernels give each atom a group of G lanes (G divides 64): lane l of the group
+// takes entries l, l+G, l+2G, ... of the atom's list row, so a group reads G consecutive 4-byte
+// entries per step, and the group's partial sums are combined by a fixed xor butterfly (every
+// lane ends with the same, order-independent-of-timing total).
+template <int G, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin by workgroup id.  Renumber them so each XCD
+// walks one contiguous range of blocks (rows are cell-sorted, so its atoms are spatially close
+// and their neighbours' rows stay in that XCD's L2).
+__device__ __forceinline__ int xcd_block(int nblocks) {
+  const int b = blockIdx.x, x = b & 7, k = b >> 3, q = nblocks >> 3, r = nblocks & 7;
+  return x * q + min(x, r) + k;
+}
+
+// neighbour list rows: entry k of row i at nbr[i * cap + k]
+__device__ __forceinline__ const uint32_t* list_row(const uint32_t* nbr, int cap, int i) {
+  return nbr + (size_t)i * cap;
 }
 
 // energy/virial: block reduction of one double per thread into ev_partial[slot][blockIdx]

@@ -6,9 +6,11 @@
 // a_0 = G_i.  Backward (hand-written): d_L = s_out * w_out * (1 - a_L^2),
 // d_{l-1} = (d_l W_l^T) * (1 - a_{l-1}^2),  dE/dG = d_1 W_1^T.
 //
-// Kernel shape: one wave (64 lanes) owns a tile of 32 atoms of one element and runs the whole
-// forward and backward for them.  Activations live in a wave-private LDS slab
-// [32 x width] per layer (padded rows, no bank-conflicting strides); every GEMM is
+// Kernel shape: one workgroup of kMlpWaves waves owns a tile of 32 atoms of one element and runs
+// the whole forward and backward for them; within a layer wave w computes the 32-column output
+// blocks w, w + kMlpWaves, ... and a barrier separates the layers.  Activations live in the
+// workgroup's LDS slab [32 x width] per layer (padded rows, no bank-conflicting strides); every
+// GEMM is
 // 32 x N x K on v_mfma_f32_32x32x16_{bf16,f16} (K = 16 per instruction) or
 // v_mfma_f32_32x32x2f32 for the FP32 form, with FP32 accumulation.  Weights are read as MFMA B
 // fragments straight from global memory (L2-resident: 86 KB per element in BF16), in two
@@ -132,13 +134,16 @@ __device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf
   return acc;
 }
 
+constexpr int kMlpWaves = 4;
+
 template <int KIND>
-__global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
+__global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
                                            float* __restrict__ e_atom, const int* __restrict__ type_idx) {
   typedef typename Prec<KIND>::T T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NT = 64 * kMlpWaves;
   const int tile = blockIdx.x;
   int e = 0;
   while (e + 1 < kMaxType && tile >= P.tile_off[e + 1]) ++e;
@@ -152,7 +157,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
   // stage the descriptor rows standardised in fp32, zero padding rows/cols
   const float* shv = P.shift[e];
   const float* scv = P.scale[e];
-  for (int idx = lane; idx < 32 * Dp; idx += 64) {
+  for (int idx = tid; idx < 32 * Dp; idx += NT) {
     const int r = idx / Dp, c = idx - r * Dp;
     float v = 0.0f;
     if (r < nrows && c < D) v = (G[(size_t)rows[r] * D + c] - shv[c]) * scv[c];
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     const T* wt = reinterpret_cast<const T*>(P.Wt[e][l]);
     const float* bias = P.b[e][l];
     const float un = 1.0f / P.s_w[l];
-    for (int cb = 0; cb < P.H[l] / 32; ++cb) {
+    for (int cb = wave; cb < P.H[l] / 32; cb += kMlpWaves) {
       f32x16 acc;
       if constexpr (KIND == 1)
         acc = gemm_block_pairs(src, nullptr, lds_src, wt, reinterpret_cast<const T*>(P.Wt_lo[e][l]), K, cb, lane);
@@ -188,7 +193,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
   const int HL = P.H[nh - 1];
   const T* aL = lds + P.slab_off[nh];
   const int ldl = P.ld[nh];
-  {
+  if (wave == 0) {
     const int r = lane & 31, half = lane >> 5;
     const float* wo = P.wout[e];
     float s = 0.0f;
@@ -203,7 +208,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     T* dL = lds + P.slab_off[nh];
     const float* wo = P.wout[e];
     const float sc = P.out_scale * P.s_d[nh - 1];
-    for (int idx = lane; idx < 32 * HL; idx += 64) {
+    for (int idx = tid; idx < 32 * HL; idx += NT) {
       const int r = idx / HL, c = idx - r * HL;
       const float a = Prec<KIND>::back(dL[r * ldl + c]);
       const float v = sc * wo[c] * (1.0f - a * a);
@@ -222,7 +227,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     const T* w = reinterpret_cast<const T*>(P.W[e][l]);
     const float un = 1.0f / (P.s_d[l] * P.s_w[l]);
     const float sd = P.s_d[l - 1];
-    for (int cb = 0; cb < P.H[l - 1] / 32; ++cb) {
+    for (int cb = wave; cb < P.H[l - 1] / 32; cb += kMlpWaves) {
       f32x16 acc;
       if constexpr (KIND == 1)
         acc = gemm_block_pairs(dl, lds + P.lo_off[l + 1], ldl2, w, reinterpret_cast<const T*>(P.W_lo[e][l]), P.H[l], cb,
@@ -247,7 +252,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
     const T* d1 = lds + P.slab_off[1];
     const T* w = reinterpret_cast<const T*>(P.W[e][0]);
     const float un = 1.0f / (P.s_d[0] * P.s_w[0]);
-    for (int cb = 0; cb < Dp / 32; ++cb) {
+    for (int cb = wave; cb < Dp / 32; cb += kMlpWaves) {
       f32x16 acc;
       if constexpr (KIND == 1)
         acc = gemm_block_pairs(d1, lds + P.lo_off[1], P.ld[1], w, reinterpret_cast<const T*>(P.W_lo[e][0]), P.H[0], cb,
@@ -267,11 +272,24 @@ __global__ void __launch_bounds__(64) k_mlp(MlpDev P, const float* __restrict__ 
   }
 }
 
-__global__ void k_type_index(int n, const int* typ, const int* off, int* fill, int* idx) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int t = typ[i];
-  idx[off[t] + atomicAdd(&fill[t], 1)] = i;
+// rows of each element, wave-aggregated: one atomic per wave and element (a per-row atomic on
+// a handful of counters serialises)
+__global__ void __launch_bounds__(256) k_type_index(int n, int ntype, const int* typ, const int* off, int* fill,
+                                                    int* idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int t = i < n ? typ[i] : -1;
+  for (int e = 0; e < ntype; ++e) {
+    const unsigned long long bal = __ballot(t == e);
+    if (!bal) continue;
+    int base = 0;
+    if (lane == __ffsll((long long)bal) - 1) base = atomicAdd(&fill[e], __popcll(bal));
+    base = __shfl(base, __ffsll((long long)bal) - 1, 64);
+    if (t == e) {
+      const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+      idx[off[e] + base + rank] = i;
+    }
+  }
 }
 
 __global__ void k_zero_small(int* p, int n) {
@@ -328,8 +346,8 @@ int type_index_enqueue(nnpe_ctx* ctx) {
   if (!ctx->type_idx) return NNPE_OK;
   MlpState* S = ctx->mlp;
   hipLaunchKernelGGL(k_zero_small, dim3(1), dim3(64), 0, ctx->stream, ctx->type_fill, ctx->ntype);
-  hipLaunchKernelGGL(k_type_index, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->typ,
-                     S->d_type_off, ctx->type_fill, ctx->type_idx);
+  hipLaunchKernelGGL(k_type_index, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->ntype,
+                     ctx->typ, S->d_type_off, ctx->type_fill, ctx->type_idx);
   NNPE_CHECK_LAUNCH(ctx);
   return NNPE_OK;
 }
@@ -501,11 +519,11 @@ int mlp_enqueue(nnpe_ctx* ctx, bool ev) {
   const MlpDev& d = S->dev;
   if (S->ntiles > 0) {
     if (S->kind == 0)
-      hipLaunchKernelGGL(k_mlp<0>, dim3(S->ntiles), dim3(64), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
+      hipLaunchKernelGGL(k_mlp<0>, dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
     else if (S->kind == 1)
-      hipLaunchKernelGGL(k_mlp<1>, dim3(S->ntiles), dim3(64), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
+      hipLaunchKernelGGL(k_mlp<1>, dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
     else
-      hipLaunchKernelGGL(k_mlp<2>, dim3(S->ntiles), dim3(64), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
+      hipLaunchKernelGGL(k_mlp<2>, dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
     NNPE_CHECK_LAUNCH(ctx);
   }
   if (ev) {

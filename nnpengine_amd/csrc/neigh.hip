@@ -8,8 +8,8 @@
 //   cells, and the test (xi-xj)^2+(yi-yj)^2+(zi-zj)^2 < (rc+skin)^2 without contraction.
 // The GPU build differs in mechanics only: counting sorts with atomics followed by a per-cell
 // sort on the atom id (deterministic order), ghosts as (source row, image code) records, and a
-// full list written ELL-padded (thread i writes entry k at k*stride+i, so a wave's stores
-// coalesce) instead of per-thread half lists.
+// full list in padded rows (entry k of atom i at nbr[i*cap + k]) written by wavefront-ballot
+// compaction, instead of per-thread half lists.
 #include <algorithm>
 #include <cmath>
 
@@ -91,12 +91,25 @@ __global__ void k_scatter(int n, const int* cell_of, const int* start, int* fill
   order[start[c] + atomicAdd(&fill[c], 1)] = i;
 }
 
-// deterministic order inside a cell: ascending atom id (insertion sort of a short segment)
-__global__ void k_cell_sort(int ncell, const int* start, int* order, const int4* info) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int b = start[c], e = start[c + 1];
-  for (int k = b + 1; k < e; ++k) {
+// deterministic order inside a cell: ascending atom id.  One wave per cell: a cell of up to 64
+// entries is ranked in registers (each lane counts the smaller keys of the others through
+// shuffles, keys are unique) and scattered to its rank; longer cells (rare) fall back to an
+// insertion sort by lane 0.
+__global__ void __launch_bounds__(256) k_cell_sort(int ncell, const int* start, int* order, const int4* info) {
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (c >= ncell) return;  // whole wave
+  const int b = start[c], n = start[c + 1] - b;
+  if (n <= 1) return;
+  if (n <= 64) {
+    const int v = lane < n ? order[b + lane] : 0;
+    const int key = lane < n ? info[v].x : 0x7FFFFFFF;
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += __shfl(key, k, 64) < key;
+    if (lane < n) order[b + rank] = v;
+    return;
+  }
+  if (lane != 0) return;
+  for (int k = b + 1; k < b + n; ++k) {
     int v = order[k], key = info[v].x, m = k - 1;
     while (m >= b && info[order[m]].x > key) { order[m + 1] = order[m]; --m; }
     order[m + 1] = v;
@@ -172,18 +185,30 @@ __global__ void k_ghost_scatter(int cap, const DevCounters* cnt, const int* g_sr
 }
 
 // ENVIRON::sort_ghost order made deterministic: ascending (source atom id, image code)
-__global__ void k_ghost_cell_sort(int ncell, const int* gstart, int* gs_src, int* gs_code, const int4* info) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int b = gstart[c], e = gstart[c + 1];
-  for (int k = b + 1; k < e; ++k) {
-    int s = gs_src[k], cd = gs_code[k];
-    long long key = (long long)info[s].x * 32 + cd;
+// ghosts of a cell in (atom id, image code) order, ranked as k_cell_sort
+__global__ void __launch_bounds__(256) k_ghost_cell_sort(int ncell, const int* gstart, int* gs_src, int* gs_code,
+                                                         const int4* info) {
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (c >= ncell) return;  // whole wave
+  const int b = gstart[c], n = gstart[c + 1] - b;
+  if (n <= 1) return;
+  if (n <= 64) {
+    const int sv = lane < n ? gs_src[b + lane] : 0, cd = lane < n ? gs_code[b + lane] : 0;
+    const long long key = lane < n ? (long long)info[sv].x * 32 + cd : 0x7FFFFFFFFFFFFFFFLL;
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += __shfl(key, k, 64) < key;
+    if (lane < n) { gs_src[b + rank] = sv; gs_code[b + rank] = cd; }
+    return;
+  }
+  if (lane != 0) return;
+  for (int k = b + 1; k < b + n; ++k) {
+    int sv = gs_src[k], cd = gs_code[k];
+    long long key = (long long)info[sv].x * 32 + cd;
     int m = k - 1;
     while (m >= b && (long long)info[gs_src[m]].x * 32 + gs_code[m] > key) {
       gs_src[m + 1] = gs_src[m]; gs_code[m + 1] = gs_code[m]; --m;
     }
-    gs_src[m + 1] = s; gs_code[m + 1] = cd;
+    gs_src[m + 1] = sv; gs_code[m + 1] = cd;
   }
 }
 
@@ -217,65 +242,145 @@ __global__ void k_xmol(int n, float ghost_lim, float bl, float m_bl, float h_bl,
 }
 
 // NEIGHLIST::build (src/neighlist.cpp:190-331) as a full list: every real neighbour j != i and
-// every ghost image within (rc+skin), or in a stencil cell flagged "no check".  Optionally a
-// second (short) list of the entries within a smaller cutoff.
+// every ghost image within (rc+skin), or in a stencil cell flagged "no check", with the
+// special class of the pair; optionally a second (short) list of the entries within a smaller
+// cutoff.  One workgroup per home cell with wavefront-ballot compaction: all atoms of a cell
+// share one stencil, so the workgroup stages the candidates (real rows and ghost images of the
+// stencil cells, in stencil order) in LDS once per batch; each wave then takes one home atom at
+// a time, its 64 lanes test 64 candidates per step, and a ballot + mbcnt gives every accepted
+// lane its slot, so a wave appends up to 64 entries to the atom's row with one coalesced store.
+// Entry order in a row: stencil cell, real before ghost, then position in the cell.
+constexpr int kBuildBatch = 1024;  // candidates staged per pass
+constexpr int kBuildHome = 64;     // home atoms per pass
+constexpr int kBuildMaxRanges = 2 * 343;
+
 template <bool kShort>
-__global__ void __launch_bounds__(256) k_build(int n, int stride, int num_gx, const int4* __restrict__ stencil, int nst,
-                                               float cutsq, float cutsq_s, float bl, float m_bl,
-                                               const float4* __restrict__ xq, const int4* __restrict__ info,
-                                               const int* __restrict__ cell, const int* __restrict__ cstart,
-                                               const int* __restrict__ gstart, const int* __restrict__ gs_src,
-                                               const int* __restrict__ gs_code, const int* __restrict__ mt_natom,
-                                               const int* __restrict__ mt_off, const int* __restrict__ mt_sp,
-                                               int cap, int cap_s, uint32_t* __restrict__ nbr, int* __restrict__ cnt_out,
-                                               uint32_t* __restrict__ nbr_s, int* __restrict__ cnt_s_out,
-                                               float4* __restrict__ xprev, DevCounters* counters) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 xi = xq[i];
-  xprev[i] = xi;
-  const int4 ii = info[i];
-  const int ci = cell[i];
-  const int cz = ci % num_gx, cy = (ci / num_gx) % num_gx, cx = ci / (num_gx * num_gx);
-  const int mtn = mt_natom[ii.w];
-  const int* sp_row = mt_sp + mt_off[ii.w] + ii.z * mtn;
-  int nl = 0, ns = 0;
-  auto emit = [&](int j, int code, float rsq, int4 ij) {
-    int sp = (ij.y == ii.y) ? sp_row[ij.z] : 0;
-    uint32_t e = NNPE_MAKE_ENTRY(j, code, sp);
-    if (nl < cap) nbr[(size_t)nl * stride + i] = e;
-    ++nl;
-    if (kShort && rsq < cutsq_s) {
-      if (ns < cap_s) nbr_s[(size_t)ns * stride + i] = e;
-      ++ns;
-    }
-  };
-  for (int s = 0; s < nst; ++s) {
+__global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, const int4* __restrict__ stencil, int nst,
+                                                     float cutsq, float cutsq_s, float bl, float m_bl,
+                                                     const float4* __restrict__ xq, const int4* __restrict__ info,
+                                                     const int* __restrict__ cstart, const int* __restrict__ gstart,
+                                                     const int* __restrict__ gs_src, const int* __restrict__ gs_code,
+                                                     const int* __restrict__ mt_natom, const int* __restrict__ mt_off,
+                                                     const int* __restrict__ mt_sp, int cap, int cap_s,
+                                                     uint32_t* __restrict__ nbr, int* __restrict__ cnt_out,
+                                                     uint32_t* __restrict__ nbr_s, int* __restrict__ cnt_s_out,
+                                                     float4* __restrict__ xprev, DevCounters* counters) {
+  __shared__ int r_begin[kBuildMaxRanges], r_pref[kBuildMaxRanges + 1];
+  __shared__ unsigned char r_flag[kBuildMaxRanges];  // bit 0: ghost, bit 1: no distance check
+  __shared__ float4 c_pos[kBuildBatch];
+  __shared__ uint32_t c_ent[kBuildBatch];            // row | code << 25 | nocheck << 31
+  __shared__ int2 c_mol[kBuildBatch];                // molecule id, intra id
+  __shared__ int h_nl[kBuildHome], h_ns[kBuildHome];
+  __shared__ int s_tot;
+  const int c = xcd_block(gridDim.x);
+  if (c >= ncell) return;
+  const int hb = cstart[c], he = cstart[c + 1];
+  if (hb == he) return;  // uniform across the workgroup
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cz = c % num_gx, cy = (c / num_gx) % num_gx, cx = c / (num_gx * num_gx);
+  // candidate ranges of the stencil, then their exclusive prefix sum
+  for (int s = tid; s < nst; s += blockDim.x) {
     const int4 st = stencil[s];
     const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
-    if (ax < 0 || ay < 0 || az < 0 || ax >= num_gx || ay >= num_gx || az >= num_gx) continue;
-    const int c = (ax * num_gx + ay) * num_gx + az;
-    const bool nocheck = st.w & kStencilNoCheck;
-    for (int j = cstart[c], je = cstart[c + 1]; j < je; ++j) {
-      if (j == i) continue;
-      const float4 xj = xq[j];
-      const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
-      if (nocheck || rsq < cutsq) emit(j, 13, rsq, info[j]);
+    int nr = 0, ng = 0, br = 0, bg = 0;
+    if (ax >= 0 && ay >= 0 && az >= 0 && ax < num_gx && ay < num_gx && az < num_gx) {
+      const int nc = (ax * num_gx + ay) * num_gx + az;
+      br = cstart[nc]; nr = cstart[nc + 1] - br;
+      bg = gstart[nc]; ng = gstart[nc + 1] - bg;
     }
-    for (int g = gstart[c], ge = gstart[c + 1]; g < ge; ++g) {
-      const int src = gs_src[g], code = gs_code[g];
-      const float4 xj = image_pos(xq[src], code, bl, m_bl);
-      const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
-      if (nocheck || rsq < cutsq) emit(src, code, rsq, info[src]);
-    }
+    const unsigned char nock = (st.w & kStencilNoCheck) ? 2 : 0;
+    r_begin[2 * s] = br; r_pref[2 * s] = nr; r_flag[2 * s] = nock;
+    r_begin[2 * s + 1] = bg; r_pref[2 * s + 1] = ng; r_flag[2 * s + 1] = nock | 1;
   }
-  cnt_out[i] = min(nl, cap);
-  if (nl > cap) atomicMax(&counters->overflow_long, nl);
-  atomicMax(&counters->max_long, nl);
-  if (kShort) {
-    cnt_s_out[i] = min(ns, cap_s);
-    if (ns > cap_s) atomicMax(&counters->overflow_short, ns);
-    atomicMax(&counters->max_short, ns);
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int k = 0; k < 2 * nst; ++k) { const int v = r_pref[k]; r_pref[k] = run; run += v; }
+    r_pref[2 * nst] = run;
+    s_tot = run;
+  }
+  __syncthreads();
+  const int total = s_tot, nr2 = 2 * nst;
+  for (int h0 = hb; h0 < he; h0 += kBuildHome) {
+    const int h1 = min(he, h0 + kBuildHome);
+    for (int k = tid; k < h1 - h0; k += blockDim.x) { h_nl[k] = 0; h_ns[k] = 0; }
+    for (int t0 = 0; t0 < total; t0 += kBuildBatch) {
+      const int tn = min(kBuildBatch, total - t0);
+      __syncthreads();
+      for (int u = tid; u < tn; u += blockDim.x) {
+        const int t = t0 + u;
+        int lo = 0, hi = nr2 - 1;  // last range with r_pref[k] <= t
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (r_pref[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const int pos = r_begin[lo] + (t - r_pref[lo]);
+        const unsigned char fl = r_flag[lo];
+        int row, code;
+        float4 p;
+        if (fl & 1) { row = gs_src[pos]; code = gs_code[pos]; p = image_pos(xq[row], code, bl, m_bl); }
+        else { row = pos; code = 13; p = xq[pos]; }
+        const int4 in = info[row];
+        c_pos[u] = p;
+        c_ent[u] = NNPE_MAKE_ENTRY(row, code, 0) | ((fl & 2) ? 0x80000000u : 0u);
+        c_mol[u] = make_int2(in.y, in.z);
+      }
+      __syncthreads();
+      for (int h = h0 + wave; h < h1; h += blockDim.x / 64) {
+        const float4 xi = xq[h];
+        const int4 ii = info[h];
+        const int mtn = mt_natom[ii.w];
+        const int* sp_row = mt_sp + mt_off[ii.w] + ii.z * mtn;
+        int nl = h_nl[h - h0], ns = h_ns[h - h0];
+        uint32_t* rowl = nbr + (size_t)h * cap;
+        uint32_t* rows = kShort ? nbr_s + (size_t)h * cap_s : nullptr;
+        for (int u0 = 0; u0 < tn; u0 += 64) {
+          const int u = u0 + lane;
+          bool take = false, take_s = false;
+          uint32_t e = 0;
+          if (u < tn) {
+            const uint32_t ce = c_ent[u];
+            const int j = NNPE_ENTRY_J(ce);
+            const bool ghost = NNPE_ENTRY_CODE(ce) != 13;
+            if (ghost || j != h) {
+              const float4 xj = c_pos[u];
+              const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+              take = (ce & 0x80000000u) || rsq < cutsq;
+              take_s = kShort && take && rsq < cutsq_s;
+              const int2 mj = c_mol[u];
+              const int sp = (mj.x == ii.y) ? sp_row[mj.y] : 0;
+              e = (ce & 0x7FFFFFFFu) | ((uint32_t)sp << 30);
+            }
+          }
+          const unsigned long long bal = __ballot(take);
+          const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+          if (take && nl + rank < cap) rowl[nl + rank] = e;
+          nl += __popcll(bal);
+          if (kShort) {
+            const unsigned long long bs = __ballot(take_s);
+            const int rs = __builtin_amdgcn_mbcnt_hi((unsigned)(bs >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bs, 0));
+            if (take_s && ns + rs < cap_s) rows[ns + rs] = e;
+            ns += __popcll(bs);
+          }
+        }
+        if (lane == 0) { h_nl[h - h0] = nl; h_ns[h - h0] = ns; }
+      }
+    }
+    __syncthreads();
+    for (int k = tid; k < h1 - h0; k += blockDim.x) {
+      const int h = h0 + k, nl = h_nl[k];
+      xprev[h] = xq[h];
+      cnt_out[h] = min(nl, cap);
+      if (nl > cap) atomicMax(&counters->overflow_long, nl);
+      atomicMax(&counters->max_long, nl);
+      if (kShort) {
+        const int ns = h_ns[k];
+        cnt_s_out[h] = min(ns, cap_s);
+        if (ns > cap_s) atomicMax(&counters->overflow_short, ns);
+        atomicMax(&counters->max_short, ns);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -294,14 +399,14 @@ __global__ void k_isvalid(int n, float dx2, const float4* xq, const float4* xpre
 __global__ void k_clear_invalid(DevCounters* cnt) { cnt->invalid = 0; }
 
 // list export in atom ids
-__global__ void k_export(int n, int stride, const uint32_t* nbr, const int* cnt, const long long* offs, const int4* info,
+__global__ void k_export(int n, int cap, const uint32_t* nbr, const int* cnt, const long long* offs, const int4* info,
                          int* ai, int* aj, int* code, int* sp) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   long long o = offs[i];
   int a = info[i].x;
   for (int k = 0; k < cnt[i]; ++k) {
-    uint32_t e = nbr[(size_t)k * stride + i];
+    uint32_t e = nbr[(size_t)i * cap + k];
     ai[o + k] = a;
     aj[o + k] = info[NNPE_ENTRY_J(e)].x;
     code[o + k] = NNPE_ENTRY_CODE(e);
@@ -419,7 +524,8 @@ int neigh_enqueue_build(nnpe_ctx* ctx) {
                      ctx->g_rdx, ctx->num_gx, ctx->xq, ctx->xd, ctx->t_cell, ctx->cell_cnt, ctx->counters);
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->cell_cnt, ctx->cell_start, G);
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->t_cell, ctx->cell_start, ctx->cell_fill, ctx->order);
-  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(G, 64)), dim3(64), 0, s, G, ctx->cell_start, ctx->order, ctx->info);
+  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for((long long)G * 64, 256)), dim3(256), 0, s, G, ctx->cell_start, ctx->order,
+                     ctx->info);
   hipLaunchKernelGGL(k_gather, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->order, ctx->t_cell, ctx->xq, ctx->v,
                      ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->t_xq, ctx->t_v, ctx->t_fprev,
                      ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->cell, ctx->inv);
@@ -435,23 +541,25 @@ int neigh_enqueue_build(nnpe_ctx* ctx) {
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->gcell_cnt, ctx->gcell_start, G);
   hipLaunchKernelGGL(k_ghost_scatter, dim3(blocks_for(ctx->ghost_cap, bs)), dim3(bs), 0, s, ctx->ghost_cap, ctx->counters,
                      ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_start, ctx->gcell_fill, ctx->gs_src, ctx->gs_code);
-  hipLaunchKernelGGL(k_ghost_cell_sort, dim3(blocks_for(G, 64)), dim3(64), 0, s, G, ctx->gcell_start, ctx->gs_src, ctx->gs_code,
-                     ctx->info);
+  hipLaunchKernelGGL(k_ghost_cell_sort, dim3(blocks_for((long long)G * 64, 256)), dim3(256), 0, s, G, ctx->gcell_start,
+                     ctx->gs_src, ctx->gs_code, ctx->info);
   hipLaunchKernelGGL(k_xmol, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->ghost_lim, ctx->bl, ctx->m_bl, ctx->h_bl, ctx->xq,
                      ctx->info, ctx->mol_first, ctx->inv, ctx->xmol_code);
+  if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
   if (ctx->cap_short) {
-    hipLaunchKernelGGL(k_build<true>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->stride, ctx->num_gx, ctx->stencil_pair,
+    hipLaunchKernelGGL(k_build_cells<true>, dim3(G), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_pair,
                        ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
-                       ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom,
-                       ctx->mt_special_off, ctx->mt_special, ctx->cap_long, ctx->cap_short, ctx->nbr_long, ctx->cnt_long,
-                       ctx->nbr_short, ctx->cnt_short, ctx->xprev, ctx->counters);
+                       ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
+                       ctx->mt_special, ctx->cap_long, ctx->cap_short, ctx->nbr_long, ctx->cnt_long, ctx->nbr_short,
+                       ctx->cnt_short, ctx->xprev, ctx->counters);
   } else {
-    hipLaunchKernelGGL(k_build<false>, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, ctx->stride, ctx->num_gx, ctx->stencil_pair,
-                       ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
-                       ctx->cell, ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom,
-                       ctx->mt_special_off, ctx->mt_special, ctx->cap_long, ctx->cap_short, ctx->nbr_long, ctx->cnt_long,
-                       ctx->nbr_short, ctx->cnt_short, ctx->xprev, ctx->counters);
+    hipLaunchKernelGGL(k_build_cells<false>, dim3(G), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_pair,
+                       ctx->n_stencil_pair, ctx->cutsq_list, 0.0f, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
+                       ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
+                       ctx->mt_special, ctx->cap_long, 0, ctx->nbr_long, ctx->cnt_long, nullptr, nullptr, ctx->xprev,
+                       ctx->counters);
   }
+
   NNPE_CHECK_LAUNCH(ctx);
   ctx->nbuild++;
   return type_index_enqueue(ctx);  // rows of each element moved with the sort
@@ -559,7 +667,7 @@ extern "C" int nnpe_neigh_export(nnpe_ctx* ctx, int which, int* ai, int* aj, int
   NNPE_CHECK_HIP(ctx, dev_alloc(dai, tot)); NNPE_CHECK_HIP(ctx, dev_alloc(daj, tot));
   NNPE_CHECK_HIP(ctx, dev_alloc(dc, tot)); NNPE_CHECK_HIP(ctx, dev_alloc(ds, tot));
   NNPE_CHECK_HIP(ctx, hipMemcpy(doffs, offs.data(), sizeof(long long) * N, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_export, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->stride,
+  hipLaunchKernelGGL(k_export, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, which ? ctx->cap_short : ctx->cap_long,
                      which ? ctx->nbr_short : ctx->nbr_long, dcnt, doffs, ctx->info, dai, daj, dc, ds);
   NNPE_CHECK_LAUNCH(ctx);
   NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));

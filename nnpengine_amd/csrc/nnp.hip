@@ -31,6 +31,8 @@ constexpr int kNRad = 16;
 constexpr int kNEta = 4;
 constexpr int kNMom = 11;      // A0, A1[3], A2[xx,yy,zz,xy,xz,yz], S
 constexpr float kPi = 3.14159265358979f;
+constexpr int kTblG = 16;      // lanes per atom, tabulated form
+constexpr int kSfG = 16;       // lanes per atom, symmetry functions
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   int i = min(a, b), j = max(a, b);
@@ -40,22 +42,26 @@ __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
 // ---------------------------------------------------------------------------------------------
 // (1) tabulated reference form
 
-// NNPOTENTIAL_TBL::computeDescriptor over a full list: each atom sums its own neighbours
-__global__ void __launch_bounds__(256) k_tbl_desc(int n, int stride, const uint32_t* __restrict__ nbr,
+// NNPOTENTIAL_TBL::computeDescriptor over a full list: each atom sums its own neighbours (a
+// group of G lanes per atom, butterfly-reduced)
+template <int G>
+__global__ void __launch_bounds__(256) k_tbl_desc(int n, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                   const int* __restrict__ typ, int ntype, float bl, float m_bl,
                                                   float cutsq, float rdr, int nb1, const float* __restrict__ tdesc,
-                                                  float* __restrict__ G) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                                                  float* __restrict__ G_out) {
+  const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  if (i >= n) return;  // whole group
   const float4 xi = xq[i];
   const int ti = typ[i];
   float acc[kND];
 #pragma unroll
   for (int k = 0; k < kND; ++k) acc[k] = 0.0f;
   const int m = cnt[i];
-  for (int e = 0; e < m; ++e) {
-    const uint32_t ent = nbr[(size_t)e * stride + i];
+  const uint32_t* row = list_row(nbr, cap, i);
+  for (int e = lane; e < m; e += G) {
+    const uint32_t ent = row[e];
     const int j = NNPE_ENTRY_J(ent);
     const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -72,19 +78,23 @@ __global__ void __launch_bounds__(256) k_tbl_desc(int n, int stride, const uint3
     for (int k = 0; k < kND; ++k) acc[k] += t[k] * ipw1 + t[kND + k] * ipw2;
   }
 #pragma unroll
-  for (int k = 0; k < kND; ++k) G[(size_t)i * kND + k] = acc[k];
+  for (int k = 0; k < kND; ++k) {
+    const float v = group_sum<G>(acc[k]);
+    if (k % G == lane) G_out[(size_t)i * kND + k] = v;
+  }
 }
 
 // NNPOTENTIAL_TBL::compute<ev> over a full list (each directed entry: half energy / virial)
-template <bool EV>
-__global__ void __launch_bounds__(256) k_tbl_force(int n, int stride, const uint32_t* __restrict__ nbr,
+template <bool EV, int G>
+__global__ void __launch_bounds__(256) k_tbl_force(int n, int cap, const uint32_t* __restrict__ nbr,
                                                    const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                    const int* __restrict__ typ, int ntype, float bl, float m_bl,
                                                    float cutsq, float rdr, int nb1, const float* __restrict__ tfdesc,
                                                    const float* __restrict__ tpair, const float* __restrict__ tfpair,
                                                    const float* __restrict__ dG, float4* __restrict__ f, double* pe,
                                                    double* pw) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
   double ea = 0, wa = 0;
   if (i < n) {
     const float4 xi = xq[i];
@@ -94,8 +104,9 @@ __global__ void __launch_bounds__(256) k_tbl_force(int n, int stride, const uint
     for (int k = 0; k < kND; ++k) gi[k] = dG[(size_t)i * kND + k];
     float fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     const int m = cnt[i];
-    for (int q = 0; q < m; ++q) {
-      const uint32_t ent = nbr[(size_t)q * stride + i];
+    const uint32_t* row = list_row(nbr, cap, i);
+    for (int q = lane; q < m; q += G) {
+      const uint32_t ent = row[q];
       const int j = NNPE_ENTRY_J(ent), sp = NNPE_ENTRY_SP(ent);
       const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
       float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -120,9 +131,12 @@ __global__ void __launch_bounds__(256) k_tbl_force(int n, int stride, const uint
         w += 0.5f * fpair * rsq;
       }
     }
-    float4 fo = f[i];
-    fo.x += fx; fo.y += fy; fo.z += fz;
-    f[i] = fo;
+    fx = group_sum<G>(fx); fy = group_sum<G>(fy); fz = group_sum<G>(fz);
+    if (lane == 0) {
+      float4 fo = f[i];
+      fo.x += fx; fo.y += fy; fo.z += fz;
+      f[i] = fo;
+    }
     ea = e; wa = w;
   }
   if (EV) block_sum2_store<256>(ea, wa, pe, pw);
@@ -144,102 +158,105 @@ __device__ __forceinline__ void sf_cut(float r, float rc, float& fc, float& dfc)
   dfc = -0.5f * kPi / rc * s;
 }
 
-// moments per atom: [NT][kNEta][kNMom] then radial [NT][kNRad]
-template <int NT>
-__global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, const uint32_t* __restrict__ nbr,
+// moments per atom: [NT][kNEta][kNMom] then radial [NT][kNRad].  A group of G lanes per atom,
+// one pass over the atom's list row per neighbour element a (so only that element's 60
+// accumulators are live): each lane accumulates the moments of its share of the neighbours, the
+// group butterfly-sums them, and the lanes split the stores (lane l writes the values whose
+// index is l mod G) to the moment rows and to an LDS copy, from which the angular descriptors
+// are formed at the end.
+template <int NT, int G>
+__global__ void __launch_bounds__(256) k_sf_desc(int n, int cap, const uint32_t* __restrict__ nbr,
                                                  const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                  const int* __restrict__ typ, float bl, float m_bl, SfParams P,
-                                                 float* __restrict__ Mout, float* __restrict__ G) {
+                                                 float* __restrict__ Mout, float* __restrict__ G_out) {
   constexpr int NP = NT * (NT + 1) / 2;
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
+  constexpr int RAD0 = NT * kNEta * kNMom;
   constexpr int D = NT * kNRad + NP * kNEta * 4;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 xi = xq[i];
-  float A[NT][kNEta][kNMom];
-  float R[NT][kNRad];
-#pragma unroll
-  for (int a = 0; a < NT; ++a) {
-#pragma unroll
-    for (int m = 0; m < kNEta; ++m)
-#pragma unroll
-      for (int c = 0; c < kNMom; ++c) A[a][m][c] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kNRad; ++k) R[a][k] = 0.0f;
-  }
-  const int cn = cnt[i];
-  for (int q = 0; q < cn; ++q) {
-    const uint32_t ent = nbr[(size_t)q * stride + i];
-    const int j = NNPE_ENTRY_J(ent);
-    const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
-    const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
-    const float rsq = ux * ux + uy * uy + uz * uz;
-    if (!(rsq < P.rcsq)) continue;
-    const float r = sqrtf(rsq), ir = 1.0f / r;
-    const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
-    float fc, dfc;
-    sf_cut(r, P.rc, fc, dfc);
-    const int tj = typ[j];
-    float rad[kNRad];
-#pragma unroll
-    for (int k = 0; k < kNRad; ++k) {
-      const float d = r - P.mu[k];
-      rad[k] = __expf(-P.eta_r * d * d) * fc;
-    }
-    float wv[kNEta];
-#pragma unroll
-    for (int m = 0; m < kNEta; ++m) wv[m] = __expf(-P.eta[m] * rsq) * fc;
+  __shared__ float s_mom[256 / G][NMT];
+  const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
+  const bool live = i < n;
+  float* sm = s_mom[grp];
+  float* mo = Mout + (size_t)(live ? i : 0) * NMT;
+  if (live) {
+    const float4 xi = xq[i];
+    const int cn = cnt[i];
+    const uint32_t* row = list_row(nbr, cap, i);
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
-      if (tj == a) {
+      float A[kNEta][kNMom];
+      float R[kNRad];
 #pragma unroll
-        for (int k = 0; k < kNRad; ++k) R[a][k] += rad[k];
+      for (int m = 0; m < kNEta; ++m)
+#pragma unroll
+        for (int c = 0; c < kNMom; ++c) A[m][c] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kNRad; ++k) R[k] = 0.0f;
+      for (int q = lane; q < cn; q += G) {
+        const uint32_t ent = row[q];
+        const int j = NNPE_ENTRY_J(ent);
+        if (NT > 1 && typ[j] != a) continue;
+        const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
+        const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
+        const float rsq = ux * ux + uy * uy + uz * uz;
+        if (!(rsq < P.rcsq)) continue;
+        const float r = sqrtf(rsq), ir = 1.0f / r;
+        const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
+        float fc, dfc;
+        sf_cut(r, P.rc, fc, dfc);
+#pragma unroll
+        for (int k = 0; k < kNRad; ++k) {
+          const float d = r - P.mu[k];
+          R[k] += __expf(-P.eta_r * d * d) * fc;
+        }
 #pragma unroll
         for (int m = 0; m < kNEta; ++m) {
-          const float w = wv[m], wx = w * hx, wy = w * hy, wz = w * hz;
-          A[a][m][0] += w;
-          A[a][m][1] += wx; A[a][m][2] += wy; A[a][m][3] += wz;
-          A[a][m][4] += wx * hx; A[a][m][5] += wy * hy; A[a][m][6] += wz * hz;
-          A[a][m][7] += wx * hy; A[a][m][8] += wx * hz; A[a][m][9] += wy * hz;
-          A[a][m][10] += w * w;
+          const float w = __expf(-P.eta[m] * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
+          A[m][0] += w;
+          A[m][1] += wx; A[m][2] += wy; A[m][3] += wz;
+          A[m][4] += wx * hx; A[m][5] += wy * hy; A[m][6] += wz * hz;
+          A[m][7] += wx * hy; A[m][8] += wx * hz; A[m][9] += wy * hz;
+          A[m][10] += w * w;
         }
+      }
+#pragma unroll
+      for (int m = 0; m < kNEta; ++m)
+#pragma unroll
+        for (int c = 0; c < kNMom; ++c) {
+          const float v = group_sum<G>(A[m][c]);
+          const int idx = (a * kNEta + m) * kNMom + c;
+          if (idx % G == lane) { mo[idx] = v; sm[idx] = v; }
+        }
+#pragma unroll
+      for (int k = 0; k < kNRad; ++k) {
+        const float v = group_sum<G>(R[k]);
+        const int idx = RAD0 + a * kNRad + k;
+        if (idx % G == lane) { mo[idx] = v; sm[idx] = v; }
       }
     }
   }
-  float* mo = Mout + (size_t)i * NMT;
-#pragma unroll
-  for (int a = 0; a < NT; ++a)
-#pragma unroll
-    for (int m = 0; m < kNEta; ++m)
-#pragma unroll
-      for (int c = 0; c < kNMom; ++c) mo[(a * kNEta + m) * kNMom + c] = A[a][m][c];
-#pragma unroll
-  for (int a = 0; a < NT; ++a)
-#pragma unroll
-    for (int k = 0; k < kNRad; ++k) mo[NT * kNEta * kNMom + a * kNRad + k] = R[a][k];
-  float* g = G + (size_t)i * D;
-#pragma unroll
-  for (int a = 0; a < NT; ++a)
-#pragma unroll
-    for (int k = 0; k < kNRad; ++k) g[a * kNRad + k] = R[a][k];
-  int pp = 0;
-#pragma unroll
-  for (int a = 0; a < NT; ++a)
-#pragma unroll
-    for (int b = a; b < NT; ++b, ++pp)
-#pragma unroll
-      for (int m = 0; m < kNEta; ++m) {
-        const float* X = A[a][m];
-        const float* Y = A[b][m];
-        float P0 = X[0] * Y[0], P1 = X[1] * Y[1] + X[2] * Y[2] + X[3] * Y[3];
-        float P2 = X[4] * Y[4] + X[5] * Y[5] + X[6] * Y[6] + 2.0f * (X[7] * Y[7] + X[8] * Y[8] + X[9] * Y[9]);
-        if (a == b) { P0 -= X[10]; P1 -= X[10]; P2 -= X[10]; }
-        float* o = g + NT * kNRad + (pp * kNEta + m) * 4;
-        o[0] = P0 + P1;
-        o[1] = P0 - P1;
-        o[2] = 0.5f * (P0 + 2.0f * P1 + P2);
-        o[3] = 0.5f * (P0 - 2.0f * P1 + P2);
-      }
+  __syncthreads();
+  if (!live) return;
+  float* g = G_out + (size_t)i * D;
+  for (int k = lane; k < NT * kNRad; k += G) g[k] = sm[RAD0 + k];
+  // angular: pair (a, b), exponent m -> 4 values; entry t of the NP * kNEta list per lane
+  for (int t = lane; t < NP * kNEta; t += G) {
+    const int pp = t / kNEta, m = t - pp * kNEta;
+    int a = 0, b = 0, rem = pp;
+    while (rem >= NT - a) { rem -= NT - a; ++a; }
+    b = a + rem;
+    const float* X = sm + (a * kNEta + m) * kNMom;
+    const float* Y = sm + (b * kNEta + m) * kNMom;
+    float P0 = X[0] * Y[0], P1 = X[1] * Y[1] + X[2] * Y[2] + X[3] * Y[3];
+    float P2 = X[4] * Y[4] + X[5] * Y[5] + X[6] * Y[6] + 2.0f * (X[7] * Y[7] + X[8] * Y[8] + X[9] * Y[9]);
+    if (a == b) { P0 -= X[10]; P1 -= X[10]; P2 -= X[10]; }
+    float* o = g + NT * kNRad + t * 4;
+    o[0] = P0 + P1;
+    o[1] = P0 - P1;
+    o[2] = 0.5f * (P0 + 2.0f * P1 + P2);
+    o[3] = 0.5f * (P0 - 2.0f * P1 + P2);
+  }
 }
 
 // dE/d(moments) of each atom from dE/dG and its moments
@@ -313,24 +330,32 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
   }
 }
 
-template <int NT, bool EV>
-__global__ void __launch_bounds__(256) k_sf_force(int n, int stride, const uint32_t* __restrict__ nbr,
+// A group of G lanes per atom; the atom's own adjoint row is staged in LDS (one row per group),
+// each lane gathers the neighbour's element slice of its pairs from global memory.
+template <int NT, bool EV, int G>
+__global__ void __launch_bounds__(256) k_sf_force(int n, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                   const int* __restrict__ typ, float bl, float m_bl, SfParams P,
                                                   const float* __restrict__ adj, float4* __restrict__ f, double* pe,
                                                   double* pw) {
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int RAD0 = NT * kNEta * kNMom;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float s_adj[256 / G][NMT];
+  const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
+  if (i < n)
+    for (int k = lane; k < NMT; k += G) s_adj[grp][k] = adj[(size_t)i * NMT + k];
+  __syncthreads();
   double wacc = 0;
   if (i < n) {
     const float4 xi = xq[i];
     const int ti = typ[i];
-    const float* ai = adj + (size_t)i * NMT;
+    const float* ai = s_adj[grp];
     float fx = 0, fy = 0, fz = 0, vir = 0;
     const int cn = cnt[i];
-    for (int q = 0; q < cn; ++q) {
-      const uint32_t ent = nbr[(size_t)q * stride + i];
+    const uint32_t* row = list_row(nbr, cap, i);
+    for (int q = lane; q < cn; q += G) {
+      const uint32_t ent = row[q];
       const int j = NNPE_ENTRY_J(ent);
       const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
       const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
@@ -350,14 +375,15 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, const uint3
       }
       const float* aj = adj + (size_t)j * NMT;
       // radial: sum_n g_n'(r) (dR_i[tj][n] + dR_j[ti][n])
-      float rad = 0.0f;
+      float rad = 0.0f, radi = 0.0f;
       const float* ri = ai + RAD0 + tj * kNRad;
       const float* rj = aj + RAD0 + ti * kNRad;
 #pragma unroll
       for (int k = 0; k < kNRad; ++k) {
         const float d = r - P.mu[k];
-        const float ex = __expf(-P.eta_r * d * d);
-        rad += (ri[k] + rj[k]) * ex * (dfc - 2.0f * P.eta_r * d * fc);
+        const float g = __expf(-P.eta_r * d * d) * (dfc - 2.0f * P.eta_r * d * fc);
+        rad += (ri[k] + rj[k]) * g;
+        if (EV) radi += ri[k] * g;
       }
       // own adjoint, direction +h
       float al_i = 0, gix = 0, giy = 0, giz = 0;
@@ -371,20 +397,15 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, const uint3
       fx += gxi - gxj + rad * hx;
       fy += gyi - gyj + rad * hy;
       fz += gzi - gzj + rad * hz;
-      if (EV) {
-        // W_i = -(dE_i/du . u), own descriptor only (the radial share of E_i is dR_i part)
-        float radi = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kNRad; ++k) {
-          const float d = r - P.mu[k];
-          radi += ri[k] * __expf(-P.eta_r * d * d) * (dfc - 2.0f * P.eta_r * d * fc);
-        }
-        vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
-      }
+      // W_i = -(dE_i/du . u), own descriptor only
+      if (EV) vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
     }
-    float4 fo = f[i];
-    fo.x += fx; fo.y += fy; fo.z += fz;
-    f[i] = fo;
+    fx = group_sum<G>(fx); fy = group_sum<G>(fy); fz = group_sum<G>(fz);
+    if (lane == 0) {
+      float4 fo = f[i];
+      fo.x += fx; fo.y += fy; fo.z += fz;
+      f[i] = fo;
+    }
     wacc = vir;
   }
   if (EV) block_sum2_store<256>(0.0, wacc, pe, pw);
@@ -512,8 +533,8 @@ extern "C" int nnpe_nnp_tbl_set(nnpe_ctx* ctx, float cutoff, int nbin, const flo
 
 int nnp_tbl_desc_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
-  hipLaunchKernelGGL(k_tbl_desc, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride,
-                     ctx->nbr_long, ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq,
+  hipLaunchKernelGGL(k_tbl_desc<kTblG>, dim3(blocks_for((long long)ctx->natom * kTblG, 256)), dim3(256), 0, ctx->stream,
+                     ctx->natom, ctx->cap_long, ctx->nbr_long, ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq,
                      S.tbl_rdr, S.tbl_nbin + 1, S.tdesc, nnp_G(ctx));
   NNPE_CHECK_LAUNCH(ctx);
   nnp_set_current_D(ctx, kND);
@@ -522,15 +543,15 @@ int nnp_tbl_desc_enqueue(nnpe_ctx* ctx) {
 
 int nnp_tbl_force_enqueue(nnpe_ctx* ctx, bool ev) {
   NnpState& S = nnp_state(ctx);
-  const int nblk = blocks_for(ctx->natom, 256);
+  const int nblk = blocks_for((long long)ctx->natom * kTblG, 256);
   double* pe = ev_part(ctx, NNPE_TERM_NNP_PAIR, 0);
   double* pw = ev_part(ctx, NNPE_TERM_NNP_PAIR, 1);
   if (ev)
-    hipLaunchKernelGGL(k_tbl_force<true>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, ctx->nbr_long,
+    hipLaunchKernelGGL((k_tbl_force<true, kTblG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->cap_long, ctx->nbr_long,
                        ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq, S.tbl_rdr,
                        S.tbl_nbin + 1, S.tfdesc, S.tpair, S.tfpair, nnp_dG(ctx), ctx->f, pe, pw);
   else
-    hipLaunchKernelGGL(k_tbl_force<false>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, ctx->nbr_long,
+    hipLaunchKernelGGL((k_tbl_force<false, kTblG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->cap_long, ctx->nbr_long,
                        ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq, S.tbl_rdr,
                        S.tbl_nbin + 1, S.tfdesc, S.tpair, S.tfpair, nnp_dG(ctx), ctx->f, pe, pw);
   NNPE_CHECK_LAUNCH(ctx);
@@ -573,11 +594,12 @@ int sf_dim(int ntype) { return ntype * kNRad + ntype * (ntype + 1) / 2 * kNEta *
 
 int sf_desc_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
-  const int nblk = blocks_for(ctx->natom, 256);
+  const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
   const uint32_t* nbr = ctx->cap_short ? ctx->nbr_short : ctx->nbr_long;
   const int* cnt = ctx->cap_short ? ctx->cnt_short : ctx->cnt_long;
-#define NNPE_SF_DESC(NT)                                                                                          \
-  hipLaunchKernelGGL(k_sf_desc<NT>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, nbr, cnt, ctx->xq, \
+  const int cap = ctx->cap_short ? ctx->cap_short : ctx->cap_long;
+#define NNPE_SF_DESC(NT)                                                                                              \
+  hipLaunchKernelGGL((k_sf_desc<NT, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, cap, nbr, cnt, ctx->xq, \
                      ctx->typ, ctx->bl, ctx->m_bl, S.P, S.M, nnp_G(ctx))
   switch (ctx->ntype) {
     case 1: NNPE_SF_DESC(1); break;
@@ -610,13 +632,14 @@ int sf_adjoint_enqueue(nnpe_ctx* ctx) {
 // the pair-force kernel alone (uses the adjoints of the last sf_adjoint_enqueue)
 int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
   NnpState& S = nnp_state(ctx);
-  const int nblk = blocks_for(ctx->natom, 256);
+  const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
   const uint32_t* nbr = ctx->cap_short ? ctx->nbr_short : ctx->nbr_long;
   const int* cnt = ctx->cap_short ? ctx->cnt_short : ctx->cnt_long;
+  const int cap = ctx->cap_short ? ctx->cap_short : ctx->cap_long;
   double* pe = ev_part(ctx, NNPE_TERM_SF, 0);
   double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
-#define NNPE_SF_FORCE(NT, EV)                                                                                        \
-  hipLaunchKernelGGL((k_sf_force<NT, EV>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, nbr, cnt, \
+#define NNPE_SF_FORCE(NT, EV)                                                                                         \
+  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, cap, nbr, cnt, \
                      ctx->xq, ctx->typ, ctx->bl, ctx->m_bl, S.P, S.adj, ctx->f, pe, pw)
   switch (ctx->ntype) {
     case 1: if (ev) NNPE_SF_FORCE(1, true); else NNPE_SF_FORCE(1, false); break;
