@@ -227,14 +227,15 @@ def main():
                 "verlet": 1.0, "rebuild": 0.1, "gr_full": 0.1, "sq": 0.1}
     share = {k: stages[k] * f for k, f in per_step.items()}
     dom = max(share, key=share.get)
-    nmt = sysm.ntype * 4 * 11 + sysm.ntype * 16
+    nmt = sysm.ntype * 4 * 11 + sysm.ntype * 16   # moments per atom
+    nadj = sysm.ntype * 64                        # adjoint row: one 64-float slice per element
     D = WIDTHS[0]
     # algorithmic HBM bytes per launch: each input read once, each output written once
     # (list entries 4 B; xq 16 B, typ 4 B, f 16 B per atom; moments/adjoints 4*nmt B; G/dG 4*D B)
     alg_bytes = {"pair": 4 * n_long + N * (16 + 4 + 16),
                  "sf_desc": 4 * n_short + N * (16 + 4 + 4 * nmt + 4 * D),
-                 "sf_adjoint": N * 4 * (nmt + D + nmt),
-                 "sf_force": 4 * n_short + N * (16 + 4 + 4 * nmt + 16 + 16),
+                 "sf_adjoint": N * 4 * (nmt + D + nadj),
+                 "sf_force": 4 * n_short + N * (16 + 4 + 4 * nadj + 16 + 16),
                  "bonded": N * (16 + 16 + 16),
                  "verlet": N * 164,
                  "rebuild": 4 * (n_long + n_short) + N * 2 * (16 + 16 + 16 + 4 + 4 + 16 + 3 * 32),

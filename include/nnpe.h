@@ -65,7 +65,8 @@ int nnpe_state_set(nnpe_ctx* ctx, int what, const double* in);
 /* ENVIRON::init_grid + NEIGHLIST::build_bin_list + LOCAL::set_ghost_lim
  * (src/environ.cpp:143-155, src/neighlist.cpp:35-168, src/run.cpp:72-82).
  * comm cutoff = skin + max(gr_cutoff, pair_cutoff); cell size = (pair_cutoff + skin) / 2.
- * short_cutoff > 0 also builds a second list to short_cutoff + skin (NNP environments). */
+ * short_cutoff > 0 also builds short lists to short_cutoff + skin, one per neighbour element (the
+ * symmetry-function NNP environments need them). */
 int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, float gr_cutoff, float short_cutoff);
 /* Rebuild: LOCAL::refold_and_tag, ENVIRON::sort, generate_ghost, tidyup_ghost,
  * update_ghost_pos<true>, sort_ghost, update_ghost_and_info, NEIGHLIST::build<true>
@@ -151,8 +152,11 @@ int nnpe_nnp_tbl_force(nnpe_ctx* ctx, int ev);
  * per neighbour element; 4 angular exponents eta[4] x (zeta, lambda) in {1,2} x {+1,-1} per
  * element pair), cutoff rc with fc = (cos(pi r/rc)+1)/2.  D = 16*ntype + 16*npair. */
 int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, float eta_r, int neta, const float* eta);
+/* Descriptors of the current positions; also records the pairs within rc for nnpe_sf_force. */
 int nnpe_sf_descriptor(nnpe_ctx* ctx);
-/* Symmetry-function forces from the current dE/dG; adds to the force, virial in NNPE_TERM_SF. */
+/* Symmetry-function forces from the current dE/dG over the pairs recorded by the last
+ * nnpe_sf_descriptor (the reference's computeDescriptor -> nnEval -> compute order); adds to
+ * the force, virial in NNPE_TERM_SF. */
 int nnpe_sf_force(nnpe_ctx* ctx, int ev);
 
 /* TFCALL buffers (src/pair_nnp.cpp:490-496, :697-706) in atom-id order: what 0 = descriptors

@@ -173,6 +173,8 @@ int sf_dim(int ntype);
 int type_index_enqueue(nnpe_ctx* ctx);
 // NNP pair tables being refined: force / potential [npair][4][nbin+1]; nonzero if not set
 int nnp_tbl_tables(nnpe_ctx* ctx, float** ftab, float** utab, int* nbin, float* dr);
+// symmetry-function active lists sized to the short lists' capacity
+int nnp_act_reserve(nnpe_ctx* ctx);
 
 // RMDF (rmdf.hip)
 void rmdf_state_free(nnpe_ctx* ctx);

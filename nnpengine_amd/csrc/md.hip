@@ -134,6 +134,7 @@ extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, d
   if (P.pair && !ctx->have_pair) return ctx_fail(ctx, NNPE_ERR_STATE, "pair term without a table");
   if (P.nnp && !ctx->mlp) return ctx_fail(ctx, NNPE_ERR_STATE, "NNP term without nnpe_mlp_set");
   if (P.nnp == 1 && (!ctx->nnp || !nnp_state_of(ctx) || ctx->mlp == nullptr)) return ctx_fail(ctx, NNPE_ERR_STATE, "NNP tables missing");
+  if (P.nnp == 2 && !ctx->cap_short) return ctx_fail(ctx, NNPE_ERR_STATE, "symmetry functions need the short lists (nnpe_neigh_init short_cutoff)");
   if ((P.gr_every > 0) && !ctx->gr_count) return ctx_fail(ctx, NNPE_ERR_STATE, "g(r) requested without nnpe_gr_init");
   if (P.sq && !ctx->sq_ftmx) return ctx_fail(ctx, NNPE_ERR_STATE, "S(q) requested without nnpe_sq_init");
   if (P.rmdf && (!rmdf_active(ctx) || !P.sq || P.gr_every <= 0))

@@ -253,6 +253,7 @@ __global__ void k_xmol(int n, float ghost_lim, float bl, float m_bl, float h_bl,
 constexpr int kBuildBatch = 1024;  // candidates staged per pass
 constexpr int kBuildHome = 64;     // home atoms per pass
 constexpr int kBuildMaxRanges = 2 * 343;
+constexpr int kShortMaxType = 4;   // short lists: one per neighbour element
 
 template <bool kShort>
 __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, const int4* __restrict__ stencil, int nst,
@@ -261,7 +262,8 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
                                                      const int* __restrict__ cstart, const int* __restrict__ gstart,
                                                      const int* __restrict__ gs_src, const int* __restrict__ gs_code,
                                                      const int* __restrict__ mt_natom, const int* __restrict__ mt_off,
-                                                     const int* __restrict__ mt_sp, int cap, int cap_s,
+                                                     const int* __restrict__ mt_sp, const int* __restrict__ typ,
+                                                     int ntype, int stride, int cap, int cap_s,
                                                      uint32_t* __restrict__ nbr, int* __restrict__ cnt_out,
                                                      uint32_t* __restrict__ nbr_s, int* __restrict__ cnt_s_out,
                                                      float4* __restrict__ xprev, DevCounters* counters) {
@@ -270,7 +272,8 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
   __shared__ float4 c_pos[kBuildBatch];
   __shared__ uint32_t c_ent[kBuildBatch];            // row | code << 25 | nocheck << 31
   __shared__ int2 c_mol[kBuildBatch];                // molecule id, intra id
-  __shared__ int h_nl[kBuildHome], h_ns[kBuildHome];
+  __shared__ unsigned char c_typ[kBuildBatch];
+  __shared__ int h_nl[kBuildHome], h_ns[kBuildHome][kShortMaxType];
   __shared__ int s_tot;
   const int c = xcd_block(gridDim.x);
   if (c >= ncell) return;
@@ -303,7 +306,10 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
   const int total = s_tot, nr2 = 2 * nst;
   for (int h0 = hb; h0 < he; h0 += kBuildHome) {
     const int h1 = min(he, h0 + kBuildHome);
-    for (int k = tid; k < h1 - h0; k += blockDim.x) { h_nl[k] = 0; h_ns[k] = 0; }
+    for (int k = tid; k < h1 - h0; k += blockDim.x) {
+      h_nl[k] = 0;
+      for (int e = 0; e < kShortMaxType; ++e) h_ns[k][e] = 0;
+    }
     for (int t0 = 0; t0 < total; t0 += kBuildBatch) {
       const int tn = min(kBuildBatch, total - t0);
       __syncthreads();
@@ -324,6 +330,7 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
         c_pos[u] = p;
         c_ent[u] = NNPE_MAKE_ENTRY(row, code, 0) | ((fl & 2) ? 0x80000000u : 0u);
         c_mol[u] = make_int2(in.y, in.z);
+        c_typ[u] = (unsigned char)typ[row];
       }
       __syncthreads();
       for (int h = h0 + wave; h < h1; h += blockDim.x / 64) {
@@ -331,13 +338,16 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
         const int4 ii = info[h];
         const int mtn = mt_natom[ii.w];
         const int* sp_row = mt_sp + mt_off[ii.w] + ii.z * mtn;
-        int nl = h_nl[h - h0], ns = h_ns[h - h0];
+        int nl = h_nl[h - h0];
+        int ns[kShortMaxType];
+#pragma unroll
+        for (int e = 0; e < kShortMaxType; ++e) ns[e] = h_ns[h - h0][e];
         uint32_t* rowl = nbr + (size_t)h * cap;
-        uint32_t* rows = kShort ? nbr_s + (size_t)h * cap_s : nullptr;
         for (int u0 = 0; u0 < tn; u0 += 64) {
           const int u = u0 + lane;
           bool take = false, take_s = false;
           uint32_t e = 0;
+          int tj = 0;
           if (u < tn) {
             const uint32_t ce = c_ent[u];
             const int j = NNPE_ENTRY_J(ce);
@@ -350,6 +360,7 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
               const int2 mj = c_mol[u];
               const int sp = (mj.x == ii.y) ? sp_row[mj.y] : 0;
               e = (ce & 0x7FFFFFFFu) | ((uint32_t)sp << 30);
+              tj = c_typ[u];
             }
           }
           const unsigned long long bal = __ballot(take);
@@ -357,13 +368,20 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
           if (take && nl + rank < cap) rowl[nl + rank] = e;
           nl += __popcll(bal);
           if (kShort) {
-            const unsigned long long bs = __ballot(take_s);
-            const int rs = __builtin_amdgcn_mbcnt_hi((unsigned)(bs >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bs, 0));
-            if (take_s && ns + rs < cap_s) rows[ns + rs] = e;
-            ns += __popcll(bs);
+            for (int el = 0; el < ntype; ++el) {
+              const bool t = take_s && tj == el;
+              const unsigned long long bs = __ballot(t);
+              const int rs = __builtin_amdgcn_mbcnt_hi((unsigned)(bs >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bs, 0));
+              if (t && ns[el] + rs < cap_s) nbr_s[((size_t)el * stride + h) * cap_s + ns[el] + rs] = e;
+              ns[el] += __popcll(bs);
+            }
           }
         }
-        if (lane == 0) { h_nl[h - h0] = nl; h_ns[h - h0] = ns; }
+        if (lane == 0) {
+          h_nl[h - h0] = nl;
+#pragma unroll
+          for (int el = 0; el < kShortMaxType; ++el) h_ns[h - h0][el] = ns[el];
+        }
       }
     }
     __syncthreads();
@@ -374,10 +392,12 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
       if (nl > cap) atomicMax(&counters->overflow_long, nl);
       atomicMax(&counters->max_long, nl);
       if (kShort) {
-        const int ns = h_ns[k];
-        cnt_s_out[h] = min(ns, cap_s);
-        if (ns > cap_s) atomicMax(&counters->overflow_short, ns);
-        atomicMax(&counters->max_short, ns);
+        for (int el = 0; el < ntype; ++el) {
+          const int ns = h_ns[k][el];
+          cnt_s_out[(size_t)el * stride + h] = min(ns, cap_s);
+          if (ns > cap_s) atomicMax(&counters->overflow_short, ns);
+          atomicMax(&counters->max_short, ns);
+        }
       }
     }
     __syncthreads();
@@ -502,13 +522,14 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cnt_long, ctx->stride));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->cnt_long, 0, sizeof(int) * ctx->stride));
   if (ctx->cap_short) {
-    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_short, size_t(ctx->cap_short) * ctx->stride));
-    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cnt_short, ctx->stride));
-    NNPE_CHECK_HIP(ctx, hipMemset(ctx->cnt_short, 0, sizeof(int) * ctx->stride));
+    // one short list per neighbour element: rows of element e at nbr_short + e * cap_short * stride
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_short, size_t(ctx->ntype) * ctx->cap_short * ctx->stride));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cnt_short, size_t(ctx->ntype) * ctx->stride));
+    NNPE_CHECK_HIP(ctx, hipMemset(ctx->cnt_short, 0, sizeof(int) * ctx->ntype * ctx->stride));
   }
   ctx->have_grid = true;
   ctx->have_list = false;
-  return NNPE_OK;
+  return nnp_act_reserve(ctx);
 }
 
 // Enqueue the rebuild kernels (no host synchronisation; graph-capturable).
@@ -546,18 +567,19 @@ int neigh_enqueue_build(nnpe_ctx* ctx) {
   hipLaunchKernelGGL(k_xmol, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->ghost_lim, ctx->bl, ctx->m_bl, ctx->h_bl, ctx->xq,
                      ctx->info, ctx->mol_first, ctx->inv, ctx->xmol_code);
   if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
+  if (ctx->cap_short && ctx->ntype > kShortMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "short lists: at most 4 elements");
   if (ctx->cap_short) {
     hipLaunchKernelGGL(k_build_cells<true>, dim3(G), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_pair,
                        ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
                        ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
-                       ctx->mt_special, ctx->cap_long, ctx->cap_short, ctx->nbr_long, ctx->cnt_long, ctx->nbr_short,
-                       ctx->cnt_short, ctx->xprev, ctx->counters);
+                       ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, ctx->cap_short, ctx->nbr_long,
+                       ctx->cnt_long, ctx->nbr_short, ctx->cnt_short, ctx->xprev, ctx->counters);
   } else {
     hipLaunchKernelGGL(k_build_cells<false>, dim3(G), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_pair,
                        ctx->n_stencil_pair, ctx->cutsq_list, 0.0f, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
                        ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
-                       ctx->mt_special, ctx->cap_long, 0, ctx->nbr_long, ctx->cnt_long, nullptr, nullptr, ctx->xprev,
-                       ctx->counters);
+                       ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, 0, ctx->nbr_long,
+                       ctx->cnt_long, nullptr, nullptr, ctx->xprev, ctx->counters);
   }
 
   NNPE_CHECK_LAUNCH(ctx);
@@ -587,7 +609,9 @@ int neigh_check_and_grow(nnpe_ctx* ctx, bool* rebuilt) {
   }
   if (c.overflow_short) {
     ctx->cap_short = (c.overflow_short + c.overflow_short / 4 + 63) / 64 * 64;
-    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_short, size_t(ctx->cap_short) * ctx->stride));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->nbr_short, size_t(ctx->ntype) * ctx->cap_short * ctx->stride));
+    rc = nnp_act_reserve(ctx);
+    if (rc) return rc;
     grow = true;
   }
   if (grow) {
@@ -634,11 +658,15 @@ extern "C" int nnpe_neigh_size(nnpe_ctx* ctx, int which, long long* n_entries, i
   if (!ctx || (which != 0 && which != 1)) return NNPE_ERR_ARG;
   if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "no list");
   if (which == 1 && !ctx->cap_short) return ctx_fail(ctx, NNPE_ERR_STATE, "no short list");
+  const int nl = which ? ctx->ntype : 1;  // the short list is one list per neighbour element
+  long long s = 0;
   std::vector<int> h(ctx->natom);
   NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  NNPE_CHECK_HIP(ctx, hipMemcpy(h.data(), which ? ctx->cnt_short : ctx->cnt_long, sizeof(int) * ctx->natom, hipMemcpyDeviceToHost));
-  long long s = 0;
-  for (int v : h) s += v;
+  for (int el = 0; el < nl; ++el) {
+    const int* dc = which ? ctx->cnt_short + size_t(el) * ctx->stride : ctx->cnt_long;
+    NNPE_CHECK_HIP(ctx, hipMemcpy(h.data(), dc, sizeof(int) * ctx->natom, hipMemcpyDeviceToHost));
+    for (int v : h) s += v;
+  }
   if (n_entries) *n_entries = s;
   int rc = counters_read(ctx);
   if (rc) return rc;
@@ -652,24 +680,32 @@ extern "C" int nnpe_neigh_export(nnpe_ctx* ctx, int which, int* ai, int* aj, int
   if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "no list");
   if (which == 1 && !ctx->cap_short) return ctx_fail(ctx, NNPE_ERR_STATE, "no short list");
   const int N = ctx->natom;
-  std::vector<int> h(N);
+  const int nl = which ? ctx->ntype : 1;  // the short list is one list per neighbour element
+  std::vector<int> h(size_t(N) * nl);
   NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int* dcnt = which ? ctx->cnt_short : ctx->cnt_long;
-  NNPE_CHECK_HIP(ctx, hipMemcpy(h.data(), dcnt, sizeof(int) * N, hipMemcpyDeviceToHost));
-  std::vector<long long> offs(N);
+  for (int el = 0; el < nl; ++el) {
+    const int* dcs = which ? ctx->cnt_short + size_t(el) * ctx->stride : ctx->cnt_long;
+    NNPE_CHECK_HIP(ctx, hipMemcpy(h.data() + size_t(el) * N, dcs, sizeof(int) * N, hipMemcpyDeviceToHost));
+  }
+  std::vector<long long> offs(size_t(N) * nl);
   long long tot = 0;
-  for (int i = 0; i < N; ++i) { offs[i] = tot; tot += h[i]; }
+  for (size_t k = 0; k < offs.size(); ++k) { offs[k] = tot; tot += h[k]; }
   if (n_out) *n_out = tot;
   if (tot > cap) return ctx_fail(ctx, NNPE_ERR_ARG, "export capacity too small");
   long long* doffs = nullptr;
   int *dai = nullptr, *daj = nullptr, *dc = nullptr, *ds = nullptr;
-  NNPE_CHECK_HIP(ctx, dev_alloc(doffs, N));
+  NNPE_CHECK_HIP(ctx, dev_alloc(doffs, offs.size()));
   NNPE_CHECK_HIP(ctx, dev_alloc(dai, tot)); NNPE_CHECK_HIP(ctx, dev_alloc(daj, tot));
   NNPE_CHECK_HIP(ctx, dev_alloc(dc, tot)); NNPE_CHECK_HIP(ctx, dev_alloc(ds, tot));
-  NNPE_CHECK_HIP(ctx, hipMemcpy(doffs, offs.data(), sizeof(long long) * N, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_export, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, which ? ctx->cap_short : ctx->cap_long,
-                     which ? ctx->nbr_short : ctx->nbr_long, dcnt, doffs, ctx->info, dai, daj, dc, ds);
-  NNPE_CHECK_LAUNCH(ctx);
+  NNPE_CHECK_HIP(ctx, hipMemcpy(doffs, offs.data(), sizeof(long long) * offs.size(), hipMemcpyHostToDevice));
+  for (int el = 0; el < nl; ++el) {
+    const int capx = which ? ctx->cap_short : ctx->cap_long;
+    const uint32_t* nb = which ? ctx->nbr_short + size_t(el) * ctx->cap_short * ctx->stride : ctx->nbr_long;
+    const int* dcs = which ? ctx->cnt_short + size_t(el) * ctx->stride : ctx->cnt_long;
+    hipLaunchKernelGGL(k_export, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, capx, nb, dcs,
+                       doffs + size_t(el) * N, ctx->info, dai, daj, dc, ds);
+    NNPE_CHECK_LAUNCH(ctx);
+  }
   NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   NNPE_CHECK_HIP(ctx, hipMemcpy(ai, dai, sizeof(int) * tot, hipMemcpyDeviceToHost));
   NNPE_CHECK_HIP(ctx, hipMemcpy(aj, daj, sizeof(int) * tot, hipMemcpyDeviceToHost));

@@ -30,6 +30,10 @@ constexpr int kND = 20;        // NNPOTENTIAL_TBL::n_descriptor (src/pair_nnp.h:
 constexpr int kNRad = 16;
 constexpr int kNEta = 4;
 constexpr int kNMom = 11;      // A0, A1[3], A2[xx,yy,zz,xy,xz,yz], S
+// adjoint rows: one 256-byte slice per element, [4 x 11 angular][16 radial][4 pad], so the slice
+// a pair gathers from its neighbour is two aligned 128-byte lines
+constexpr int kSlice = 64;
+constexpr int kSliceRad = kNEta * kNMom;
 constexpr float kPi = 3.14159265358979f;
 constexpr int kTblG = 16;      // lanes per atom, tabulated form
 constexpr int kSfG = 16;       // lanes per atom, symmetry functions
@@ -159,16 +163,17 @@ __device__ __forceinline__ void sf_cut(float r, float rc, float& fc, float& dfc)
 }
 
 // moments per atom: [NT][kNEta][kNMom] then radial [NT][kNRad].  A group of G lanes per atom,
-// one pass over the atom's list row per neighbour element a (so only that element's 60
-// accumulators are live): each lane accumulates the moments of its share of the neighbours, the
+// one pass per neighbour element a over the atom's row of that element's short list (so only
+// that element's 60 accumulators are live and no neighbour type is looked up): each lane accumulates the moments of its share of the neighbours, the
 // group butterfly-sums them, and the lanes split the stores (lane l writes the values whose
 // index is l mod G) to the moment rows and to an LDS copy, from which the angular descriptors
 // are formed at the end.
 template <int NT, int G>
-__global__ void __launch_bounds__(256) k_sf_desc(int n, int cap, const uint32_t* __restrict__ nbr,
-                                                 const int* __restrict__ cnt, const float4* __restrict__ xq,
-                                                 const int* __restrict__ typ, float bl, float m_bl, SfParams P,
-                                                 float* __restrict__ Mout, float* __restrict__ G_out) {
+__global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
+                                                 const int* __restrict__ cnt, const float4* __restrict__ xq, float bl,
+                                                 float m_bl, SfParams P, float* __restrict__ Mout,
+                                                 float* __restrict__ G_out, uint32_t* __restrict__ act,
+                                                 int* __restrict__ act_cnt) {
   constexpr int NP = NT * (NT + 1) / 2;
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int RAD0 = NT * kNEta * kNMom;
@@ -181,10 +186,10 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int cap, const uint32_t*
   float* mo = Mout + (size_t)(live ? i : 0) * NMT;
   if (live) {
     const float4 xi = xq[i];
-    const int cn = cnt[i];
-    const uint32_t* row = list_row(nbr, cap, i);
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
+      const int cn = cnt[(size_t)a * stride + i];
+      const uint32_t* row = nbr + ((size_t)a * stride + i) * cap;
       float A[kNEta][kNMom];
       float R[kNRad];
 #pragma unroll
@@ -193,14 +198,28 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int cap, const uint32_t*
         for (int c = 0; c < kNMom; ++c) A[m][c] = 0.0f;
 #pragma unroll
       for (int k = 0; k < kNRad; ++k) R[k] = 0.0f;
-      for (int q = lane; q < cn; q += G) {
-        const uint32_t ent = row[q];
-        const int j = NNPE_ENTRY_J(ent);
-        if (NT > 1 && typ[j] != a) continue;
-        const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
-        const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
-        const float rsq = ux * ux + uy * uy + uz * uz;
-        if (!(rsq < P.rcsq)) continue;
+      // in-range entries are also compacted, in list order, into this step's active list
+      // (a group-masked ballot gives each lane its slot), so the force pass has no idle lanes
+      uint32_t* arow = act + ((size_t)a * stride + i) * cap;
+      const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (threadIdx.x & 63 & ~(G - 1));
+      const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1ull;
+      int na = 0;
+      for (int q0 = 0; q0 < cn; q0 += G) {
+        const int q = q0 + lane;
+        uint32_t ent = 0;
+        float ux = 0, uy = 0, uz = 0, rsq = 0;
+        bool in = false;
+        if (q < cn) {
+          ent = row[q];
+          const float4 xj = image_pos(xq[NNPE_ENTRY_J(ent)], NNPE_ENTRY_CODE(ent), bl, m_bl);
+          ux = xj.x - xi.x; uy = xj.y - xi.y; uz = xj.z - xi.z;
+          rsq = ux * ux + uy * uy + uz * uz;
+          in = rsq < P.rcsq;
+        }
+        const unsigned long long bal = __ballot(in) & gmask;
+        if (in) arow[na + __popcll(bal & below)] = ent;
+        na += __popcll(bal);
+        if (!in) continue;
         const float r = sqrtf(rsq), ir = 1.0f / r;
         const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
         float fc, dfc;
@@ -220,6 +239,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int cap, const uint32_t*
           A[m][10] += w * w;
         }
       }
+      if (lane == 0) act_cnt[(size_t)a * stride + i] = na;
 #pragma unroll
       for (int m = 0; m < kNEta; ++m)
 #pragma unroll
@@ -270,7 +290,7 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restri
   if (i >= n) return;
   const float* mi = M + (size_t)i * NMT;
   const float* g = dG + (size_t)i * D;
-  float* ad = adj + (size_t)i * NMT;
+  float* ad = adj + (size_t)i * NT * kSlice;
   float dA[NT][kNEta][kNMom];
 #pragma unroll
   for (int a = 0; a < NT; ++a)
@@ -301,13 +321,17 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restri
         if (a == b) dA[a][m][10] -= u0 + u1 + u2;
       }
 #pragma unroll
-  for (int a = 0; a < NT; ++a)
+  for (int a = 0; a < NT; ++a) {
+    float* sl = ad + a * kSlice;
 #pragma unroll
     for (int m = 0; m < kNEta; ++m)
 #pragma unroll
-      for (int c = 0; c < kNMom; ++c) ad[(a * kNEta + m) * kNMom + c] = dA[a][m][c];
+      for (int c = 0; c < kNMom; ++c) sl[m * kNMom + c] = dA[a][m][c];
 #pragma unroll
-  for (int k = 0; k < NT * kNRad; ++k) ad[NT * kNEta * kNMom + k] = g[k];
+    for (int k = 0; k < kNRad; ++k) sl[kSliceRad + k] = g[a * kNRad + k];
+#pragma unroll
+    for (int k = kSliceRad + kNRad; k < kSlice; ++k) sl[k] = 0.0f;
+  }
 }
 
 // dE_owner/du for one neighbour at unit vector h (u = r h) of the given element slice of an
@@ -330,16 +354,17 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
   }
 }
 
-// A group of G lanes per atom; the atom's own adjoint row is staged in LDS (one row per group),
-// each lane gathers the neighbour's element slice of its pairs from global memory.
+// A group of G lanes per atom over the step's active lists (the pairs within the cutoff, one
+// list per neighbour element, compacted by the descriptor pass); the atom's own adjoint row is
+// staged in LDS (one row per group), each lane gathers the neighbour's element slice of its
+// pairs from global memory.
 template <int NT, bool EV, int G>
-__global__ void __launch_bounds__(256) k_sf_force(int n, int cap, const uint32_t* __restrict__ nbr,
+__global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                   const int* __restrict__ typ, float bl, float m_bl, SfParams P,
                                                   const float* __restrict__ adj, float4* __restrict__ f, double* pe,
                                                   double* pw) {
-  constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
-  constexpr int RAD0 = NT * kNEta * kNMom;
+  constexpr int NMT = NT * kSlice;
   __shared__ float s_adj[256 / G][NMT];
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
@@ -352,8 +377,10 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int cap, const uint32_t
     const int ti = typ[i];
     const float* ai = s_adj[grp];
     float fx = 0, fy = 0, fz = 0, vir = 0;
-    const int cn = cnt[i];
-    const uint32_t* row = list_row(nbr, cap, i);
+#pragma unroll
+    for (int tj = 0; tj < NT; ++tj) {
+    const int cn = cnt[(size_t)tj * stride + i];
+    const uint32_t* row = nbr + ((size_t)tj * stride + i) * cap;
     for (int q = lane; q < cn; q += G) {
       const uint32_t ent = row[q];
       const int j = NNPE_ENTRY_J(ent);
@@ -365,7 +392,6 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int cap, const uint32_t
       const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
       float fc, dfc;
       sf_cut(r, P.rc, fc, dfc);
-      const int tj = typ[j];
       float wv[kNEta], dwv[kNEta];
 #pragma unroll
       for (int m = 0; m < kNEta; ++m) {
@@ -373,11 +399,12 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int cap, const uint32_t
         wv[m] = ex * fc;
         dwv[m] = ex * (dfc - 2.0f * P.eta[m] * r * fc);
       }
-      const float* aj = adj + (size_t)j * NMT;
+      const float* aj = adj + (size_t)j * NMT + ti * kSlice;  // neighbour's slice for this atom's element
+      const float* as = ai + tj * kSlice;                      // own slice for the neighbour's element
       // radial: sum_n g_n'(r) (dR_i[tj][n] + dR_j[ti][n])
       float rad = 0.0f, radi = 0.0f;
-      const float* ri = ai + RAD0 + tj * kNRad;
-      const float* rj = aj + RAD0 + ti * kNRad;
+      const float* ri = as + kSliceRad;
+      const float* rj = aj + kSliceRad;
 #pragma unroll
       for (int k = 0; k < kNRad; ++k) {
         const float d = r - P.mu[k];
@@ -387,10 +414,10 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int cap, const uint32_t
       }
       // own adjoint, direction +h
       float al_i = 0, gix = 0, giy = 0, giz = 0;
-      sf_grad_ang(ai + tj * kNEta * kNMom, wv, dwv, ir, hx, hy, hz, al_i, gix, giy, giz);
+      sf_grad_ang(as, wv, dwv, ir, hx, hy, hz, al_i, gix, giy, giz);
       // neighbour's adjoint, direction -h
       float al_j = 0, gjx = 0, gjy = 0, gjz = 0;
-      sf_grad_ang(aj + ti * kNEta * kNMom, wv, dwv, ir, -hx, -hy, -hz, al_j, gjx, gjy, gjz);
+      sf_grad_ang(aj, wv, dwv, ir, -hx, -hy, -hz, al_j, gjx, gjy, gjz);
       // F_i += dE_i/du - dE_j/du' with u' = -u; radial pieces are along h for i and -h for j
       const float gxi = gix + (al_i)*hx, gyi = giy + (al_i)*hy, gzi = giz + (al_i)*hz;
       const float gxj = gjx - al_j * hx, gyj = gjy - al_j * hy, gzj = gjz - al_j * hz;
@@ -399,6 +426,7 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int cap, const uint32_t
       fz += gzi - gzj + rad * hz;
       // W_i = -(dE_i/du . u), own descriptor only
       if (EV) vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
+    }
     }
     fx = group_sum<G>(fx); fy = group_sum<G>(fy); fz = group_sum<G>(fz);
     if (lane == 0) {
@@ -426,7 +454,10 @@ struct NnpState {
   bool sf = false;
   SfParams P{};
   float* M = nullptr;    // moments [N][NMT]
-  float* adj = nullptr;  // [N][NMT]
+  float* adj = nullptr;  // [N][ntype][kSlice]
+  uint32_t* act = nullptr;  // active lists: pairs within the cutoff, [ntype][stride][act_cap]
+  int* act_cnt = nullptr;   // [ntype][stride]
+  int act_cap = 0;
   int nmt = 0;
 };
 
@@ -439,7 +470,7 @@ void nnp_state_free(nnpe_ctx* ctx) {
   NnpState* S = ctx->nnp;
   if (!S) return;
   dev_free(S->tdesc); dev_free(S->tfdesc); dev_free(S->tpair); dev_free(S->tfpair);
-  dev_free(S->M); dev_free(S->adj);
+  dev_free(S->M); dev_free(S->adj); dev_free(S->act); dev_free(S->act_cnt);
   delete S;
   ctx->nnp = nullptr;
 }
@@ -585,9 +616,24 @@ extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, f
   for (int k = 0; k < kNEta; ++k) S.P.eta[k] = eta[k];
   S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
   NNPE_CHECK_HIP(ctx, dev_alloc(S.M, size_t(ctx->natom) * S.nmt));
-  NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->natom) * S.nmt));
+  NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->natom) * ctx->ntype * kSlice));
   S.sf = true;
+  S.act_cap = 0;
+  int rc0 = nnp_act_reserve(ctx);
+  if (rc0) return rc0;
   return nnp_desc_reserve(ctx, sf_dim(ctx->ntype));
+}
+
+// active lists sized like the short lists (called whenever their capacity changes; never inside
+// a captured step)
+int nnp_act_reserve(nnpe_ctx* ctx) {
+  NnpState* S = ctx->nnp;
+  if (!S || !S->sf || !ctx->cap_short || S->act_cap == ctx->cap_short) return NNPE_OK;
+  NNPE_CHECK_HIP(ctx, dev_alloc(S->act, size_t(ctx->ntype) * ctx->cap_short * ctx->stride));
+  NNPE_CHECK_HIP(ctx, dev_alloc(S->act_cnt, size_t(ctx->ntype) * ctx->stride));
+  NNPE_CHECK_HIP(ctx, hipMemset(S->act_cnt, 0, sizeof(int) * ctx->ntype * ctx->stride));
+  S->act_cap = ctx->cap_short;
+  return NNPE_OK;
 }
 
 int sf_dim(int ntype) { return ntype * kNRad + ntype * (ntype + 1) / 2 * kNEta * 4; }
@@ -595,12 +641,10 @@ int sf_dim(int ntype) { return ntype * kNRad + ntype * (ntype + 1) / 2 * kNEta *
 int sf_desc_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
   const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
-  const uint32_t* nbr = ctx->cap_short ? ctx->nbr_short : ctx->nbr_long;
-  const int* cnt = ctx->cap_short ? ctx->cnt_short : ctx->cnt_long;
-  const int cap = ctx->cap_short ? ctx->cap_short : ctx->cap_long;
-#define NNPE_SF_DESC(NT)                                                                                              \
-  hipLaunchKernelGGL((k_sf_desc<NT, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, cap, nbr, cnt, ctx->xq, \
-                     ctx->typ, ctx->bl, ctx->m_bl, S.P, S.M, nnp_G(ctx))
+#define NNPE_SF_DESC(NT)                                                                                           \
+  hipLaunchKernelGGL((k_sf_desc<NT, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride,         \
+                     ctx->cap_short, ctx->nbr_short, ctx->cnt_short, ctx->xq, ctx->bl, ctx->m_bl, S.P, S.M, nnp_G(ctx), \
+                     S.act, S.act_cnt)
   switch (ctx->ntype) {
     case 1: NNPE_SF_DESC(1); break;
     case 2: NNPE_SF_DESC(2); break;
@@ -633,14 +677,11 @@ int sf_adjoint_enqueue(nnpe_ctx* ctx) {
 int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
   NnpState& S = nnp_state(ctx);
   const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
-  const uint32_t* nbr = ctx->cap_short ? ctx->nbr_short : ctx->nbr_long;
-  const int* cnt = ctx->cap_short ? ctx->cnt_short : ctx->cnt_long;
-  const int cap = ctx->cap_short ? ctx->cap_short : ctx->cap_long;
   double* pe = ev_part(ctx, NNPE_TERM_SF, 0);
   double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
-#define NNPE_SF_FORCE(NT, EV)                                                                                         \
-  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, cap, nbr, cnt, \
-                     ctx->xq, ctx->typ, ctx->bl, ctx->m_bl, S.P, S.adj, ctx->f, pe, pw)
+#define NNPE_SF_FORCE(NT, EV)                                                                                     \
+  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
+                     S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->bl, ctx->m_bl, S.P, S.adj, ctx->f, pe, pw)
   switch (ctx->ntype) {
     case 1: if (ev) NNPE_SF_FORCE(1, true); else NNPE_SF_FORCE(1, false); break;
     case 2: if (ev) NNPE_SF_FORCE(2, true); else NNPE_SF_FORCE(2, false); break;
@@ -662,14 +703,15 @@ int sf_force_enqueue(nnpe_ctx* ctx, bool ev) {
 extern "C" int nnpe_sf_descriptor(nnpe_ctx* ctx) {
   if (!ctx) return NNPE_ERR_ARG;
   if (!nnp_state(ctx).sf || !ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "needs nnpe_sf_set and a list");
-  const float reach = ctx->cap_short ? ctx->short_cutoff + ctx->skin : ctx->pair_cutoff + ctx->skin;
-  if (nnp_state(ctx).P.rc > reach) return ctx_fail(ctx, NNPE_ERR_ARG, "symmetry-function cutoff beyond the list");
+  if (!ctx->cap_short) return ctx_fail(ctx, NNPE_ERR_STATE, "symmetry functions need the short lists (nnpe_neigh_init short_cutoff)");
+  if (nnp_state(ctx).P.rc > ctx->short_cutoff + ctx->skin) return ctx_fail(ctx, NNPE_ERR_ARG, "symmetry-function cutoff beyond the short list");
   return sf_desc_enqueue(ctx);
 }
 
 extern "C" int nnpe_sf_force(nnpe_ctx* ctx, int ev) {
   if (!ctx) return NNPE_ERR_ARG;
   if (!nnp_state(ctx).sf || !ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "needs nnpe_sf_set and a list");
+  if (!ctx->cap_short) return ctx_fail(ctx, NNPE_ERR_STATE, "symmetry functions need the short lists (nnpe_neigh_init short_cutoff)");
   if (nnp_current_D(ctx) != sf_dim(ctx->ntype)) return ctx_fail(ctx, NNPE_ERR_STATE, "dE/dG is not of the symmetry-function form");
   return sf_force_enqueue(ctx, ev != 0);
 }
