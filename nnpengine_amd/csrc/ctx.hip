@@ -54,6 +54,7 @@ static void free_all(nnpe_ctx* c) {
   dev_free(c->t_xd); dev_free(c->t_vd); dev_free(c->t_ad);
   dev_free(c->t_rm); dev_free(c->t_typ); dev_free(c->t_cell); dev_free(c->order); dev_free(c->t_info);
   dev_free(c->mol_first); dev_free(c->mt_natom); dev_free(c->mt_special_off); dev_free(c->mt_special);
+  dev_free(c->rbox); dev_free(c->t_rbox); dev_free(c->d_box); dev_free(c->cell_box);
   dev_free(c->cell_start); dev_free(c->cell_cnt); dev_free(c->cell_fill);
   dev_free(c->gcell_start); dev_free(c->gcell_cnt); dev_free(c->gcell_fill);
   dev_free(c->g_src); dev_free(c->g_code); dev_free(c->g_cell); dev_free(c->gs_src); dev_free(c->gs_code);
@@ -161,6 +162,9 @@ extern "C" int nnpe_system_set(nnpe_ctx* ctx, int natom, int ntype, float bl, co
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_rm, N)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_typ, N));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_cell, N)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->order, N));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_info, N));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->rbox, N)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->t_rbox, N));
+  NNPE_CHECK_HIP(ctx, hipMemset(ctx->rbox, 0, sizeof(int) * N));
+  ctx->nbox = 1;
   // FP64 integration state starts from the FP32 values
   std::vector<double4> hxd(N), hvd(N), had(N, make_double4(0, 0, 0, 0));
   for (size_t i = 0; i < N; ++i) {

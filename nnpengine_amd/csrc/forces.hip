@@ -28,7 +28,8 @@ __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
 template <bool EV, int G>
 __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __restrict__ nbr,
                                               const int* __restrict__ cnt, const float4* __restrict__ xq,
-                                              const int* __restrict__ typ, int ntype, int npair, float bl, float m_bl,
+                                              const int* __restrict__ typ, const BoxPar* __restrict__ boxes,
+                                              const int* __restrict__ rbox, int ntype, int npair,
                                               float cutsq, float cut, float rdr, float fc, float vc, int nb1,
                                               const float* __restrict__ tab, float4* __restrict__ f, double* pe,
                                               double* pw) {
@@ -41,6 +42,8 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
   if (i < n) {
     const float4 xi = xq[i];
     const int ti = typ[i];
+    const BoxPar& B = boxes[rbox[i]];
+    const float bl = B.bl, m_bl = B.m_bl;
     float fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     const int m = cnt[i];
     const uint32_t* row = list_row(nbr, cap, i);
@@ -76,14 +79,17 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
   if (EV) block_sum2_store<256>(e_acc, w_acc, pe, pw);
 }
 
-__device__ __forceinline__ float4 xmol_pos(int aid, const int* inv, const float4* xq, const int* xcode, float bl, float m_bl) {
+__device__ __forceinline__ float4 xmol_pos(int aid, const int* inv, const float4* xq, const int* xcode,
+                                           const BoxPar* boxes, const int* rbox) {
   const int r = inv[aid];
-  return image_pos(xq[r], xcode[r], bl, m_bl);
+  const BoxPar& B = boxes[rbox[r]];
+  return image_pos(xq[r], xcode[r], B.bl, B.m_bl);
 }
 
 template <bool EV>
 __global__ void __launch_bounds__(256) k_bonded(int nmol, const int* __restrict__ inv, const float4* __restrict__ xq,
-                                                const int* __restrict__ xcode, float bl, float m_bl,
+                                                const int* __restrict__ xcode, const BoxPar* __restrict__ boxes,
+                                                const int* __restrict__ rbox,
                                                 const int* __restrict__ bstart, const int* __restrict__ bterm,
                                                 const float* __restrict__ bcoef, const int* __restrict__ astart,
                                                 const int* __restrict__ aterm, const float* __restrict__ acoef,
@@ -97,7 +103,7 @@ __global__ void __launch_bounds__(256) k_bonded(int nmol, const int* __restrict_
     for (int t = bstart[m]; t < bstart[m + 1]; ++t) {
       const int* p = bterm + 3 * t;
       const float* icf = bcoef + 4 * p[0];
-      const float4 xi = xmol_pos(p[1], inv, xq, xcode, bl, m_bl), xj = xmol_pos(p[2], inv, xq, xcode, bl, m_bl);
+      const float4 xi = xmol_pos(p[1], inv, xq, xcode, boxes, rbox), xj = xmol_pos(p[2], inv, xq, xcode, boxes, rbox);
       float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
       const float dr = rr - icf[0], bdr = icf[2] - dr;
@@ -113,8 +119,8 @@ __global__ void __launch_bounds__(256) k_bonded(int nmol, const int* __restrict_
     for (int t = astart[m]; t < astart[m + 1]; ++t) {
       const int* p = aterm + 4 * t;
       const float* icf = acoef + 4 * p[0];
-      const float4 xi = xmol_pos(p[1], inv, xq, xcode, bl, m_bl), xj = xmol_pos(p[2], inv, xq, xcode, bl, m_bl),
-                   xk = xmol_pos(p[3], inv, xq, xcode, bl, m_bl);
+      const float4 xi = xmol_pos(p[1], inv, xq, xcode, boxes, rbox), xj = xmol_pos(p[2], inv, xq, xcode, boxes, rbox),
+                   xk = xmol_pos(p[3], inv, xq, xcode, boxes, rbox);
       float dx1 = xi.x - xj.x, dy1 = xi.y - xj.y, dz1 = xi.z - xj.z;
       float dx2 = xk.x - xj.x, dy2 = xk.y - xj.y, dz2 = xk.z - xj.z;
       const float r1 = 1.0f / sqrtf(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
@@ -138,8 +144,8 @@ __global__ void __launch_bounds__(256) k_bonded(int nmol, const int* __restrict_
     for (int t = dstart[m]; t < dstart[m + 1]; ++t) {
       const int* p = dterm + 5 * t;
       const float* icf = dcoef + 4 * p[0];
-      const float4 xi = xmol_pos(p[1], inv, xq, xcode, bl, m_bl), xj = xmol_pos(p[2], inv, xq, xcode, bl, m_bl),
-                   xk = xmol_pos(p[3], inv, xq, xcode, bl, m_bl), xl = xmol_pos(p[4], inv, xq, xcode, bl, m_bl);
+      const float4 xi = xmol_pos(p[1], inv, xq, xcode, boxes, rbox), xj = xmol_pos(p[2], inv, xq, xcode, boxes, rbox),
+                   xk = xmol_pos(p[3], inv, xq, xcode, boxes, rbox), xl = xmol_pos(p[4], inv, xq, xcode, boxes, rbox);
       float dx1 = xi.x - xj.x, dy1 = xi.y - xj.y, dz1 = xi.z - xj.z;
       const float dx2 = xk.x - xj.x, dy2 = xk.y - xj.y, dz2 = xk.z - xj.z;
       float dx3 = xl.x - xk.x, dy3 = xl.y - xk.y, dz3 = xl.z - xk.z;
@@ -207,11 +213,11 @@ int pair_enqueue(nnpe_ctx* ctx, bool ev) {
   double* pw = ev_part(ctx, NNPE_TERM_PAIR, 1);
   if (ev)
     hipLaunchKernelGGL((k_pair<true, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
-                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->npair, ctx->bl, ctx->m_bl, ctx->pt_cutsq,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, ctx->ntype, ctx->npair, ctx->pt_cutsq,
                        ctx->pt_cut, ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
   else
     hipLaunchKernelGGL((k_pair<false, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
-                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->npair, ctx->bl, ctx->m_bl, ctx->pt_cutsq,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, ctx->ntype, ctx->npair, ctx->pt_cutsq,
                        ctx->pt_cut, ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_PAIR, nblk);
@@ -284,12 +290,12 @@ int bonded_enqueue(nnpe_ctx* ctx, bool ev) {
   auto P = [&](int term, int w) { return ev_part(ctx, term, w); };
   if (ev)
     hipLaunchKernelGGL(k_bonded<true>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->nmol, ctx->inv, ctx->xq, ctx->xmol_code,
-                       ctx->bl, ctx->m_bl, ctx->bstart, ctx->bterm, ctx->bcoef, ctx->astart, ctx->aterm, ctx->acoef,
+                       ctx->d_box, ctx->rbox, ctx->bstart, ctx->bterm, ctx->bcoef, ctx->astart, ctx->aterm, ctx->acoef,
                        ctx->dstart, ctx->dterm, ctx->dcoef, ctx->f, P(NNPE_TERM_BOND, 0), P(NNPE_TERM_BOND, 1),
                        P(NNPE_TERM_ANGLE, 0), P(NNPE_TERM_ANGLE, 1), P(NNPE_TERM_DIHEDRAL, 0), P(NNPE_TERM_DIHEDRAL, 1));
   else
     hipLaunchKernelGGL(k_bonded<false>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->nmol, ctx->inv, ctx->xq, ctx->xmol_code,
-                       ctx->bl, ctx->m_bl, ctx->bstart, ctx->bterm, ctx->bcoef, ctx->astart, ctx->aterm, ctx->acoef,
+                       ctx->d_box, ctx->rbox, ctx->bstart, ctx->bterm, ctx->bcoef, ctx->astart, ctx->aterm, ctx->acoef,
                        ctx->dstart, ctx->dterm, ctx->dcoef, ctx->f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) {

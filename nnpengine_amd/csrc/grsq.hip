@@ -39,7 +39,8 @@ __global__ void k_gr_zero(unsigned long long* count, int npair, int nbin, int nz
 template <bool kLds, int G>
 __global__ void __launch_bounds__(256) k_gr_inner(int n, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
-                                                  const int* __restrict__ typ, int ntype, int npair, float bl, float m_bl,
+                                                  const int* __restrict__ typ, int ntype, int npair,
+                                                  const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
                                                   float rmaxsq, float rdr, int nbin_inner, int nbin,
                                                   unsigned long long* __restrict__ count) {
   extern __shared__ unsigned int s_hist[];
@@ -54,6 +55,7 @@ __global__ void __launch_bounds__(256) k_gr_inner(int n, int cap, const uint32_t
   for (int i = a0 + (int)threadIdx.x / G; i < a1; i += kGroups) {
     const float4 xi = xq[i];
     const int ti = typ[i];
+    const float bl = boxes[rbox[i]].bl, m_bl = boxes[rbox[i]].m_bl;
     const int m = cnt[i];
     const uint32_t* row = list_row(nbr, cap, i);
     for (int k = lane; k < m; k += G) {
@@ -87,11 +89,12 @@ constexpr int kGrBatch = 1024;
 constexpr int kGrMaxRanges = 2 * 9 * 9 * 9;
 
 template <bool kLds>
-__global__ void __launch_bounds__(256) k_gr_cells(int ncell, int num_gx, const int4* __restrict__ stencil, int nst,
-                                                  const float4* __restrict__ xq, const int* __restrict__ typ,
+__global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __restrict__ boxes,
+                                                  const int* __restrict__ cell_box, const int4* __restrict__ stencil,
+                                                  int nst, const float4* __restrict__ xq, const int* __restrict__ typ,
                                                   const int* __restrict__ cstart, const int* __restrict__ gstart,
                                                   const int* __restrict__ gs_src, const int* __restrict__ gs_code,
-                                                  int ntype, int npair, float bl, float m_bl, float rmaxsq, float rdr,
+                                                  int ntype, int npair, float rmaxsq, float rdr,
                                                   int nbin, unsigned long long* __restrict__ count) {
   extern __shared__ unsigned int s_hist[];
   __shared__ int r_begin[kGrMaxRanges], r_pref[kGrMaxRanges + 1];
@@ -109,14 +112,17 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, int num_gx, const i
   for (int c = c0; c < c1; ++c) {
     const int hb = cstart[c], he = cstart[c + 1];
     if (hb == he) continue;  // uniform across the workgroup
-    const int cz = c % num_gx, cy = (c / num_gx) % num_gx, cx = c / (num_gx * num_gx);
+    const BoxPar B = boxes[cell_box[c]];
+    const float bl = B.bl, m_bl = B.m_bl;
+    const int num_gx = B.num_gx, lc = c - B.cell_base;
+    const int cz = lc % num_gx, cy = (lc / num_gx) % num_gx, cx = lc / (num_gx * num_gx);
     __syncthreads();  // previous cell's ranges and batch are consumed
     for (int s = tid; s < nst; s += blockDim.x) {
       const int4 st = stencil[s];
       const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
       int nr = 0, ng = 0, br = 0, bg = 0;
       if (ax >= 0 && ay >= 0 && az >= 0 && ax < num_gx && ay < num_gx && az < num_gx) {
-        const int nc = (ax * num_gx + ay) * num_gx + az;
+        const int nc = B.cell_base + (ax * num_gx + ay) * num_gx + az;
         br = cstart[nc]; nr = cstart[nc + 1] - br;
         bg = gstart[nc]; ng = gstart[nc + 1] - bg;
       }
@@ -212,6 +218,7 @@ extern "C" int nnpe_gr_init(nnpe_ctx* ctx, int nbi, float r_max, float r_max_inn
   if (ctx) rmdf_state_free(ctx);  // a refinement is tied to the tables and grids it was set up on
   if (!ctx || nbi <= 0) return NNPE_ERR_ARG;
   if (!ctx->have_grid) return ctx_fail(ctx, NNPE_ERR_STATE, "nnpe_gr_init before nnpe_neigh_init");
+  if (ctx->nbox != 1) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "g(r) is defined for a single box");
   // GR ctor (src/grsq.cpp:28-47)
   if (r_max_inner <= 0) {
     r_max = r_max <= 0 ? ctx->h_bl : r_max;
@@ -262,11 +269,11 @@ int gr_enqueue(nnpe_ctx* ctx, bool full) {
     const int nblk = std::max(1, std::min(2048, blocks_for(N, 64)));
     if (lds <= kHistLdsMax)
       hipLaunchKernelGGL((k_gr_inner<true, kGrG>), dim3(nblk), dim3(256), lds, s, N, ctx->cap_long, ctx->nbr_long,
-                         ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq_inner,
+                         ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->d_box, ctx->rbox, ctx->gr_rmaxsq_inner,
                          ctx->gr_rdr, nz, nb, ctx->gr_count);
     else
       hipLaunchKernelGGL((k_gr_inner<false, kGrG>), dim3(nblk), dim3(256), 0, s, N, ctx->cap_long, ctx->nbr_long,
-                         ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq_inner,
+                         ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, npair, ctx->d_box, ctx->rbox, ctx->gr_rmaxsq_inner,
                          ctx->gr_rdr, nz, nb, ctx->gr_count);
   } else {
     hipLaunchKernelGGL(k_gr_zero, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, s, ctx->gr_count, npair, nb, nb);
@@ -275,13 +282,13 @@ int gr_enqueue(nnpe_ctx* ctx, bool full) {
     const int G = ctx->num_grid;
     const int nblk = std::max(1, std::min(1024, G));  // persistent workgroups, contiguous runs of cells
     if (lds <= kHistLdsMax)
-      hipLaunchKernelGGL(k_gr_cells<true>, dim3(nblk), dim3(256), lds, s, G, ctx->num_gx, ctx->stencil_gr,
+      hipLaunchKernelGGL(k_gr_cells<true>, dim3(nblk), dim3(256), lds, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_gr,
                          ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
-                         ctx->gs_code, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
+                         ctx->gs_code, ctx->ntype, npair, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
     else
-      hipLaunchKernelGGL(k_gr_cells<false>, dim3(nblk), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_gr,
+      hipLaunchKernelGGL(k_gr_cells<false>, dim3(nblk), dim3(256), 0, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_gr,
                          ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell_start, ctx->gcell_start, ctx->gs_src,
-                         ctx->gs_code, ctx->ntype, npair, ctx->bl, ctx->m_bl, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
+                         ctx->gs_code, ctx->ntype, npair, ctx->gr_rmaxsq, ctx->gr_rdr, nb, ctx->gr_count);
   }
   NNPE_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_gr_reduce, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, s, npair * nb, ctx->gr_count, ctx->gr_norm,

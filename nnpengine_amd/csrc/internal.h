@@ -39,6 +39,16 @@ struct DevCounters {
   int log_count;       // MD log records written
 };
 
+// Per-box parameters.  A context holds one periodic box, or (serving) many independent boxes
+// side by side: every row carries its box index (rbox), every cell belongs to one box (cell_box),
+// and the kernels take L, the cell grid and the ghost limit of the row's or cell's box, so boxes
+// of different sizes advance together in the same launches without interacting.
+struct BoxPar {
+  float bl, r_bl, h_bl, m_bl;           // L, 1/L, L/2, -L
+  float g_offset, g_rdx, ghost_lim, pad0;
+  int num_gx, cell_base, pad1, pad2;    // cells per axis, first global cell of the box
+};
+
 struct NnpState;  // descriptor tables and moment buffers (nnp.hip)
 struct MlpState;  // per-element network weights (mlp.hip)
 struct RmdfState; // S(q) refinement of the pair tables (rmdf.hip)
@@ -60,6 +70,12 @@ struct nnpe_ctx {
   int* inv = nullptr;       // atom id -> row
   int* cell = nullptr;      // cell of each row
   int* xmol_code = nullptr; // image code of the unwrapped-molecule position of each row
+  int nbox = 1;
+  BoxPar* d_box = nullptr;  // [nbox]
+  std::vector<BoxPar> h_box;
+  int* rbox = nullptr;      // box of each row (permuted with the rows)
+  int* t_rbox = nullptr;
+  int* cell_box = nullptr;  // box of each cell [num_grid]
   // permutation scratch
   float4 *t_xq = nullptr, *t_v = nullptr, *t_fprev = nullptr, *t_xprev = nullptr;
   double4 *t_xd = nullptr, *t_vd = nullptr, *t_ad = nullptr;
@@ -247,6 +263,10 @@ __device__ __forceinline__ const uint32_t* list_row(const uint32_t* nbr, int cap
   return nbr + (size_t)i * cap;
 }
 
+__device__ __forceinline__ int box_cell_coord(float g_offset, float x, float g_rdx) {
+  return int(__fmul_rn(__fadd_rn(g_offset, x), g_rdx));
+}
+
 // energy/virial: block reduction of one double per thread into ev_partial[slot][blockIdx]
 template <int BS>
 __device__ __forceinline__ void block_sum2_store(double a, double b, double* part_a, double* part_b) {
@@ -271,3 +291,5 @@ hipError_t ev_reserve(nnpe_ctx* ctx, int nblocks);
 int ev_finish(nnpe_ctx* ctx, int term, int nblocks);  // add partials of a term into ev_sum
 double* ev_part(nnpe_ctx* ctx, int term, int which);  // partial array for term (0 energy, 1 virial)
 int counters_read(nnpe_ctx* ctx);
+// upload ctx->h_box and a cell -> box map for the single-box layout
+int box_upload_single(nnpe_ctx* ctx);

@@ -35,11 +35,14 @@ __device__ __forceinline__ int cell_coord(float g_offset, float x, float g_rdx) 
   return int(__fmul_rn(__fadd_rn(g_offset, x), g_rdx));
 }
 
-// LOCAL::refold_and_tag (src/environ.cpp:234-262): wrap into the box and bin into the grid.
-__global__ void k_refold(int n, bool fp64, float bl, float r_bl, float g_offset, float g_rdx, int num_gx,
-                         float4* xq, double4* xd, int* cell_out, int* cell_cnt, DevCounters* cnt) {
+// LOCAL::refold_and_tag (src/environ.cpp:234-262): wrap into the row's box and bin into its grid.
+__global__ void k_refold(int n, bool fp64, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox, float4* xq,
+                         double4* xd, int* cell_out, int* cell_cnt, DevCounters* cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const BoxPar B = boxes[rbox[i]];
+  const float bl = B.bl, r_bl = B.r_bl;
+  const int num_gx = B.num_gx;
   float4 p = xq[i];
   if (fp64) {
     double4 d = xd[i];
@@ -53,12 +56,13 @@ __global__ void k_refold(int n, bool fp64, float bl, float r_bl, float g_offset,
     p.z = __fsub_rn(p.z, __fmul_rn(rintf(__fmul_rn(p.z, r_bl)), bl));
   }
   xq[i] = p;
-  int cx = cell_coord(g_offset, p.x, g_rdx), cy = cell_coord(g_offset, p.y, g_rdx), cz = cell_coord(g_offset, p.z, g_rdx);
+  int cx = cell_coord(B.g_offset, p.x, B.g_rdx), cy = cell_coord(B.g_offset, p.y, B.g_rdx),
+      cz = cell_coord(B.g_offset, p.z, B.g_rdx);
   if (cx < 0 || cy < 0 || cz < 0 || cx >= num_gx || cy >= num_gx || cz >= num_gx) {
     atomicAdd(&cnt->cell_range, 1);
     cx = min(max(cx, 0), num_gx - 1); cy = min(max(cy, 0), num_gx - 1); cz = min(max(cz, 0), num_gx - 1);
   }
-  int c = (cx * num_gx + cy) * num_gx + cz;
+  int c = B.cell_base + (cx * num_gx + cy) * num_gx + cz;
   cell_out[i] = c;
   atomicAdd(&cell_cnt[c], 1);
 }
@@ -119,12 +123,13 @@ __global__ void __launch_bounds__(256) k_cell_sort(int ncell, const int* start, 
 // ENVIRON::sort data movement (src/environ.cpp:319-361): rows permuted into cell order
 __global__ void k_gather(int n, bool fp64, const int* order, const int* cell_old, const float4* xq, const float4* v,
                          const float4* fprev, const float* rm, const int* typ, const int4* info, const double4* xd,
-                         const double4* vd, const double4* ad, float4* xq2, float4* v2, float4* fprev2, float* rm2,
-                         int* typ2, int4* info2, double4* xd2, double4* vd2, double4* ad2, int* cell_new, int* inv) {
+                         const double4* vd, const double4* ad, const int* rbox, float4* xq2, float4* v2, float4* fprev2,
+                         float* rm2, int* typ2, int4* info2, double4* xd2, double4* vd2, double4* ad2, int* rbox2,
+                         int* cell_new, int* inv) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int o = order[i];
-  xq2[i] = xq[o]; v2[i] = v[o]; fprev2[i] = fprev[o]; rm2[i] = rm[o]; typ2[i] = typ[o];
+  xq2[i] = xq[o]; v2[i] = v[o]; fprev2[i] = fprev[o]; rm2[i] = rm[o]; typ2[i] = typ[o]; rbox2[i] = rbox[o];
   int4 in = info[o];
   info2[i] = in;
   cell_new[i] = cell_old[o];
@@ -134,20 +139,23 @@ __global__ void k_gather(int n, bool fp64, const int* order, const int* cell_old
 
 __global__ void k_copy_back(int n, bool fp64, const float4* xq2, const float4* v2, const float4* fprev2, const float* rm2,
                             const int* typ2, const int4* info2, const double4* xd2, const double4* vd2, const double4* ad2,
-                            float4* xq, float4* v, float4* fprev, float* rm, int* typ, int4* info, double4* xd, double4* vd,
-                            double4* ad) {
+                            const int* rbox2, float4* xq, float4* v, float4* fprev, float* rm, int* typ, int4* info,
+                            double4* xd, double4* vd, double4* ad, int* rbox) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   xq[i] = xq2[i]; v[i] = v2[i]; fprev[i] = fprev2[i]; rm[i] = rm2[i]; typ[i] = typ2[i]; info[i] = info2[i];
+  rbox[i] = rbox2[i];
   if (fp64) { xd[i] = xd2[i]; vd[i] = vd2[i]; ad[i] = ad2[i]; }
 }
 
 // LOCAL::generate_ghost + update_ghost_pos<true> (src/environ.cpp:401-442, :488-514)
-__global__ void k_ghost_gen(int n, float ghost_lim, float bl, float m_bl, float g_offset, float g_rdx, int num_gx,
-                            int cap, const float4* xq, int* g_src, int* g_code, int* g_cell, int* gcell_cnt,
-                            DevCounters* cnt) {
+__global__ void k_ghost_gen(int n, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox, int cap,
+                            const float4* xq, int* g_src, int* g_code, int* g_cell, int* gcell_cnt, DevCounters* cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const BoxPar B = boxes[rbox[i]];
+  const float ghost_lim = B.ghost_lim, bl = B.bl, m_bl = B.m_bl, g_offset = B.g_offset, g_rdx = B.g_rdx;
+  const int num_gx = B.num_gx;
   float4 p = xq[i];
   const float mg = -ghost_lim;
   int xs = p.x > ghost_lim ? -1 : 0, xe = p.x < mg ? 1 : 0;
@@ -167,7 +175,7 @@ __global__ void k_ghost_gen(int n, float ghost_lim, float bl, float m_bl, float 
           atomicAdd(&cnt->cell_range, 1);
           cx = min(max(cx, 0), num_gx - 1); cy = min(max(cy, 0), num_gx - 1); cz = min(max(cz, 0), num_gx - 1);
         }
-        int gc = (cx * num_gx + cy) * num_gx + cz;
+        int gc = B.cell_base + (cx * num_gx + cy) * num_gx + cz;
         g_src[slot] = i; g_code[slot] = code; g_cell[slot] = gc;
         atomicAdd(&gcell_cnt[gc], 1);
       }
@@ -215,10 +223,12 @@ __global__ void __launch_bounds__(256) k_ghost_cell_sort(int ncell, const int* g
 // LOCAL::update_ghost_and_info x_mol part (src/environ.cpp:461-469): the image of an atom whose
 // every coordinate lies within L/2 of its molecule's intra-0 atom, among the images the atom
 // has ghosts for; the atom itself otherwise.
-__global__ void k_xmol(int n, float ghost_lim, float bl, float m_bl, float h_bl, const float4* xq, const int4* info,
-                       const int* mol_first, const int* inv, int* xmol_code) {
+__global__ void k_xmol(int n, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox, const float4* xq,
+                       const int4* info, const int* mol_first, const int* inv, int* xmol_code) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const BoxPar B = boxes[rbox[i]];
+  const float ghost_lim = B.ghost_lim, bl = B.bl, m_bl = B.m_bl, h_bl = B.h_bl;
   int4 in = info[i];
   int code_sel = 13;
   if (in.z != 0) {
@@ -256,8 +266,9 @@ constexpr int kBuildMaxRanges = 2 * 343;
 constexpr int kShortMaxType = 4;   // short lists: one per neighbour element
 
 template <bool kShort>
-__global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, const int4* __restrict__ stencil, int nst,
-                                                     float cutsq, float cutsq_s, float bl, float m_bl,
+__global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __restrict__ boxes,
+                                                     const int* __restrict__ cell_box, const int4* __restrict__ stencil,
+                                                     int nst, float cutsq, float cutsq_s,
                                                      const float4* __restrict__ xq, const int4* __restrict__ info,
                                                      const int* __restrict__ cstart, const int* __restrict__ gstart,
                                                      const int* __restrict__ gs_src, const int* __restrict__ gs_code,
@@ -280,14 +291,17 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, int num_gx, cons
   const int hb = cstart[c], he = cstart[c + 1];
   if (hb == he) return;  // uniform across the workgroup
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int cz = c % num_gx, cy = (c / num_gx) % num_gx, cx = c / (num_gx * num_gx);
+  const BoxPar B = boxes[cell_box[c]];
+  const float bl = B.bl, m_bl = B.m_bl;
+  const int num_gx = B.num_gx, lc = c - B.cell_base;
+  const int cz = lc % num_gx, cy = (lc / num_gx) % num_gx, cx = lc / (num_gx * num_gx);
   // candidate ranges of the stencil, then their exclusive prefix sum
   for (int s = tid; s < nst; s += blockDim.x) {
     const int4 st = stencil[s];
     const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
     int nr = 0, ng = 0, br = 0, bg = 0;
     if (ax >= 0 && ay >= 0 && az >= 0 && ax < num_gx && ay < num_gx && az < num_gx) {
-      const int nc = (ax * num_gx + ay) * num_gx + az;
+      const int nc = B.cell_base + (ax * num_gx + ay) * num_gx + az;
       br = cstart[nc]; nr = cstart[nc + 1] - br;
       bg = gstart[nc]; ng = gstart[nc + 1] - bg;
     }
@@ -470,8 +484,10 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
   // src/run.cpp:76-79
   const float comm = skin + std::max(gr_cutoff, pair_cutoff);
   ctx->comm_cutoff = comm;
-  if (!(ctx->h_bl > comm))
-    return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "box too small: L/2 must exceed skin + max(rc, r_gr) (single-image ghosts)");
+  // one shell of images: atoms within comm of both faces (L/2 < comm < L) get images on both
+  // sides, which the reference's single-sided ghosts cannot (it needs L/2 > comm)
+  if (!(ctx->bl > comm))
+    return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "box too small: L must exceed skin + max(rc, r_gr) (one image shell)");
   ctx->ghost_lim = ctx->h_bl - comm;
   const float gl = float(0.5 * (skin + pair_cutoff));
   ctx->g_rdx = 1.0f / (ctx->g_dx = gl);
@@ -499,11 +515,11 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gcell_fill, G));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->cell_start, 0, sizeof(int) * (G + 1)));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->gcell_start, 0, sizeof(int) * (G + 1)));
-  // ghost capacity: the shell volume fraction with a 30% margin (+4096), at most 7 images/atom
+  // ghost capacity: the shell volume fraction with a 30% margin (+4096), at most 26 images/atom
   double vin = double(ctx->bl), vout = double(ctx->bl) + 2.0 * comm;
   double frac = (vout * vout * vout - vin * vin * vin) / (vin * vin * vin);
   long long gcap = (long long)(1.3 * frac * ctx->natom) + 4096;
-  gcap = std::min<long long>(gcap, 7LL * ctx->natom + 64);
+  gcap = std::min<long long>(gcap, 26LL * ctx->natom + 64);
   ctx->ghost_cap = int(gcap);
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_src, gcap)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_code, gcap));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->g_cell, gcap)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gs_src, gcap));
@@ -527,9 +543,26 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
     NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cnt_short, size_t(ctx->ntype) * ctx->stride));
     NNPE_CHECK_HIP(ctx, hipMemset(ctx->cnt_short, 0, sizeof(int) * ctx->ntype * ctx->stride));
   }
+  ctx->nbox = 1;
+  ctx->h_box.assign(1, BoxPar{});
+  BoxPar& B = ctx->h_box[0];
+  B.bl = ctx->bl; B.r_bl = ctx->r_bl; B.h_bl = ctx->h_bl; B.m_bl = ctx->m_bl;
+  B.g_offset = ctx->g_offset; B.g_rdx = ctx->g_rdx; B.ghost_lim = ctx->ghost_lim;
+  B.num_gx = ctx->num_gx; B.cell_base = 0;
+  int rcb = box_upload_single(ctx);
+  if (rcb) return rcb;
   ctx->have_grid = true;
   ctx->have_list = false;
   return nnp_act_reserve(ctx);
+}
+
+// one box: every cell belongs to box 0
+int box_upload_single(nnpe_ctx* ctx) {
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->d_box, ctx->h_box.size()));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->d_box, ctx->h_box.data(), sizeof(BoxPar) * ctx->h_box.size(), hipMemcpyHostToDevice));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_box, ctx->num_grid));
+  NNPE_CHECK_HIP(ctx, hipMemset(ctx->cell_box, 0, sizeof(int) * ctx->num_grid));
+  return NNPE_OK;
 }
 
 // Enqueue the rebuild kernels (no host synchronisation; graph-capturable).
@@ -541,42 +574,42 @@ int neigh_enqueue_build(nnpe_ctx* ctx) {
   hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->cell_fill, G);
   hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->gcell_cnt, G);
   hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->gcell_fill, G);
-  hipLaunchKernelGGL(k_refold, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->bl, ctx->r_bl, ctx->g_offset,
-                     ctx->g_rdx, ctx->num_gx, ctx->xq, ctx->xd, ctx->t_cell, ctx->cell_cnt, ctx->counters);
+  hipLaunchKernelGGL(k_refold, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->d_box, ctx->rbox, ctx->xq, ctx->xd,
+                     ctx->t_cell, ctx->cell_cnt, ctx->counters);
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->cell_cnt, ctx->cell_start, G);
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->t_cell, ctx->cell_start, ctx->cell_fill, ctx->order);
   hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for((long long)G * 64, 256)), dim3(256), 0, s, G, ctx->cell_start, ctx->order,
                      ctx->info);
   hipLaunchKernelGGL(k_gather, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->order, ctx->t_cell, ctx->xq, ctx->v,
-                     ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->t_xq, ctx->t_v, ctx->t_fprev,
-                     ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->cell, ctx->inv);
+                     ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->rbox, ctx->t_xq, ctx->t_v,
+                     ctx->t_fprev, ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->t_rbox, ctx->cell,
+                     ctx->inv);
   // copy the permuted rows back so every array keeps its buffer (captured graphs replay with
   // fixed pointers)
   hipLaunchKernelGGL(k_copy_back, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->t_xq, ctx->t_v, ctx->t_fprev,
-                     ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->xq, ctx->v, ctx->fprev, ctx->rm,
-                     ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad);
+                     ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->t_rbox, ctx->xq, ctx->v,
+                     ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->rbox);
   NNPE_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_ghost_gen, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->ghost_lim, ctx->bl, ctx->m_bl, ctx->g_offset,
-                     ctx->g_rdx, ctx->num_gx, ctx->ghost_cap, ctx->xq, ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_cnt,
-                     ctx->counters);
+  hipLaunchKernelGGL(k_ghost_gen, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->d_box, ctx->rbox, ctx->ghost_cap, ctx->xq,
+                     ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_cnt, ctx->counters);
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->gcell_cnt, ctx->gcell_start, G);
   hipLaunchKernelGGL(k_ghost_scatter, dim3(blocks_for(ctx->ghost_cap, bs)), dim3(bs), 0, s, ctx->ghost_cap, ctx->counters,
                      ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_start, ctx->gcell_fill, ctx->gs_src, ctx->gs_code);
   hipLaunchKernelGGL(k_ghost_cell_sort, dim3(blocks_for((long long)G * 64, 256)), dim3(256), 0, s, G, ctx->gcell_start,
                      ctx->gs_src, ctx->gs_code, ctx->info);
-  hipLaunchKernelGGL(k_xmol, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->ghost_lim, ctx->bl, ctx->m_bl, ctx->h_bl, ctx->xq,
-                     ctx->info, ctx->mol_first, ctx->inv, ctx->xmol_code);
+  hipLaunchKernelGGL(k_xmol, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->d_box, ctx->rbox, ctx->xq, ctx->info,
+                     ctx->mol_first, ctx->inv, ctx->xmol_code);
   if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
   if (ctx->cap_short && ctx->ntype > kShortMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "short lists: at most 4 elements");
   if (ctx->cap_short) {
-    hipLaunchKernelGGL(k_build_cells<true>, dim3(G), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_pair,
-                       ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
+    hipLaunchKernelGGL(k_build_cells<true>, dim3(G), dim3(256), 0, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair,
+                       ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->xq, ctx->info,
                        ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
                        ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, ctx->cap_short, ctx->nbr_long,
                        ctx->cnt_long, ctx->nbr_short, ctx->cnt_short, ctx->xprev, ctx->counters);
   } else {
-    hipLaunchKernelGGL(k_build_cells<false>, dim3(G), dim3(256), 0, s, G, ctx->num_gx, ctx->stencil_pair,
-                       ctx->n_stencil_pair, ctx->cutsq_list, 0.0f, ctx->bl, ctx->m_bl, ctx->xq, ctx->info,
+    hipLaunchKernelGGL(k_build_cells<false>, dim3(G), dim3(256), 0, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair,
+                       ctx->n_stencil_pair, ctx->cutsq_list, 0.0f, ctx->xq, ctx->info,
                        ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
                        ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, 0, ctx->nbr_long,
                        ctx->cnt_long, nullptr, nullptr, ctx->xprev, ctx->counters);

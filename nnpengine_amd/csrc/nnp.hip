@@ -51,7 +51,8 @@ __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
 template <int G>
 __global__ void __launch_bounds__(256) k_tbl_desc(int n, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
-                                                  const int* __restrict__ typ, int ntype, float bl, float m_bl,
+                                                  const int* __restrict__ typ, int ntype, const BoxPar* __restrict__ boxes,
+                                                  const int* __restrict__ rbox,
                                                   float cutsq, float rdr, int nb1, const float* __restrict__ tdesc,
                                                   float* __restrict__ G_out) {
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
@@ -59,6 +60,7 @@ __global__ void __launch_bounds__(256) k_tbl_desc(int n, int cap, const uint32_t
   if (i >= n) return;  // whole group
   const float4 xi = xq[i];
   const int ti = typ[i];
+  const float bl = boxes[rbox[i]].bl, m_bl = boxes[rbox[i]].m_bl;
   float acc[kND];
 #pragma unroll
   for (int k = 0; k < kND; ++k) acc[k] = 0.0f;
@@ -92,7 +94,8 @@ __global__ void __launch_bounds__(256) k_tbl_desc(int n, int cap, const uint32_t
 template <bool EV, int G>
 __global__ void __launch_bounds__(256) k_tbl_force(int n, int cap, const uint32_t* __restrict__ nbr,
                                                    const int* __restrict__ cnt, const float4* __restrict__ xq,
-                                                   const int* __restrict__ typ, int ntype, float bl, float m_bl,
+                                                   const int* __restrict__ typ, int ntype, const BoxPar* __restrict__ boxes,
+                                                   const int* __restrict__ rbox,
                                                    float cutsq, float rdr, int nb1, const float* __restrict__ tfdesc,
                                                    const float* __restrict__ tpair, const float* __restrict__ tfpair,
                                                    const float* __restrict__ dG, float4* __restrict__ f, double* pe,
@@ -103,6 +106,7 @@ __global__ void __launch_bounds__(256) k_tbl_force(int n, int cap, const uint32_
   if (i < n) {
     const float4 xi = xq[i];
     const int ti = typ[i];
+    const float bl = boxes[rbox[i]].bl, m_bl = boxes[rbox[i]].m_bl;
     float gi[kND];
 #pragma unroll
     for (int k = 0; k < kND; ++k) gi[k] = dG[(size_t)i * kND + k];
@@ -170,8 +174,9 @@ __device__ __forceinline__ void sf_cut(float r, float rc, float& fc, float& dfc)
 // are formed at the end.
 template <int NT, int G>
 __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
-                                                 const int* __restrict__ cnt, const float4* __restrict__ xq, float bl,
-                                                 float m_bl, SfParams P, float* __restrict__ Mout,
+                                                 const int* __restrict__ cnt, const float4* __restrict__ xq,
+                                                 const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
+                                                 SfParams P, float* __restrict__ Mout,
                                                  float* __restrict__ G_out, uint32_t* __restrict__ act,
                                                  int* __restrict__ act_cnt) {
   constexpr int NP = NT * (NT + 1) / 2;
@@ -186,6 +191,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   float* mo = Mout + (size_t)(live ? i : 0) * NMT;
   if (live) {
     const float4 xi = xq[i];
+    const float bl = boxes[rbox[i]].bl, m_bl = boxes[rbox[i]].m_bl;
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
       const int cn = cnt[(size_t)a * stride + i];
@@ -361,7 +367,8 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
 template <int NT, bool EV, int G>
 __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
-                                                  const int* __restrict__ typ, float bl, float m_bl, SfParams P,
+                                                  const int* __restrict__ typ, const BoxPar* __restrict__ boxes,
+                                                  const int* __restrict__ rbox, SfParams P,
                                                   const float* __restrict__ adj, float4* __restrict__ f, double* pe,
                                                   double* pw) {
   constexpr int NMT = NT * kSlice;
@@ -375,6 +382,7 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
   if (i < n) {
     const float4 xi = xq[i];
     const int ti = typ[i];
+    const float bl = boxes[rbox[i]].bl, m_bl = boxes[rbox[i]].m_bl;
     const float* ai = s_adj[grp];
     float fx = 0, fy = 0, fz = 0, vir = 0;
 #pragma unroll
@@ -565,7 +573,7 @@ extern "C" int nnpe_nnp_tbl_set(nnpe_ctx* ctx, float cutoff, int nbin, const flo
 int nnp_tbl_desc_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
   hipLaunchKernelGGL(k_tbl_desc<kTblG>, dim3(blocks_for((long long)ctx->natom * kTblG, 256)), dim3(256), 0, ctx->stream,
-                     ctx->natom, ctx->cap_long, ctx->nbr_long, ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq,
+                     ctx->natom, ctx->cap_long, ctx->nbr_long, ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->d_box, ctx->rbox, S.tbl_cutsq,
                      S.tbl_rdr, S.tbl_nbin + 1, S.tdesc, nnp_G(ctx));
   NNPE_CHECK_LAUNCH(ctx);
   nnp_set_current_D(ctx, kND);
@@ -579,11 +587,11 @@ int nnp_tbl_force_enqueue(nnpe_ctx* ctx, bool ev) {
   double* pw = ev_part(ctx, NNPE_TERM_NNP_PAIR, 1);
   if (ev)
     hipLaunchKernelGGL((k_tbl_force<true, kTblG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->cap_long, ctx->nbr_long,
-                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq, S.tbl_rdr,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->d_box, ctx->rbox, S.tbl_cutsq, S.tbl_rdr,
                        S.tbl_nbin + 1, S.tfdesc, S.tpair, S.tfpair, nnp_dG(ctx), ctx->f, pe, pw);
   else
     hipLaunchKernelGGL((k_tbl_force<false, kTblG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->cap_long, ctx->nbr_long,
-                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->bl, ctx->m_bl, S.tbl_cutsq, S.tbl_rdr,
+                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->ntype, ctx->d_box, ctx->rbox, S.tbl_cutsq, S.tbl_rdr,
                        S.tbl_nbin + 1, S.tfdesc, S.tpair, S.tfpair, nnp_dG(ctx), ctx->f, pe, pw);
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_NNP_PAIR, nblk);
@@ -643,7 +651,7 @@ int sf_desc_enqueue(nnpe_ctx* ctx) {
   const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
 #define NNPE_SF_DESC(NT)                                                                                           \
   hipLaunchKernelGGL((k_sf_desc<NT, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride,         \
-                     ctx->cap_short, ctx->nbr_short, ctx->cnt_short, ctx->xq, ctx->bl, ctx->m_bl, S.P, S.M, nnp_G(ctx), \
+                     ctx->cap_short, ctx->nbr_short, ctx->cnt_short, ctx->xq, ctx->d_box, ctx->rbox, S.P, S.M, nnp_G(ctx), \
                      S.act, S.act_cnt)
   switch (ctx->ntype) {
     case 1: NNPE_SF_DESC(1); break;
@@ -681,7 +689,7 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
   double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
 #define NNPE_SF_FORCE(NT, EV)                                                                                     \
   hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
-                     S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->bl, ctx->m_bl, S.P, S.adj, ctx->f, pe, pw)
+                     S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, S.P, S.adj, ctx->f, pe, pw)
   switch (ctx->ntype) {
     case 1: if (ev) NNPE_SF_FORCE(1, true); else NNPE_SF_FORCE(1, false); break;
     case 2: if (ev) NNPE_SF_FORCE(2, true); else NNPE_SF_FORCE(2, false); break;
