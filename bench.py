@@ -110,6 +110,73 @@ def cpu_baseline(sysm, par, norm):
                       f"charged at 1/10 per step; oracle float32 restatement, 1 thread"}
 
 
+def _serve_box(args):
+    from nnpengine_amd import system as S
+    nmol, seed = args
+    return S.water_box(nmol, 62.1 * (nmol / 8000.0) ** (1.0 / 3.0), seed=seed)
+
+
+def serving(par, norm, nreq, steps, seed=2027):
+    """Serving form: nreq independent water boxes (molecule counts log-uniform in [64, 4096],
+    seeded, density of the 24k box), each `steps` NVE steps of the same BF16 NNP + LJ/Coulomb +
+    bonded step, arriving as a Poisson stream at 80 % of the measured serving capacity; every
+    request's latency runs from its arrival to the return of its final state."""
+    from concurrent.futures import ProcessPoolExecutor
+    from nnpengine_amd import system as S
+    from nnpengine_amd.engine import Serve
+    rng = np.random.default_rng(seed)
+    nmols = np.exp(rng.uniform(np.log(64), np.log(4096), nreq)).astype(int)
+    with ProcessPoolExecutor(max_workers=8) as ex:
+        boxes = list(ex.map(_serve_box, [(int(n), seed + k) for k, n in enumerate(nmols)]))
+    ref = boxes[0]
+    srv = Serve(ref, WIDTHS, par, norm[0], norm[1], precision="bf16", max_atoms=400000, max_boxes=256,
+                pair_cutoff=RC_PAIR, skin=SKIN, nnp_cutoff=RC_NNP, rebuild_every=10, dt=DT, fp64=True)
+    natom = np.array([b.natom for b in boxes])
+    work = natom * steps
+    # capacity: a closed batch of the first 32 requests, all submitted at once
+    t0 = time.perf_counter()
+    ids = [srv.submit(b, steps) for b in boxes[:32]]
+    left = set(ids)
+    while left:
+        for rid in srv.advance():
+            srv.result(rid)
+            left.discard(rid)
+    cap = work[:32].sum() / (time.perf_counter() - t0)
+    # open loop: Poisson arrivals at 80 % of capacity
+    rate = 0.8 * cap / work.mean()
+    arrive = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    sub_of = {}
+    lat = np.zeros(nreq)
+    done = 0
+    k = 0
+    peak = 0
+    t0 = time.perf_counter()
+    while done < nreq:
+        now = time.perf_counter() - t0
+        while k < nreq and arrive[k] <= now:
+            sub_of[srv.submit(boxes[k], steps)] = k
+            k += 1
+        st = srv.stats()
+        peak = max(peak, st["resident"])
+        if st["resident"] == 0 and st["queued"] == 0:
+            time.sleep(max(0.0, min(arrive[k] - now, 0.01)) if k < nreq else 0.0)
+            continue
+        for rid in srv.advance():
+            srv.result(rid)
+            j = sub_of.pop(rid)
+            lat[j] = time.perf_counter() - t0 - arrive[j]
+            done += 1
+    span = time.perf_counter() - t0 - arrive[0]
+    st = srv.stats()
+    srv.close()
+    return {"requests": nreq, "steps_per_request": steps, "atoms_min": int(natom.min()), "atoms_max": int(natom.max()),
+            "atoms_mean": float(natom.mean()), "capacity_atom_steps_per_s": float(cap), "arrival_rate_per_s": float(rate),
+            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(lat.max())},
+            "atom_steps_per_s": float(work.sum() / span), "peak_resident_boxes": int(peak),
+            "relayouts": int(st["relayouts"]), "chunks": int(st["chunks"]), "arena_bytes": int(st["arena_bytes"])}
+
+
 def force_errors(sysm, par, norm):
     """NNP and total force relative RMS vs the FP64 oracle on the starting snapshot."""
     from nnpengine_amd import model as M
@@ -169,6 +236,8 @@ def main():
     ap.add_argument("--no-err", action="store_true")
     ap.add_argument("--lat-samples", type=int, default=200, help="single-step replays for the latency percentiles")
     ap.add_argument("--stage-reps", type=int, default=20, help="launches per stage in the stage profile")
+    ap.add_argument("--serve-requests", type=int, default=256, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is "
@@ -254,6 +323,12 @@ def main():
                 "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
                 "flop_per_atom": flop_atom}
 
+    for e_ in engines.values():
+        e_.close()
+    serve = None
+    if args.serve_requests > 0:
+        serve = serving(par, norm, args.serve_requests, args.serve_steps)
+        log("[bench] serving:", {k: v for k, v in serve.items() if k != "latency_s"}, serve["latency_s"])
     errs = None if args.no_err else force_errors(sysm, par, norm)
     cpu = None if args.no_cpu else cpu_baseline(sysm, par, norm)
 
@@ -281,6 +356,7 @@ def main():
         "bf16_speedup_vs_fp32": t_f / t_b,
         "force_error_vs_fp64": errs,
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "serving": serve,
         "stage_ms": stages,
         "mlp": mlp_roof,
         "roofline": roof,

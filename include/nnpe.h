@@ -202,6 +202,64 @@ int nnpe_md_stats(nnpe_ctx* ctx, long long* out);
 /* Restart the step counter (the next step evaluates forces on the positions as they are). */
 int nnpe_md_reset_step(nnpe_ctx* ctx);
 
+/* ---- Serving: a request loop over independent boxes -----------------------------------------
+ * No reference counterpart (the reference runs one system per process).  Queued boxes are
+ * admitted first-in first-out while the arena's atom capacity (its preallocated HBM footprint)
+ * allows; all resident boxes advance together, in chunks of rebuild_every steps, in the same
+ * fused launches (one arena of rows, cells and ghosts with per-box L and grids), the steps of a
+ * chunk replayed from captured hipGraphs.  Tabulated LJ/Coulomb, bonded terms and the
+ * symmetry-function NNP, as nnpe_md_run with pair = bonded = 1, nnp = 2. */
+typedef struct nnpe_serve nnpe_serve;
+typedef struct nnpe_serve_config {
+  int device;
+  int max_atoms;          /* arena capacity in atoms (sets the HBM footprint) */
+  int max_boxes;          /* boxes resident at once */
+  int ntype;
+  float pair_cutoff, skin, nnp_cutoff;
+  int rebuild_every;      /* chunk length: lists are rebuilt at every chunk start */
+  double dt;
+  int fp64;
+  int pair_nbin;          /* tabulated pair, [npair][nbin+1] each (nnpe_pair_table) */
+  const float* pair_force;
+  const float* pair_potential;
+  int nbond_types, nangle_types, ndihedral_types;       /* nnpe_bonded_set coefficients */
+  const float* bond_coef;
+  const float* angle_coef;
+  const float* dihedral_coef;
+  int nmoltype;           /* molecule types, as nnpe_system_set */
+  const int* mt_natom;
+  const int* mt_special_off;
+  const int* mt_special;
+  const float* sf_mu;     /* symmetry functions, as nnpe_sf_set (16 radial, 4 angular) */
+  float sf_eta_r;
+  const float* sf_eta;
+  int nlayer;             /* network, as nnpe_mlp_set */
+  const int* widths;
+  const float* params;
+  float out_scale;
+  int precision;
+  const float* in_shift;
+  const float* in_scale;
+} nnpe_serve_config;
+int nnpe_serve_create(const nnpe_serve_config* cfg, nnpe_serve** out);
+int nnpe_serve_destroy(nnpe_serve* s);
+/* Queue one box (topology as nnpe_system_set with ids local to the box; bonded terms as
+ * nnpe_bonded_set, local atom ids, grouped by molecule); nsteps a multiple of rebuild_every. */
+int nnpe_serve_submit(nnpe_serve* s, int natom, float bl, const float* x, const float* v, const float* q,
+                      const float* mass, const int* typ, const int* mol_id, const int* intra_id, int nmol,
+                      const int* mol_type, int nbond, const int* bonds, int nangle, const int* angles, int ndih,
+                      const int* dihedrals, int nsteps, long long* id);
+/* Admit what fits, advance every resident box by one chunk, retire the finished boxes: their
+ * ids go to done_ids (up to cap), their count to ndone. */
+int nnpe_serve_advance(nnpe_serve* s, long long* done_ids, int cap, int* ndone);
+/* Final positions and velocities [natom][3] of a finished request (local atom order); the
+ * request is released. */
+int nnpe_serve_result(nnpe_serve* s, long long id, double* x, double* v);
+/* out[8]: resident boxes, resident atoms, queued, finished unread, chunks, relayouts, arena bytes,
+ * atom capacity */
+int nnpe_serve_stats(nnpe_serve* s, long long* out);
+const char* nnpe_serve_last_error(nnpe_serve* s);
+
 /* Per-stage timing (no reference counterpart: the reference has no timers).  Average ms of one
  * launch of `stage` over `reps` back-to-back launches, HIP events on the context stream. */
 enum nnpe_stage { NNPE_STAGE_PAIR = 0, NNPE_STAGE_BONDED = 1, NNPE_STAGE_SF_DESC = 2, NNPE_STAGE_MLP = 3,

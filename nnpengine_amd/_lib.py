@@ -69,6 +69,14 @@ SIGNATURES = [
     ("nnpe_md_stats", _I, [_P, _PLL]),
     ("nnpe_md_reset_step", _I, [_P]),
     ("nnpe_profile_stage", _I, [_P, _I, _I, _PD]),
+    ("nnpe_serve_create", _I, [_P, C.POINTER(_P)]),
+    ("nnpe_serve_destroy", _I, [_P]),
+    ("nnpe_serve_submit", _I, [_P, _I, _F, _PF, _PF, _PF, _PF, _PI, _PI, _PI, _I, _PI, _I, _PI, _I, _PI, _I, _PI, _I,
+                               _PLL]),
+    ("nnpe_serve_advance", _I, [_P, _PLL, _I, _PI]),
+    ("nnpe_serve_result", _I, [_P, C.c_longlong, _PD, _PD]),
+    ("nnpe_serve_stats", _I, [_P, _PLL]),
+    ("nnpe_serve_last_error", C.c_char_p, [_P]),
 ]
 
 STAGES = {"pair": 0, "bonded": 1, "sf_desc": 2, "mlp": 3, "sf_adjoint": 4, "sf_force": 5, "rebuild": 6,
@@ -81,6 +89,22 @@ class MdParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("fp64", C.c_int), ("rebuild_every", C.c_int), ("pair", C.c_int),
                 ("bonded", C.c_int), ("nnp", C.c_int), ("gr_every", C.c_int), ("gr_long_every", C.c_int),
                 ("sq", C.c_int), ("log_every", C.c_int), ("use_graph", C.c_int), ("rmdf", C.c_int)]
+
+
+
+class ServeConfig(C.Structure):
+    """struct nnpe_serve_config (include/nnpe.h)."""
+    _fields_ = [("device", C.c_int), ("max_atoms", C.c_int), ("max_boxes", C.c_int), ("ntype", C.c_int),
+                ("pair_cutoff", C.c_float), ("skin", C.c_float), ("nnp_cutoff", C.c_float),
+                ("rebuild_every", C.c_int), ("dt", C.c_double), ("fp64", C.c_int),
+                ("pair_nbin", C.c_int), ("pair_force", _PF), ("pair_potential", _PF),
+                ("nbond_types", C.c_int), ("nangle_types", C.c_int), ("ndihedral_types", C.c_int),
+                ("bond_coef", _PF), ("angle_coef", _PF), ("dihedral_coef", _PF),
+                ("nmoltype", C.c_int), ("mt_natom", _PI), ("mt_special_off", _PI), ("mt_special", _PI),
+                ("sf_mu", _PF), ("sf_eta_r", C.c_float), ("sf_eta", _PF),
+                ("nlayer", C.c_int), ("widths", _PI), ("params", _PF), ("out_scale", C.c_float),
+                ("precision", C.c_int), ("in_shift", _PF), ("in_scale", _PF)]
+
 
 _lib = None
 

@@ -266,6 +266,7 @@ extern "C" int nnpe_bonded_set(nnpe_ctx* ctx, int kind, int ntyp, const float* c
   if (nterm) NNPE_CHECK_HIP(ctx, hipMemcpy(*pterm, terms, sizeof(int) * w * nterm, hipMemcpyHostToDevice));
   NNPE_CHECK_HIP(ctx, dev_alloc(*pstart, start.size()));
   NNPE_CHECK_HIP(ctx, hipMemcpy(*pstart, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+  ctx->start_len[kind] = int(start.size());
   NNPE_CHECK_HIP(ctx, dev_alloc(*pcoef, size_t(4) * std::max(ntyp, 1)));
   if (ntyp) NNPE_CHECK_HIP(ctx, hipMemcpy(*pcoef, coef, sizeof(float) * 4 * ntyp, hipMemcpyHostToDevice));
   if (kind == 0) { ctx->nbtyp = ntyp; ctx->nbond = nterm; }
@@ -275,14 +276,18 @@ extern "C" int nnpe_bonded_set(nnpe_ctx* ctx, int kind, int ntyp, const float* c
 }
 
 int bonded_enqueue(nnpe_ctx* ctx, bool ev) {
-  // kinds never set get an all-zero offset table so the kernel sees no terms
+  // the kernel reads offsets [0, nmol]: kinds without terms get an all-zero table of that size
+  // (again when the molecule count changed); a kind whose terms were set for another molecule
+  // table is an error
   for (int kind = 0; kind < 3; ++kind) {
     int** pstart = kind == 0 ? &ctx->bstart : kind == 1 ? &ctx->astart : &ctx->dstart;
-    if (!*pstart) {
-      std::vector<int> z(ctx->nmol + 1, 0);
-      NNPE_CHECK_HIP(ctx, dev_alloc(*pstart, z.size()));
-      NNPE_CHECK_HIP(ctx, hipMemcpy(*pstart, z.data(), sizeof(int) * z.size(), hipMemcpyHostToDevice));
-    }
+    const int nterm = kind == 0 ? ctx->nbond : kind == 1 ? ctx->nangle : ctx->ndih;
+    if (*pstart && ctx->start_len[kind] == ctx->nmol + 1) continue;
+    if (nterm > 0) return ctx_fail(ctx, NNPE_ERR_STATE, "bonded terms were set for another molecule count");
+    std::vector<int> z(ctx->nmol + 1, 0);
+    NNPE_CHECK_HIP(ctx, dev_alloc(*pstart, z.size()));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(*pstart, z.data(), sizeof(int) * z.size(), hipMemcpyHostToDevice));
+    ctx->start_len[kind] = int(z.size());
   }
   if (ctx->nbond + ctx->nangle + ctx->ndih == 0) return NNPE_OK;
   const int nblk = blocks_for(ctx->nmol, 256);

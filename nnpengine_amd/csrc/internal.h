@@ -124,6 +124,7 @@ struct nnpe_ctx {
   float *bcoef = nullptr, *acoef = nullptr, *dcoef = nullptr;
   int *bterm = nullptr, *aterm = nullptr, *dterm = nullptr;   // atom ids as given
   int *bstart = nullptr, *astart = nullptr, *dstart = nullptr; // per-molecule offsets [nmol+1]
+  int start_len[3] = {0, 0, 0};  // entries of each offset table (nmol + 1 of the molecule table it was built for)
 
   // ---- verlet --------------------------------------------------------------------------------
   double dt = 1.0;
@@ -293,3 +294,9 @@ double* ev_part(nnpe_ctx* ctx, int term, int which);  // partial array for term 
 int counters_read(nnpe_ctx* ctx);
 // upload ctx->h_box and a cell -> box map for the single-box layout
 int box_upload_single(nnpe_ctx* ctx);
+// grid and ghost parameters of one box of edge bl (cell_base 0)
+BoxPar box_geometry(float bl, float pair_cutoff, float skin, float gr_cutoff);
+// MLP tiles from ctx->h_typecount (after the element counts change)
+int mlp_retile(nnpe_ctx* ctx);
+// position update skipping the rows of boxes flagged in box_skip (device, per box)
+int verlet_enqueue_x_skip(nnpe_ctx* ctx, const unsigned char* box_skip);

@@ -383,19 +383,6 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
     for (int s = 1; s <= d.nh; ++s) { d.lo_off[s] = off; off += 32 * d.ld[s]; }
   S->lds_bytes = size_t(off) * esz;
   if (S->lds_bytes > 64 * 1024) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "MLP tile does not fit the 64 KiB LDS budget");
-  // tiles of 32 atoms per element
-  std::vector<int> toff(ctx->ntype + 1, 0);
-  ctx->h_type_off.assign(ctx->ntype, 0);
-  int acc = 0;
-  for (int t = 0; t < ctx->ntype; ++t) {
-    ctx->h_type_off[t] = acc;
-    acc += ctx->h_typecount[t];
-    d.count[t] = ctx->h_typecount[t];
-    d.type_off[t] = ctx->h_type_off[t];
-    toff[t + 1] = toff[t] + (ctx->h_typecount[t] + 31) / 32;
-  }
-  for (int t = 0; t <= kMaxType; ++t) d.tile_off[t] = toff[std::min(t, ctx->ntype)];
-  S->ntiles = toff[ctx->ntype];
   // weights: per element, per layer W [in][out] then b [out]
   size_t pofs = 0;
   std::vector<double> maxw(nlayer, 0.0), rowsum(nlayer, 0.0);
@@ -505,13 +492,35 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
     d.shift[t] = p;
     d.scale[t] = p + d.D;
   }
-  NNPE_CHECK_HIP(ctx, dev_alloc(S->d_type_off, ctx->ntype));
-  NNPE_CHECK_HIP(ctx, hipMemcpy(S->d_type_off, ctx->h_type_off.data(), sizeof(int) * ctx->ntype, hipMemcpyHostToDevice));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->type_idx, ctx->natom));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->type_fill, ctx->ntype));
-  int rc = nnp_desc_reserve(ctx, d.D);
+  int rc = mlp_retile(ctx);
+  if (rc) return rc;
+  rc = nnp_desc_reserve(ctx, d.D);
   if (rc) return rc;
   return type_index_enqueue(ctx);
+}
+
+// tiles of 32 atoms per element, from the element counts
+int mlp_retile(nnpe_ctx* ctx) {
+  MlpState* S = ctx->mlp;
+  if (!S) return NNPE_OK;
+  MlpDev& d = S->dev;
+  std::vector<int> toff(ctx->ntype + 1, 0);
+  ctx->h_type_off.assign(ctx->ntype, 0);
+  int acc = 0;
+  for (int t = 0; t < ctx->ntype; ++t) {
+    ctx->h_type_off[t] = acc;
+    acc += ctx->h_typecount[t];
+    d.count[t] = ctx->h_typecount[t];
+    d.type_off[t] = ctx->h_type_off[t];
+    toff[t + 1] = toff[t] + (ctx->h_typecount[t] + 31) / 32;
+  }
+  for (int t = 0; t <= kMaxType; ++t) d.tile_off[t] = toff[std::min(t, ctx->ntype)];
+  S->ntiles = toff[ctx->ntype];
+  if (!S->d_type_off) NNPE_CHECK_HIP(ctx, dev_alloc(S->d_type_off, ctx->ntype));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(S->d_type_off, ctx->h_type_off.data(), sizeof(int) * ctx->ntype, hipMemcpyHostToDevice));
+  return NNPE_OK;
 }
 
 int mlp_enqueue(nnpe_ctx* ctx, bool ev) {

@@ -488,11 +488,12 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
   // sides, which the reference's single-sided ghosts cannot (it needs L/2 > comm)
   if (!(ctx->bl > comm))
     return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "box too small: L must exceed skin + max(rc, r_gr) (one image shell)");
-  ctx->ghost_lim = ctx->h_bl - comm;
-  const float gl = float(0.5 * (skin + pair_cutoff));
-  ctx->g_rdx = 1.0f / (ctx->g_dx = gl);
-  ctx->g_offset = comm + ctx->h_bl;
-  ctx->num_gx = int(std::ceil((ctx->g_offset * 2) * ctx->g_rdx));
+  const BoxPar G0 = box_geometry(ctx->bl, pair_cutoff, skin, gr_cutoff);
+  ctx->ghost_lim = G0.ghost_lim;
+  ctx->g_dx = float(0.5 * (skin + pair_cutoff));
+  ctx->g_rdx = G0.g_rdx;
+  ctx->g_offset = G0.g_offset;
+  ctx->num_gx = G0.num_gx;
   ctx->num_gxsq = ctx->num_gx * ctx->num_gx;
   ctx->num_grid = ctx->num_gxsq * ctx->num_gx;
   const float lc = pair_cutoff + skin;
@@ -544,16 +545,27 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
     NNPE_CHECK_HIP(ctx, hipMemset(ctx->cnt_short, 0, sizeof(int) * ctx->ntype * ctx->stride));
   }
   ctx->nbox = 1;
-  ctx->h_box.assign(1, BoxPar{});
-  BoxPar& B = ctx->h_box[0];
-  B.bl = ctx->bl; B.r_bl = ctx->r_bl; B.h_bl = ctx->h_bl; B.m_bl = ctx->m_bl;
-  B.g_offset = ctx->g_offset; B.g_rdx = ctx->g_rdx; B.ghost_lim = ctx->ghost_lim;
-  B.num_gx = ctx->num_gx; B.cell_base = 0;
+  ctx->h_box.assign(1, G0);
   int rcb = box_upload_single(ctx);
   if (rcb) return rcb;
   ctx->have_grid = true;
   ctx->have_list = false;
   return nnp_act_reserve(ctx);
+}
+
+// ENVIRON::set_bl + init_grid + LOCAL::set_ghost_lim (src/environ.h:105, src/environ.cpp:143-155,
+// src/run.cpp:76-82) for one box of edge bl, in the reference's FP32 operations
+BoxPar box_geometry(float bl, float pair_cutoff, float skin, float gr_cutoff) {
+  BoxPar B{};
+  B.bl = bl; B.r_bl = 1.0f / bl; B.h_bl = bl * .5f; B.m_bl = -bl;
+  const float comm = skin + std::max(gr_cutoff, pair_cutoff);
+  B.ghost_lim = B.h_bl - comm;
+  const float gl = float(0.5 * (skin + pair_cutoff));
+  B.g_rdx = 1.0f / gl;
+  B.g_offset = comm + B.h_bl;
+  B.num_gx = int(std::ceil((B.g_offset * 2) * B.g_rdx));
+  B.cell_base = 0;
+  return B;
 }
 
 // one box: every cell belongs to box 0

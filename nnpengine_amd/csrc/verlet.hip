@@ -12,9 +12,12 @@ namespace {
 constexpr float kAccelF = 4.184e-4f;          // src/environ.h:78
 constexpr double kEkinF = 1195.0286802753603; // src/environ.h:79
 
-__global__ void k_update_x32(int n, float dt, float hdt, float4* xq, const float4* v, const float4* f, float4* fprev) {
+// skip (optional): rows of boxes flagged there keep their positions (a box's first step)
+__global__ void k_update_x32(int n, float dt, float hdt, float4* xq, const float4* v, const float4* f, float4* fprev,
+                             const int* rbox, const unsigned char* skip) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (skip && skip[rbox[i]]) return;
   float4 x = xq[i];
   const float4 vv = v[i], ff = f[i];
   x.x += (vv.x + ff.x * hdt) * dt;
@@ -38,9 +41,11 @@ __global__ void k_update_v32(int n, float hdt, const float* rm, float4* v, float
   v[i] = vv;
 }
 
-__global__ void k_update_x64(int n, double dt, double4* xd, const double4* vd, double4* ad, const float4* f, float4* xq) {
+__global__ void k_update_x64(int n, double dt, double4* xd, const double4* vd, double4* ad, const float4* f, float4* xq,
+                             const int* rbox, const unsigned char* skip) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (skip && skip[rbox[i]]) return;
   double4 x = xd[i];
   const double4 vv = vd[i], a = ad[i];
   const double hdt = 0.5 * dt;
@@ -148,19 +153,21 @@ extern "C" int nnpe_verlet_set(nnpe_ctx* ctx, double dt, int fp64) {
   return verlet_switch_precision(ctx, fp64 != 0);
 }
 
-int verlet_enqueue_x(nnpe_ctx* ctx) {
+int verlet_enqueue_x_skip(nnpe_ctx* ctx, const unsigned char* box_skip) {
   const int N = ctx->natom;
   if (ctx->fp64)
     hipLaunchKernelGGL(k_update_x64, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->dt, ctx->xd, ctx->vd, ctx->ad,
-                       ctx->f, ctx->xq);
+                       ctx->f, ctx->xq, ctx->rbox, box_skip);
   else {
     const float dt = float(ctx->dt), hdt = dt * .5f;  // VERLET::setDt (src/verlet.cpp:206-210)
     hipLaunchKernelGGL(k_update_x32, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, dt, hdt, ctx->xq, ctx->v, ctx->f,
-                       ctx->fprev);
+                       ctx->fprev, ctx->rbox, box_skip);
   }
   NNPE_CHECK_LAUNCH(ctx);
   return NNPE_OK;
 }
+
+int verlet_enqueue_x(nnpe_ctx* ctx) { return verlet_enqueue_x_skip(ctx, nullptr); }
 
 int verlet_enqueue_v(nnpe_ctx* ctx) {
   const int N = ctx->natom;

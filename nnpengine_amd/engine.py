@@ -282,3 +282,95 @@ class Engine:
         ms = C.c_double(0)
         self._check(self.L.nnpe_profile_stage(self.h, _lib.STAGES[stage], reps, C.byref(ms)))
         return ms.value
+
+
+class Serve:
+    """Request loop over independent boxes (nnpe_serve_*): boxes are queued with submit(), admitted
+    by the arena's atom capacity, advanced together one chunk per advance(), and collected with
+    result()."""
+
+    def __init__(self, sysm_ref, widths, params, shift, scale, precision="bf16", max_atoms=200000, max_boxes=256,
+                 pair_cutoff=9.0, skin=2.0, nnp_cutoff=6.0, rebuild_every=10, dt=0.5, fp64=True, pair_nbin=900,
+                 lj=None, device=0):
+        """sysm_ref: a System of the same kind as the requests (molecule types, bonded coefficients,
+        element count)."""
+        from nnpengine_amd import model as M
+        from nnpengine_amd import system as S
+        self.L = _lib.load()
+        self.rebuild_every = rebuild_every
+        ft, ut = S.lj_table(sysm_ref, pair_cutoff, pair_nbin, lj if lj is not None else S.WATER_LJ)
+        sfp = M.sf_params(nnp_cutoff)
+        offs, flat = sysm_ref.special_flat()
+        coefs = []
+        for c in (sysm_ref.bond_coef, sysm_ref.angle_coef, sysm_ref.dihedral_coef):
+            coefs.append(np.zeros((0, 4), np.float32) if c is None else _f32(c))
+        keep = dict(ft=_f32(ft), ut=_f32(ut), mt_natom=_i32([m.n_atom for m in sysm_ref.moltypes]), offs=_i32(offs),
+                    flat=_i32(flat), mu=_f32(sfp["mu"]), eta=_f32(sfp["eta_a"]), widths=_i32(widths), params=_f32(params),
+                    shift=_f32(shift), scale=_f32(scale), c0=coefs[0], c1=coefs[1], c2=coefs[2])
+        self._keep = keep
+        prec = {"fp32": 0, "bf16": 1, "fp16": 2}[precision]
+        cfg = _lib.ServeConfig(device, max_atoms, max_boxes, sysm_ref.ntype, pair_cutoff, skin, nnp_cutoff, rebuild_every,
+                               dt, int(fp64), pair_nbin, _p(keep["ft"], C.c_float), _p(keep["ut"], C.c_float),
+                               len(coefs[0]), len(coefs[1]), len(coefs[2]), _p(keep["c0"], C.c_float),
+                               _p(keep["c1"], C.c_float), _p(keep["c2"], C.c_float), len(sysm_ref.moltypes),
+                               _p(keep["mt_natom"], C.c_int), _p(keep["offs"], C.c_int), _p(keep["flat"], C.c_int),
+                               _p(keep["mu"], C.c_float), float(sfp["eta_r"]), _p(keep["eta"], C.c_float),
+                               len(widths) - 1, _p(keep["widths"], C.c_int), _p(keep["params"], C.c_float), 0.5, prec,
+                               _p(keep["shift"], C.c_float), _p(keep["scale"], C.c_float))
+        h = C.c_void_p()
+        rc = self.L.nnpe_serve_create(C.byref(cfg), C.byref(h))
+        if rc:
+            raise _lib.NnpeError(rc, "nnpe_serve_create")
+        self.h = h
+        self.natom = {}
+
+    def _check(self, rc):
+        if rc:
+            raise _lib.NnpeError(rc, self.L.nnpe_serve_last_error(self.h).decode())
+
+    def submit(self, sysm, nsteps):
+        """Queue a box (a System); returns the request id."""
+        k = [_f32(sysm.x), _f32(sysm.v), _f32(sysm.q), _f32(sysm.mass), _i32(sysm.typ), _i32(sysm.mol_id),
+             _i32(sysm.intra_id), _i32(sysm.mol_type)]
+        terms = [_i32(sysm.expand(n)) for n in ("bond", "angle", "dihedral")]
+        tp = [_p(t, C.c_int) if len(t) else None for t in terms]
+        rid = C.c_longlong(0)
+        self._check(self.L.nnpe_serve_submit(self.h, sysm.natom, C.c_float(sysm.bl), _p(k[0], C.c_float),
+                                             _p(k[1], C.c_float), _p(k[2], C.c_float), _p(k[3], C.c_float),
+                                             _p(k[4], C.c_int), _p(k[5], C.c_int), _p(k[6], C.c_int), sysm.nmol,
+                                             _p(k[7], C.c_int), len(terms[0]), tp[0], len(terms[1]), tp[1],
+                                             len(terms[2]), tp[2], nsteps, C.byref(rid)))
+        self.natom[rid.value] = sysm.natom
+        return rid.value
+
+    def advance(self):
+        """One chunk for every resident box; returns the ids that finished."""
+        cap = 4096
+        ids = np.zeros(cap, np.int64)
+        n = C.c_int(0)
+        self._check(self.L.nnpe_serve_advance(self.h, _p(ids, C.c_longlong), cap, C.byref(n)))
+        return ids[:min(n.value, cap)].tolist()
+
+    def result(self, rid):
+        n = self.natom.pop(rid)
+        x = np.zeros((n, 3), np.float64)
+        v = np.zeros((n, 3), np.float64)
+        self._check(self.L.nnpe_serve_result(self.h, rid, _p(x, C.c_double), _p(v, C.c_double)))
+        return x, v
+
+    def stats(self):
+        out = np.zeros(8, np.int64)
+        self._check(self.L.nnpe_serve_stats(self.h, _p(out, C.c_longlong)))
+        keys = ("resident", "atoms", "queued", "finished", "chunks", "relayouts", "arena_bytes", "max_atoms")
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nnpe_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

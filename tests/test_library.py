@@ -24,3 +24,18 @@ def test_binding_struct_matches_header():
     body = re.search(r"typedef struct nnpe_md_params \{(.*?)\} nnpe_md_params;", txt, re.S).group(1)
     fields = re.findall(r"\b(?:double|int)\s+(\w+);", body)
     assert fields == [f for f, _ in _lib.MdParams._fields_]
+
+
+def test_serve_config_struct_matches_header():
+    import re
+    with open(os.path.join(os.path.dirname(_lib.LIB_PATH), "..", "include", "nnpe.h")) as fh:
+        txt = fh.read()
+    body = re.search(r"typedef struct nnpe_serve_config \{(.*?)\} nnpe_serve_config;", txt, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names += [n.strip().lstrip("*").strip() for n in re.sub(r"^(const\s+)?\w+\s*\*?", "", decl).split(",")]
+    assert names == [f for f, _ in _lib.ServeConfig._fields_]
