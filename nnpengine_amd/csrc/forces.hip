@@ -19,6 +19,7 @@ __constant__ float c_special_lj[4] = {1.0f, 0.0f, 0.0f, 0.5f};               // 
 __constant__ float c_special_coul[4] = {1.0f, 0.0f, 0.0f, float(5 / 6.)};
 constexpr float kElectricK = 332.06371f;                                      // src/environ.h:77
 constexpr int kPairG = 16;  // lanes per atom
+constexpr int kPairU = 4;   // list entries per lane in flight at once
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   int i = min(a, b), j = max(a, b);
@@ -33,6 +34,7 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
                                               float cutsq, float cut, float rdr, float fc, float vc, int nb1,
                                               const float* __restrict__ tab, float4* __restrict__ f, double* pe,
                                               double* pw) {
+#pragma clang fp contract(fast)
   extern __shared__ float s_tab[];  // [npair][nb1] F/r
   for (int k = threadIdx.x; k < npair * nb1; k += blockDim.x) s_tab[k] = tab[k];
   __syncthreads();
@@ -47,29 +49,59 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
     float fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     const int m = cnt[i];
     const uint32_t* row = list_row(nbr, cap, i);
-    for (int k = lane; k < m; k += G) {
-      const uint32_t ent = row[k];
-      const int j = NNPE_ENTRY_J(ent), code = NNPE_ENTRY_CODE(ent), sp = NNPE_ENTRY_SP(ent);
-      const float4 xj = image_pos(xq[j], code, bl, m_bl);
-      float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-      const float rsq = dx * dx + dy * dy + dz * dz;
-      if (!(rsq < cutsq)) continue;
-      const float r = sqrtf(rsq);
-      const int p = pair_index(ti, typ[j], ntype);
-      float ipw1 = r * rdr - 0.5f;
-      const int t1 = int(ipw1), t2 = t1 + 1;
-      const float ipw2 = ipw1 - float(t1);
-      ipw1 = 1.0f - ipw2;
-      const float* ft = s_tab + p * nb1;
-      const float slj = c_special_lj[sp], scl = c_special_coul[sp];
-      float fpair = (ft[t1] * ipw1 + ft[t2] * ipw2) * slj;
-      const float qq = xi.w * xj.w * kElectricK;
-      fpair += qq * (1.0f / (r * rsq) + fc / r) * scl;
-      fx += (dx *= fpair); fy += (dy *= fpair); fz += (dz *= fpair);
-      if (EV) {
-        const float* ut = tab + npair * nb1 + p * nb1;
-        e += 0.5f * ((ut[t1] * ipw1 + ut[t2] * ipw2) * slj + qq * (1.0f / r + vc + fc * (cut - r)) * scl);
-        w += 0.5f * fpair * rsq;
+    // kPairU entries per lane and their gathers are in flight together; the entries of the
+    // next batch are loaded before this batch is computed.  Each lane still sums its pairs in
+    // list order.
+    uint32_t en[kPairU];
+#pragma unroll
+    for (int u = 0; u < kPairU; ++u) {
+      const int k = lane + u * G;
+      en[u] = k < m ? row[k] : 0u;
+    }
+    for (int k0 = lane; k0 < m; k0 += kPairU * G) {
+      uint32_t ec[kPairU];
+      float4 xr[kPairU];
+      int tr[kPairU];
+#pragma unroll
+      for (int u = 0; u < kPairU; ++u) {
+        ec[u] = en[u];
+        const int k = k0 + (kPairU + u) * G;
+        en[u] = k < m ? row[k] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kPairU; ++u) {
+        const int j = k0 + u * G < m ? (int)NNPE_ENTRY_J(ec[u]) : i;
+        xr[u] = xq[j];
+        tr[u] = typ[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kPairU; ++u) {
+        if (k0 + u * G >= m) break;
+        const uint32_t ent = ec[u];
+        const int code = NNPE_ENTRY_CODE(ent), sp = NNPE_ENTRY_SP(ent);
+        const float4 xj = image_pos(xr[u], code, bl, m_bl);
+        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float rsq = dist2_rn(dx, dy, dz);  // the cut as the reference rounds it
+        if (!(rsq < cutsq)) continue;
+        const float ir = __builtin_amdgcn_rsqf(rsq), r = rsq * ir;
+        const int p = pair_index(ti, tr[u], ntype);
+        float ipw1 = r * rdr - 0.5f;
+        const int t1 = int(ipw1), t2 = t1 + 1;
+        const float ipw2 = ipw1 - float(t1);
+        ipw1 = 1.0f - ipw2;
+        const float* ft = s_tab + p * nb1;
+        const float slj = c_special_lj[sp], scl = c_special_coul[sp];
+        float fpair = (ft[t1] * ipw1 + ft[t2] * ipw2) * slj;
+        const float qq = xi.w * xj.w * kElectricK;
+        fpair += qq * (ir * ir * ir + fc * ir) * scl;
+        fx += (dx *= fpair); fy += (dy *= fpair); fz += (dz *= fpair);
+        if (EV) {
+          const float* ut = tab + npair * nb1 + p * nb1;
+          // energies sum terms of both signs that cancel: correctly rounded r and 1/r here
+          const float re = sqrtf(rsq);
+          e += 0.5f * ((ut[t1] * ipw1 + ut[t2] * ipw2) * slj + qq * (1.0f / re + vc + fc * (cut - re)) * scl);
+          w += 0.5f * fpair * rsq;
+        }
       }
     }
     fx = group_sum<G>(fx); fy = group_sum<G>(fy); fz = group_sum<G>(fz);

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -244,11 +245,46 @@ __device__ __forceinline__ float dist2_rn(float dx, float dy, float dz) {
 // takes entries l, l+G, l+2G, ... of the atom's list row, so a group reads G consecutive 4-byte
 // entries per step, and the group's partial sums are combined by a fixed xor butterfly (every
 // lane ends with the same, order-independent-of-timing total).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
 template <int G, class T>
 __device__ __forceinline__ T group_sum(T v) {
+  if constexpr (G == 16 && std::is_same<T, float>::value) {
+    // DPP inside each 16-lane row (no LDS traffic): rotate by 8 and by 4 (row_ror), then swap
+    // pairs at distance 2 and 1 (quad_perm).  After the rotate by 8 each lane holds a_l + a_(l^8),
+    // which has period 8, so the rotate by 4 reads the same value as lane l^4 and so on: every
+    // step adds the same operands in the same order as the xor butterfly below (bit-identical).
+    v += dpp_f32<0x128>(v);
+    v += dpp_f32<0x124>(v);
+    v += dpp_f32<0x4E>(v);
+    v += dpp_f32<0xB1>(v);
+    return v;
+  } else {
 #pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-  return v;
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+  }
+}
+
+// group_sum of N values at once, step by step across the values (the same sums, bit for bit,
+// as N separate group_sum calls; independent DPP steps fill each other's hazard slots)
+template <int G, int N>
+__device__ __forceinline__ void group_sum_n(float* v) {
+  if constexpr (G == 16) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += dpp_f32<0x128>(v[k]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += dpp_f32<0x124>(v[k]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += dpp_f32<0x4E>(v[k]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += dpp_f32<0xB1>(v[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = group_sum<G>(v[k]);
+  }
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin by workgroup id.  Renumber them so each XCD

@@ -70,28 +70,31 @@ struct MlpDev {
 template <int KIND> struct Prec;
 template <> struct Prec<0> {  // FP32
   typedef float T;
+  typedef float F;  // one lane's A or B fragment of one MFMA
   static constexpr int KS = 2;
-  __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], c, 0, 0, 0);
-  }
+  static constexpr int NB = 16;  // MFMAs per batch of prefetched B fragments
+  __device__ static F ld(const T* p) { return *p; }
+  __device__ static f32x16 mma(F a, F b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0); }
   __device__ static T cvt(float v) { return v; }
   __device__ static float back(T v) { return v; }
 };
 template <> struct Prec<1> {  // BF16
   typedef __bf16 T;
+  typedef bf16x8 F;
   static constexpr int KS = 16;
-  __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8*>(a), *reinterpret_cast<const bf16x8*>(b), c, 0, 0, 0);
-  }
+  static constexpr int NB = 8;
+  __device__ static F ld(const T* p) { return *reinterpret_cast<const bf16x8*>(p); }
+  __device__ static f32x16 mma(F a, F b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
   __device__ static T cvt(float v) { return (__bf16)v; }
   __device__ static float back(T v) { return (float)v; }
 };
 template <> struct Prec<2> {  // FP16
   typedef _Float16 T;
+  typedef f16x8 F;
   static constexpr int KS = 16;
-  __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_f16(*reinterpret_cast<const f16x8*>(a), *reinterpret_cast<const f16x8*>(b), c, 0, 0, 0);
-  }
+  static constexpr int NB = 8;
+  __device__ static F ld(const T* p) { return *reinterpret_cast<const f16x8*>(p); }
+  __device__ static f32x16 mma(F a, F b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
   __device__ static T cvt(float v) { return (_Float16)v; }
   __device__ static float back(T v) { return (float)v; }
 };
@@ -101,35 +104,60 @@ __device__ __forceinline__ int crow(int lane, int j) { return 8 * (j >> 2) + 4 *
 
 // out[32 x 32 block cb] = A[32 x K] (LDS, row stride lda) * B, B fragment rows from global
 // `bm` laid out [n][K] (k contiguous).  A/B fragments: lane l holds row/col l%32 and the
-// KS/2 consecutive k starting at ks*KS + (KS/2)*(l/32).
+// KS/2 consecutive k starting at ks*KS + (KS/2)*(l/32).  The B fragments of NB consecutive
+// MFMAs are loaded into registers before the MFMA chain, so a chain waits for one round trip
+// to L2 instead of one per MFMA.
 template <int KIND>
 __device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, int lda, const typename Prec<KIND>::T* bm,
                                              int K, int cb, int lane) {
   typedef Prec<KIND> P;
-  constexpr int KS = P::KS, HALF = KS / 2;
+  typedef typename P::F F;
+  constexpr int KS = P::KS, HALF = KS / 2, NB = P::NB;
   f32x16 acc;
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
   const typename P::T* arow = A + (lane & 31) * lda + HALF * (lane >> 5);
   const typename P::T* brow = bm + (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
-  for (int k0 = 0; k0 < K; k0 += KS) acc = P::mma(arow + k0, brow + k0, acc);
+  for (int kb = 0; kb < K; kb += NB * KS) {
+    F b[NB];
+#pragma unroll
+    for (int s = 0; s < NB; ++s)
+      if (kb + s * KS < K) b[s] = P::ld(brow + kb + s * KS);
+#pragma unroll
+    for (int s = 0; s < NB; ++s)
+      if (kb + s * KS < K) acc = P::mma(P::ld(arow + kb + s * KS), b[s], acc);
+  }
   return acc;
 }
 
-// BF16 pair products: sum of A.B over the hi/lo parts given (nullptr = part absent)
+// BF16 pair products: sum of A.B over the hi/lo parts given (nullptr = part absent), B
+// fragments prefetched as in gemm_block
 __device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf16* Alo, int lda, const __bf16* Bhi,
                                                    const __bf16* Blo, int K, int cb, int lane) {
   typedef Prec<1> P;
-  constexpr int KS = P::KS, HALF = KS / 2;
+  typedef P::F F;
+  constexpr int KS = P::KS, HALF = KS / 2, NB = P::NB;
   f32x16 acc;
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
   const size_t ao = (size_t)(lane & 31) * lda + HALF * (lane >> 5);
   const size_t bo = (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
-  for (int k0 = 0; k0 < K; k0 += KS) {
-    acc = P::mma(Ahi + ao + k0, Bhi + bo + k0, acc);
-    if (Blo) acc = P::mma(Ahi + ao + k0, Blo + bo + k0, acc);
-    if (Alo) acc = P::mma(Alo + ao + k0, Bhi + bo + k0, acc);
+  for (int kb = 0; kb < K; kb += NB * KS) {
+    F bh[NB], bl[NB];
+#pragma unroll
+    for (int s = 0; s < NB; ++s)
+      if (kb + s * KS < K) {
+        bh[s] = P::ld(Bhi + bo + kb + s * KS);
+        if (Blo) bl[s] = P::ld(Blo + bo + kb + s * KS);
+      }
+#pragma unroll
+    for (int s = 0; s < NB; ++s)
+      if (kb + s * KS < K) {
+        const F ah = P::ld(Ahi + ao + kb + s * KS);
+        acc = P::mma(ah, bh[s], acc);
+        if (Blo) acc = P::mma(ah, bl[s], acc);
+        if (Alo) acc = P::mma(P::ld(Alo + ao + kb + s * KS), bh[s], acc);
+      }
   }
   return acc;
 }
@@ -154,14 +182,39 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
   T* Gs = lds + P.slab_off[0];
   const int ldg = P.ld[0];
 
-  // stage the descriptor rows standardised in fp32, zero padding rows/cols
+  // stage the descriptor rows standardised in fp32, zero padding rows/cols; row ids go through
+  // LDS so every global load of the loop is independent of the others (they overlap)
+  __shared__ int srow[32];
+  if (tid < 32) srow[tid] = tid < nrows ? rows[tid] : -1;
+  __syncthreads();
   const float* shv = P.shift[e];
   const float* scv = P.scale[e];
-  for (int idx = tid; idx < 32 * Dp; idx += NT) {
-    const int r = idx / Dp, c = idx - r * Dp;
-    float v = 0.0f;
-    if (r < nrows && c < D) v = (G[(size_t)rows[r] * D + c] - shv[c]) * scv[c];
-    Gs[r * ldg + c] = Prec<KIND>::cvt(v);
+  if ((D & 3) == 0) {
+    const int D4 = D >> 2, Dp4 = Dp >> 2;
+    const float4* G4 = reinterpret_cast<const float4*>(G);
+    const float4* sh4 = reinterpret_cast<const float4*>(shv);
+    const float4* sc4 = reinterpret_cast<const float4*>(scv);
+#pragma unroll 4
+    for (int idx = tid; idx < 32 * Dp4; idx += NT) {
+      const int r = idx / Dp4, c4 = idx - r * Dp4;
+      const int row = srow[r];
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (row >= 0 && c4 < D4) {
+        const float4 g = G4[(size_t)row * D4 + c4], sh = sh4[c4], sc = sc4[c4];
+        v = make_float4((g.x - sh.x) * sc.x, (g.y - sh.y) * sc.y, (g.z - sh.z) * sc.z, (g.w - sh.w) * sc.w);
+      }
+      T* d = Gs + r * ldg + 4 * c4;
+      d[0] = Prec<KIND>::cvt(v.x); d[1] = Prec<KIND>::cvt(v.y); d[2] = Prec<KIND>::cvt(v.z); d[3] = Prec<KIND>::cvt(v.w);
+    }
+  } else {
+#pragma unroll 4
+    for (int idx = tid; idx < 32 * Dp; idx += NT) {
+      const int r = idx / Dp, c = idx - r * Dp;
+      const int row = srow[r];
+      float v = 0.0f;
+      if (row >= 0 && c < D) v = (G[(size_t)row * D + c] - shv[c]) * scv[c];
+      Gs[r * ldg + c] = Prec<KIND>::cvt(v);
+    }
   }
   __syncthreads();
 
@@ -189,33 +242,32 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
     __syncthreads();
   }
 
-  // energy of each row: lanes l and l+32 split the last hidden layer
-  const int HL = P.H[nh - 1];
-  const T* aL = lds + P.slab_off[nh];
-  const int ldl = P.ld[nh];
-  if (wave == 0) {
-    const int r = lane & 31, half = lane >> 5;
-    const float* wo = P.wout[e];
-    float s = 0.0f;
-    for (int k = half * (HL / 2); k < (half + 1) * (HL / 2); ++k) s += Prec<KIND>::back(aL[r * ldl + k]) * wo[k];
-    s += __shfl_xor(s, 32, 64);
-    if (half == 0 && r < nrows) e_atom[rows[r]] = P.out_scale * (s + P.bout[e]);
-  }
-  __syncthreads();
-
-  // backward: d_L = s_out * w_out * (1 - a_L^2), stored in place of a_L (scaled by s_d[nh-1])
+  // energy of each row and d_L = s_out * w_out * (1 - a_L^2) (stored in place of a_L, scaled by
+  // s_d[nh-1]) in one pass: 8 threads per row, each over every 8th column, then a 3-step
+  // butterfly over the 8
+  static_assert(NT == 256, "one row per 8 threads");
   {
+    const int HL = P.H[nh - 1];
     T* dL = lds + P.slab_off[nh];
+    const int ldl = P.ld[nh];
     const float* wo = P.wout[e];
     const float sc = P.out_scale * P.s_d[nh - 1];
-    for (int idx = tid; idx < 32 * HL; idx += NT) {
-      const int r = idx / HL, c = idx - r * HL;
+    const int r = tid >> 3, sub = tid & 7;
+    float s = 0.0f;
+#pragma unroll 4
+    for (int c = sub; c < HL; c += 8) {
       const float a = Prec<KIND>::back(dL[r * ldl + c]);
-      const float v = sc * wo[c] * (1.0f - a * a);
+      const float w = wo[c];
+      s += a * w;
+      const float v = sc * w * (1.0f - a * a);
       const T hi = Prec<KIND>::cvt(v);
       dL[r * ldl + c] = hi;
       if constexpr (KIND == 1) lds[P.lo_off[nh] + r * ldl + c] = Prec<KIND>::cvt(v - Prec<KIND>::back(hi));
     }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 4, 64);
+    if (sub == 0 && r < nrows) e_atom[srow[r]] = P.out_scale * (s + P.bout[e]);
   }
   __syncthreads();
   for (int l = nh - 1; l >= 1; --l) {
@@ -265,7 +317,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int r = crow(lane, j);
-          if (r < nrows) dG[(size_t)rows[r] * D + col] = acc[j] * sc;
+          if (r < nrows) dG[(size_t)srow[r] * D + col] = acc[j] * sc;
         }
       }
     }

@@ -35,6 +35,7 @@ constexpr int kNMom = 11;      // A0, A1[3], A2[xx,yy,zz,xy,xz,yz], S
 constexpr int kSlice = 64;
 constexpr int kSliceRad = kNEta * kNMom;
 constexpr float kPi = 3.14159265358979f;
+constexpr int kSfU = 4;  // list entries per lane in flight at once (descriptor and force passes)
 constexpr int kTblG = 16;      // lanes per atom, tabulated form
 constexpr int kSfG = 16;       // lanes per atom, symmetry functions
 
@@ -154,16 +155,23 @@ __global__ void __launch_bounds__(256) k_tbl_force(int n, int cap, const uint32_
 // (2) symmetry functions
 
 struct SfParams {
-  float rc, rcsq, eta_r;
+  float rc, rcsq, eta_r, rc_inv;
   float mu[kNRad];
   float eta[kNEta];
 };
 
-__device__ __forceinline__ void sf_cut(float r, float rc, float& fc, float& dfc) {
-  float s, c;
-  sincosf(kPi * r / rc, &s, &c);
-  fc = 0.5f * (c + 1.0f);
-  dfc = -0.5f * kPi / rc * s;
+// fc = (cos(pi r/rc) + 1)/2 and dfc/dr for 0 <= r <= rc.  With y = pi (r/rc - 1/2), |y| <= pi/2:
+// cos(pi r/rc) = -sin(y), sin(pi r/rc) = cos(y), both from their Taylor series about 0 (sin to
+// y^11, cos to y^12; truncation below 6e-8 on the range), evaluated by Horner's rule.
+__device__ __forceinline__ void sf_cut(float r, float rc_inv, float& fc, float& dfc) {
+#pragma clang fp contract(fast)
+  const float y = kPi * (r * rc_inv - 0.5f), y2 = y * y;
+  const float s =
+      y * (1.0f + y2 * (-1.0f / 6 + y2 * (1.0f / 120 + y2 * (-1.0f / 5040 + y2 * (1.0f / 362880 + y2 * (-1.0f / 39916800))))));
+  const float c =
+      1.0f + y2 * (-0.5f + y2 * (1.0f / 24 + y2 * (-1.0f / 720 + y2 * (1.0f / 40320 + y2 * (-1.0f / 3628800 + y2 * (1.0f / 479001600))))));
+  fc = 0.5f * (1.0f - s);
+  dfc = -0.5f * kPi * rc_inv * c;
 }
 
 // moments per atom: [NT][kNEta][kNMom] then radial [NT][kNRad].  A group of G lanes per atom,
@@ -179,6 +187,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
                                                  SfParams P, float* __restrict__ Mout,
                                                  float* __restrict__ G_out, uint32_t* __restrict__ act,
                                                  int* __restrict__ act_cnt) {
+#pragma clang fp contract(fast)
   constexpr int NP = NT * (NT + 1) / 2;
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int RAD0 = NT * kNEta * kNMom;
@@ -210,55 +219,72 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
       const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (threadIdx.x & 63 & ~(G - 1));
       const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1ull;
       int na = 0;
-      for (int q0 = 0; q0 < cn; q0 += G) {
-        const int q = q0 + lane;
-        uint32_t ent = 0;
-        float ux = 0, uy = 0, uz = 0, rsq = 0;
-        bool in = false;
-        if (q < cn) {
-          ent = row[q];
-          const float4 xj = image_pos(xq[NNPE_ENTRY_J(ent)], NNPE_ENTRY_CODE(ent), bl, m_bl);
-          ux = xj.x - xi.x; uy = xj.y - xi.y; uz = xj.z - xi.z;
-          rsq = ux * ux + uy * uy + uz * uz;
-          in = rsq < P.rcsq;
-        }
-        const unsigned long long bal = __ballot(in) & gmask;
-        if (in) arow[na + __popcll(bal & below)] = ent;
-        na += __popcll(bal);
-        if (!in) continue;
-        const float r = sqrtf(rsq), ir = 1.0f / r;
-        const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
-        float fc, dfc;
-        sf_cut(r, P.rc, fc, dfc);
+      // kSfU entries per lane and their position gathers are in flight together; lanes still
+      // accumulate their entries in list order and the ballots run in list order
+      for (int q0 = 0; q0 < cn; q0 += kSfU * G) {
+        uint32_t ec[kSfU];
+        float4 xr[kSfU];
 #pragma unroll
-        for (int k = 0; k < kNRad; ++k) {
-          const float d = r - P.mu[k];
-          R[k] += __expf(-P.eta_r * d * d) * fc;
+        for (int u = 0; u < kSfU; ++u) {
+          const int q = q0 + u * G + lane;
+          ec[u] = q < cn ? row[q] : 0u;
         }
 #pragma unroll
-        for (int m = 0; m < kNEta; ++m) {
-          const float w = __expf(-P.eta[m] * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
-          A[m][0] += w;
-          A[m][1] += wx; A[m][2] += wy; A[m][3] += wz;
-          A[m][4] += wx * hx; A[m][5] += wy * hy; A[m][6] += wz * hz;
-          A[m][7] += wx * hy; A[m][8] += wx * hz; A[m][9] += wy * hz;
-          A[m][10] += w * w;
+        for (int u = 0; u < kSfU; ++u) {
+          const int q = q0 + u * G + lane;
+          xr[u] = xq[q < cn ? (int)NNPE_ENTRY_J(ec[u]) : i];
+        }
+#pragma unroll
+        for (int u = 0; u < kSfU; ++u) {
+          if (q0 + u * G >= cn) break;  // uniform across the group
+          const int q = q0 + u * G + lane;
+          const uint32_t ent = ec[u];
+          float ux = 0, uy = 0, uz = 0, rsq = 0;
+          bool in = false;
+          if (q < cn) {
+            const float4 xj = image_pos(xr[u], NNPE_ENTRY_CODE(ent), bl, m_bl);
+            ux = xj.x - xi.x; uy = xj.y - xi.y; uz = xj.z - xi.z;
+            rsq = dist2_rn(ux, uy, uz);  // rounded as written: the force pass makes the same cut
+            in = rsq < P.rcsq;
+          }
+          const unsigned long long bal = __ballot(in) & gmask;
+          if (in) arow[na + __popcll(bal & below)] = ent;
+          na += __popcll(bal);
+          if (!in) continue;
+          const float ir = __builtin_amdgcn_rsqf(rsq), r = rsq * ir;
+          const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
+          float fc, dfc;
+          sf_cut(r, P.rc_inv, fc, dfc);
+#pragma unroll
+          for (int k = 0; k < kNRad; ++k) {
+            const float d = r - P.mu[k];
+            R[k] += __expf(-P.eta_r * d * d) * fc;
+          }
+#pragma unroll
+          for (int m = 0; m < kNEta; ++m) {
+            const float w = __expf(-P.eta[m] * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
+            A[m][0] += w;
+            A[m][1] += wx; A[m][2] += wy; A[m][3] += wz;
+            A[m][4] += wx * hx; A[m][5] += wy * hy; A[m][6] += wz * hz;
+            A[m][7] += wx * hy; A[m][8] += wx * hz; A[m][9] += wy * hz;
+            A[m][10] += w * w;
+          }
         }
       }
       if (lane == 0) act_cnt[(size_t)a * stride + i] = na;
+      group_sum_n<G, kNEta * kNMom>(&A[0][0]);
+      group_sum_n<G, kNRad>(R);
 #pragma unroll
       for (int m = 0; m < kNEta; ++m)
 #pragma unroll
         for (int c = 0; c < kNMom; ++c) {
-          const float v = group_sum<G>(A[m][c]);
           const int idx = (a * kNEta + m) * kNMom + c;
-          if (idx % G == lane) { mo[idx] = v; sm[idx] = v; }
+          if (idx % G == lane) { mo[idx] = A[m][c]; sm[idx] = A[m][c]; }
         }
 #pragma unroll
       for (int k = 0; k < kNRad; ++k) {
-        const float v = group_sum<G>(R[k]);
         const int idx = RAD0 + a * kNRad + k;
-        if (idx % G == lane) { mo[idx] = v; sm[idx] = v; }
+        if (idx % G == lane) { mo[idx] = R[k]; sm[idx] = R[k]; }
       }
     }
   }
@@ -289,6 +315,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
 template <int NT>
 __global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restrict__ M, const float* __restrict__ dG,
                                                     float* __restrict__ adj) {
+#pragma clang fp contract(fast)
   constexpr int NP = NT * (NT + 1) / 2;
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int D = NT * kNRad + NP * kNEta * 4;
@@ -344,6 +371,7 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restri
 // adjoint row: the angular part (vector) and the radial part's scalar folded into `along`
 __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const float* wv, const float* dwv, float ir,
                                             float hx, float hy, float hz, float& along, float& gx, float& gy, float& gz) {
+#pragma clang fp contract(fast)
 #pragma unroll
   for (int m = 0; m < kNEta; ++m) {
     const float* e = d + m * kNMom;
@@ -371,6 +399,7 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
                                                   const int* __restrict__ rbox, SfParams P,
                                                   const float* __restrict__ adj, float4* __restrict__ f, double* pe,
                                                   double* pw) {
+#pragma clang fp contract(fast)
   constexpr int NMT = NT * kSlice;
   __shared__ float s_adj[256 / G][NMT];
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
@@ -389,17 +418,30 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
     for (int tj = 0; tj < NT; ++tj) {
     const int cn = cnt[(size_t)tj * stride + i];
     const uint32_t* row = nbr + ((size_t)tj * stride + i) * cap;
-    for (int q = lane; q < cn; q += G) {
-      const uint32_t ent = row[q];
+    // kSfU pairs per lane: entries and positions gathered together, then the pairs in order
+    for (int q0 = lane; q0 < cn; q0 += kSfU * G) {
+    uint32_t ec[kSfU];
+    float4 xr[kSfU];
+#pragma unroll
+    for (int u = 0; u < kSfU; ++u) {
+      const int q = q0 + u * G;
+      ec[u] = q < cn ? row[q] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kSfU; ++u) xr[u] = xq[q0 + u * G < cn ? (int)NNPE_ENTRY_J(ec[u]) : i];
+#pragma unroll
+    for (int u = 0; u < kSfU; ++u) {
+      if (q0 + u * G >= cn) break;
+      const uint32_t ent = ec[u];
       const int j = NNPE_ENTRY_J(ent);
-      const float4 xj = image_pos(xq[j], NNPE_ENTRY_CODE(ent), bl, m_bl);
+      const float4 xj = image_pos(xr[u], NNPE_ENTRY_CODE(ent), bl, m_bl);
       const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
-      const float rsq = ux * ux + uy * uy + uz * uz;
+      const float rsq = dist2_rn(ux, uy, uz);
       if (!(rsq < P.rcsq)) continue;
-      const float r = sqrtf(rsq), ir = 1.0f / r;
+      const float ir = __builtin_amdgcn_rsqf(rsq), r = rsq * ir;
       const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
       float fc, dfc;
-      sf_cut(r, P.rc, fc, dfc);
+      sf_cut(r, P.rc_inv, fc, dfc);
       float wv[kNEta], dwv[kNEta];
 #pragma unroll
       for (int m = 0; m < kNEta; ++m) {
@@ -434,6 +476,7 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
       fz += gzi - gzj + rad * hz;
       // W_i = -(dE_i/du . u), own descriptor only
       if (EV) vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
+    }
     }
     }
     fx = group_sum<G>(fx); fy = group_sum<G>(fy); fz = group_sum<G>(fz);
@@ -619,7 +662,7 @@ extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, f
   if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
   if (ctx->ntype < 1 || ctx->ntype > 4) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "symmetry functions support 1-4 elements");
   NnpState& S = nnp_state(ctx);
-  S.P.rc = rc; S.P.rcsq = rc * rc; S.P.eta_r = eta_r;
+  S.P.rc = rc; S.P.rcsq = rc * rc; S.P.eta_r = eta_r; S.P.rc_inv = 1.0f / rc;
   for (int k = 0; k < kNRad; ++k) S.P.mu[k] = mu[k];
   for (int k = 0; k < kNEta; ++k) S.P.eta[k] = eta[k];
   S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
