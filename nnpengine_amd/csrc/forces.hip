@@ -15,15 +15,15 @@
 
 namespace {
 
-__constant__ float c_special_lj[4] = {1.0f, 0.0f, 0.0f, 0.5f};               // src/molecule.h:11
-__constant__ float c_special_coul[4] = {1.0f, 0.0f, 0.0f, float(5 / 6.)};
+// special-pair factors (src/molecule.h:11): class 0 full, classes 1 and 2 excluded, class 3 scaled
+constexpr float kSpecialLj3 = 0.5f, kSpecialCoul3 = float(5 / 6.);
 constexpr float kElectricK = 332.06371f;                                      // src/environ.h:77
 constexpr int kPairG = 16;  // lanes per atom
 constexpr int kPairU = 4;   // list entries per lane in flight at once
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
-  int i = min(a, b), j = max(a, b);
-  return i * ntype - (i * (i - 1)) / 2 + (j - i);
+  const int i = min(a, b), j = max(a, b);  // small non-negative values: 24-bit multiplies
+  return __mul24(i, ntype) - (__mul24(i, i - 1) >> 1) + (j - i);
 }
 
 template <bool EV, int G>
@@ -89,14 +89,16 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
         const int t1 = int(ipw1), t2 = t1 + 1;
         const float ipw2 = ipw1 - float(t1);
         ipw1 = 1.0f - ipw2;
-        const float* ft = s_tab + p * nb1;
-        const float slj = c_special_lj[sp], scl = c_special_coul[sp];
+        const float* ft = s_tab + __mul24(p, nb1);
+        // special-pair factors by select (a per-lane index into constant memory is a load)
+        const float slj = sp == 0 ? 1.0f : (sp == 3 ? kSpecialLj3 : 0.0f);
+        const float scl = sp == 0 ? 1.0f : (sp == 3 ? kSpecialCoul3 : 0.0f);
         float fpair = (ft[t1] * ipw1 + ft[t2] * ipw2) * slj;
         const float qq = xi.w * xj.w * kElectricK;
         fpair += qq * (ir * ir * ir + fc * ir) * scl;
         fx += (dx *= fpair); fy += (dy *= fpair); fz += (dz *= fpair);
         if (EV) {
-          const float* ut = tab + npair * nb1 + p * nb1;
+          const float* ut = tab + __mul24(npair + p, nb1);
           // energies sum terms of both signs that cancel: correctly rounded r and 1/r here
           const float re = sqrtf(rsq);
           e += 0.5f * ((ut[t1] * ipw1 + ut[t2] * ipw2) * slj + qq * (1.0f / re + vc + fc * (cut - re)) * scl);

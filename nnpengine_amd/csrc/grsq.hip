@@ -22,8 +22,8 @@ constexpr int kHistLdsMax = 48 * 1024;
 constexpr int kGrG = 16;  // lanes per atom, inner tally
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
-  int i = min(a, b), j = max(a, b);
-  return i * ntype - (i * (i - 1)) / 2 + (j - i);
+  const int i = min(a, b), j = max(a, b);  // small non-negative values: 24-bit multiplies
+  return __mul24(i, ntype) - (__mul24(i, i - 1) >> 1) + (j - i);
 }
 
 __device__ __forceinline__ int gr_bin(float rsq, float rdr) { return int(__fmul_rn(sqrtf(rsq), rdr)); }
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_gr_inner(int n, int cap, const uint32_t
   }
 }
 
-// Full tally by home cell: a persistent workgroup takes a contiguous run of home cells; for each,
+// Full tally by home cell: persistent workgroups take the cells round-robin; for each home cell,
 // the candidates of its stencil cells (real rows, then ghost images, per stencil cell) are
 // staged in LDS in batches and each wave tests 64 of them per step against one home atom,
 // adding 2 for a real j with a larger row (each unordered real pair once) and 1 for a ghost
@@ -100,16 +100,18 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
   __shared__ int r_begin[kGrMaxRanges], r_pref[kGrMaxRanges + 1];
   __shared__ unsigned char r_ghost[kGrMaxRanges];
   __shared__ float4 c_pos[kGrBatch];
-  __shared__ int c_row[kGrBatch];  // real row, or -1 - type for a ghost image
-  __shared__ int s_tot;
+  __shared__ int c_row[kGrBatch];           // real row, or -1 for a ghost image
+  __shared__ unsigned char c_typ[kGrBatch];
+  __shared__ int s_wsum[256 / 64 + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (kLds) {
     for (int k = tid; k < npair * nbin; k += blockDim.x) s_hist[k] = 0u;
   }
-  const int per = (ncell + gridDim.x - 1) / gridDim.x;
-  const int c0 = xcd_block(gridDim.x) * per, c1 = min(ncell, c0 + per);
   const int nr2 = 2 * nst;
-  for (int c = c0; c < c1; ++c) {
+  // cells dealt round-robin: the cells holding atoms form a block in the middle of the grid
+  // (the margin holds only ghosts), so contiguous runs would load some workgroups with many
+  // occupied cells and leave others idle
+  for (int c = xcd_block(gridDim.x); c < ncell; c += gridDim.x) {
     const int hb = cstart[c], he = cstart[c + 1];
     if (hb == he) continue;  // uniform across the workgroup
     const BoxPar B = boxes[cell_box[c]];
@@ -130,14 +132,7 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
       r_begin[2 * s + 1] = bg; r_pref[2 * s + 1] = ng; r_ghost[2 * s + 1] = 1;
     }
     __syncthreads();
-    if (tid == 0) {
-      int run = 0;
-      for (int k = 0; k < nr2; ++k) { const int v = r_pref[k]; r_pref[k] = run; run += v; }
-      r_pref[nr2] = run;
-      s_tot = run;
-    }
-    __syncthreads();
-    const int total = s_tot;
+    const int total = block_scan_excl<256>(r_pref, nr2, s_wsum);
     for (int t0 = 0; t0 < total; t0 += kGrBatch) {
       const int tn = min(kGrBatch, total - t0);
       __syncthreads();
@@ -152,10 +147,12 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
         if (r_ghost[lo]) {
           const int src = gs_src[pos];
           c_pos[u] = image_pos(xq[src], gs_code[pos], bl, m_bl);
-          c_row[u] = -1 - typ[src];
+          c_row[u] = -1;
+          c_typ[u] = (unsigned char)typ[src];
         } else {
           c_pos[u] = xq[pos];
           c_row[u] = pos;
+          c_typ[u] = (unsigned char)typ[pos];
         }
       }
       __syncthreads();
@@ -170,8 +167,7 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
           if (!(rsq < rmaxsq)) continue;
           const int b = gr_bin(rsq, rdr);
           if (b >= nbin) continue;
-          const int tj = jr >= 0 ? typ[jr] : -1 - jr;
-          const int p = pair_index(ti, tj, ntype);
+          const int p = pair_index(ti, c_typ[u], ntype);
           const unsigned int wgt = jr >= 0 ? 2u : 1u;
           if (kLds) atomicAdd(&s_hist[p * nbin + b], wgt);
           else atomicAdd(&count[(size_t)p * nbin + b], (unsigned long long)wgt);
@@ -280,7 +276,7 @@ int gr_enqueue(nnpe_ctx* ctx, bool full) {
     if (ctx->n_stencil_gr > kGrMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "g(r) stencil too large");
     const size_t lds = size_t(npair) * nb * sizeof(unsigned int);
     const int G = ctx->num_grid;
-    const int nblk = std::max(1, std::min(1024, G));  // persistent workgroups, contiguous runs of cells
+    const int nblk = std::max(1, std::min(1024, G));  // persistent workgroups, cells round-robin
     if (lds <= kHistLdsMax)
       hipLaunchKernelGGL(k_gr_cells<true>, dim3(nblk), dim3(256), lds, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_gr,
                          ctx->n_stencil_gr, ctx->xq, ctx->typ, ctx->cell_start, ctx->gcell_start, ctx->gs_src,

@@ -226,13 +226,17 @@ inline int blocks_for(long long n, int bs) { return int((n + bs - 1) / bs); }
 __device__ __forceinline__ float image_shift(int c, float bl, float m_bl) {
   return c == 1 ? 0.0f : (c > 1 ? bl : m_bl);
 }
+// code = 9a + 3b + c with a, b, c in 0..2, decoded by exact multiply-shift divisions on the
+// code range 0..26 (a = code/9 = (57 code) >> 9, b = rem/3 = (11 rem) >> 5 for rem <= 8)
 __device__ __forceinline__ float4 image_pos(float4 xs, int code, float bl, float m_bl) {
-  if (code == 13) return xs;
+  const unsigned u = (unsigned)code;
+  const unsigned a = __umul24(u, 57u) >> 9, rem = u - __umul24(9u, a);
+  const unsigned b = __umul24(rem, 11u) >> 5, c = rem - __umul24(3u, b);
   float4 r = xs;
-  r.x = __fadd_rn(xs.x, image_shift(code / 9, bl, m_bl));
-  r.y = __fadd_rn(xs.y, image_shift((code / 3) % 3, bl, m_bl));
-  r.z = __fadd_rn(xs.z, image_shift(code % 3, bl, m_bl));
-  return r;
+  r.x = __fadd_rn(xs.x, image_shift((int)a, bl, m_bl));
+  r.y = __fadd_rn(xs.y, image_shift((int)b, bl, m_bl));
+  r.z = __fadd_rn(xs.z, image_shift((int)c, bl, m_bl));
+  return code == 13 ? xs : r;
 }
 // (a-b)^2 summed left to right with IEEE single rounding and no contraction, as the
 // reference's sqr(dx)+sqr(dy)+sqr(dz) (src/neighlist.cpp:260)
@@ -285,6 +289,38 @@ __device__ __forceinline__ void group_sum_n(float* v) {
 #pragma unroll
     for (int k = 0; k < N; ++k) v[k] = group_sum<G>(v[k]);
   }
+}
+
+// Exclusive prefix sum of v[0..n) in place by the whole workgroup of NT threads (each thread
+// scans a contiguous chunk, wave totals through LDS); v[n] receives the total, which is also
+// returned.  s_wsum: NT/64 + 1 ints of LDS.  Ends with a barrier.
+template <int NT>
+__device__ __forceinline__ int block_scan_excl(int* v, int n, int* s_wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int k = (n + NT - 1) / NT;
+  const int b = min(tid * k, n), e = min(b + k, n);
+  int sum = 0;
+  for (int q = b; q < e; ++q) sum += v[q];
+  int inc = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) s_wsum[w] = inc;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int q = 0; q < NT / 64; ++q) { const int t = s_wsum[q]; s_wsum[q] = run; run += t; }
+    s_wsum[NT / 64] = run;
+  }
+  __syncthreads();
+  int run = s_wsum[w] + inc - sum;
+  for (int q = b; q < e; ++q) { const int t = v[q]; v[q] = run; run += t; }
+  const int total = s_wsum[NT / 64];
+  if (tid == 0) v[n] = total;
+  __syncthreads();
+  return total;
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin by workgroup id.  Renumber them so each XCD

@@ -151,32 +151,51 @@ __global__ void k_copy_back(int n, bool fp64, const float4* xq2, const float4* v
 // LOCAL::generate_ghost + update_ghost_pos<true> (src/environ.cpp:401-442, :488-514)
 __global__ void k_ghost_gen(int n, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox, int cap,
                             const float4* xq, int* g_src, int* g_code, int* g_cell, int* gcell_cnt, DevCounters* cnt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const BoxPar B = boxes[rbox[i]];
-  const float ghost_lim = B.ghost_lim, bl = B.bl, m_bl = B.m_bl, g_offset = B.g_offset, g_rdx = B.g_rdx;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  BoxPar B{};
+  float4 p = make_float4(0, 0, 0, 0);
+  int xs = 0, xe = 0, ys = 0, ye = 0, zs = 0, ze = 0;
+  if (i < n) {
+    B = boxes[rbox[i]];
+    p = xq[i];
+    const float ghost_lim = B.ghost_lim, mg = -ghost_lim;
+    xs = p.x > ghost_lim ? -1 : 0; xe = p.x < mg ? 1 : 0;
+    ys = p.y > ghost_lim ? -1 : 0; ye = p.y < mg ? 1 : 0;
+    zs = p.z > ghost_lim ? -1 : 0; ze = p.z < mg ? 1 : 0;
+  }
+  const int nimg = (xe - xs + 1) * (ye - ys + 1) * (ze - zs + 1) - 1;  // 0 for no image
+  // slots: one atomic per wave, each lane's images at its exclusive prefix (the cell sort
+  // orders the ghosts afterwards, so the slot order does not matter)
+  int inc = nimg;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  const int wtot = __shfl(inc, 63, 64);
+  int base = 0;
+  if (lane == 0 && wtot > 0) base = atomicAdd(&cnt->nghost, wtot);
+  base = __shfl(base, 0, 64);
+  if (nimg == 0) return;
+  int slot = base + inc - nimg;
+  const float bl = B.bl, m_bl = B.m_bl, g_offset = B.g_offset, g_rdx = B.g_rdx;
   const int num_gx = B.num_gx;
-  float4 p = xq[i];
-  const float mg = -ghost_lim;
-  int xs = p.x > ghost_lim ? -1 : 0, xe = p.x < mg ? 1 : 0;
-  int ys = p.y > ghost_lim ? -1 : 0, ye = p.y < mg ? 1 : 0;
-  int zs = p.z > ghost_lim ? -1 : 0, ze = p.z < mg ? 1 : 0;
-  if (xs == xe && ys == ye && zs == ze) return;
   for (int a = xs; a <= xe; ++a)
     for (int b = ys; b <= ye; ++b)
       for (int c = zs; c <= ze; ++c) {
         if (a == 0 && b == 0 && c == 0) continue;
-        int code = (a + 1) * 9 + (b + 1) * 3 + (c + 1);
-        int slot = atomicAdd(&cnt->nghost, 1);
-        if (slot >= cap) { atomicAdd(&cnt->ghost_overflow, 1); continue; }
+        const int code = (a + 1) * 9 + (b + 1) * 3 + (c + 1);
+        const int sl = slot++;
+        if (sl >= cap) { atomicAdd(&cnt->ghost_overflow, 1); continue; }
         float4 g = image_pos(p, code, bl, m_bl);
         int cx = cell_coord(g_offset, g.x, g_rdx), cy = cell_coord(g_offset, g.y, g_rdx), cz = cell_coord(g_offset, g.z, g_rdx);
         if (cx < 0 || cy < 0 || cz < 0 || cx >= num_gx || cy >= num_gx || cz >= num_gx) {
           atomicAdd(&cnt->cell_range, 1);
           cx = min(max(cx, 0), num_gx - 1); cy = min(max(cy, 0), num_gx - 1); cz = min(max(cz, 0), num_gx - 1);
         }
-        int gc = B.cell_base + (cx * num_gx + cy) * num_gx + cz;
-        g_src[slot] = i; g_code[slot] = code; g_cell[slot] = gc;
+        const int gc = B.cell_base + (cx * num_gx + cy) * num_gx + cz;
+        g_src[sl] = i; g_code[sl] = code; g_cell[sl] = gc;
         atomicAdd(&gcell_cnt[gc], 1);
       }
 }
@@ -285,12 +304,14 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
   __shared__ int2 c_mol[kBuildBatch];                // molecule id, intra id
   __shared__ unsigned char c_typ[kBuildBatch];
   __shared__ int h_nl[kBuildHome], h_ns[kBuildHome][kShortMaxType];
-  __shared__ int s_tot;
+  __shared__ int s_wsum[256 / 64 + 1];
+  __shared__ int s_max[2];  // longest full and short row of this workgroup
   const int c = xcd_block(gridDim.x);
   if (c >= ncell) return;
   const int hb = cstart[c], he = cstart[c + 1];
   if (hb == he) return;  // uniform across the workgroup
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < 2) s_max[tid] = 0;
   const BoxPar B = boxes[cell_box[c]];
   const float bl = B.bl, m_bl = B.m_bl;
   const int num_gx = B.num_gx, lc = c - B.cell_base;
@@ -310,14 +331,8 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
     r_begin[2 * s + 1] = bg; r_pref[2 * s + 1] = ng; r_flag[2 * s + 1] = nock | 1;
   }
   __syncthreads();
-  if (tid == 0) {
-    int run = 0;
-    for (int k = 0; k < 2 * nst; ++k) { const int v = r_pref[k]; r_pref[k] = run; run += v; }
-    r_pref[2 * nst] = run;
-    s_tot = run;
-  }
-  __syncthreads();
-  const int total = s_tot, nr2 = 2 * nst;
+  const int nr2 = 2 * nst;
+  const int total = block_scan_excl<256>(r_pref, nr2, s_wsum);
   for (int h0 = hb; h0 < he; h0 += kBuildHome) {
     const int h1 = min(he, h0 + kBuildHome);
     for (int k = tid; k < h1 - h0; k += blockDim.x) {
@@ -404,17 +419,22 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
       xprev[h] = xq[h];
       cnt_out[h] = min(nl, cap);
       if (nl > cap) atomicMax(&counters->overflow_long, nl);
-      atomicMax(&counters->max_long, nl);
+      atomicMax(&s_max[0], nl);
       if (kShort) {
         for (int el = 0; el < ntype; ++el) {
           const int ns = h_ns[k][el];
           cnt_s_out[(size_t)el * stride + h] = min(ns, cap_s);
           if (ns > cap_s) atomicMax(&counters->overflow_short, ns);
-          atomicMax(&counters->max_short, ns);
+          atomicMax(&s_max[1], ns);
         }
       }
     }
     __syncthreads();
+  }
+  // one device-scope atomic per workgroup (a per-atom atomic on one address serialises)
+  if (tid == 0) {
+    atomicMax(&counters->max_long, s_max[0]);
+    if (kShort) atomicMax(&counters->max_short, s_max[1]);
   }
 }
 

@@ -40,8 +40,8 @@ constexpr int kTblG = 16;      // lanes per atom, tabulated form
 constexpr int kSfG = 16;       // lanes per atom, symmetry functions
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
-  int i = min(a, b), j = max(a, b);
-  return i * ntype - (i * (i - 1)) / 2 + (j - i);
+  const int i = min(a, b), j = max(a, b);  // small non-negative values: 24-bit multiplies
+  return __mul24(i, ntype) - (__mul24(i, i - 1) >> 1) + (j - i);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -462,20 +462,40 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
         rad += (ri[k] + rj[k]) * g;
         if (EV) radi += ri[k] * g;
       }
-      // own adjoint, direction +h
-      float al_i = 0, gix = 0, giy = 0, giz = 0;
-      sf_grad_ang(as, wv, dwv, ir, hx, hy, hz, al_i, gix, giy, giz);
-      // neighbour's adjoint, direction -h
-      float al_j = 0, gjx = 0, gjy = 0, gjz = 0;
-      sf_grad_ang(aj, wv, dwv, ir, -hx, -hy, -hz, al_j, gjx, gjy, gjz);
-      // F_i += dE_i/du - dE_j/du' with u' = -u; radial pieces are along h for i and -h for j
-      const float gxi = gix + (al_i)*hx, gyi = giy + (al_i)*hy, gzi = giz + (al_i)*hz;
-      const float gxj = gjx - al_j * hx, gyj = gjy - al_j * hy, gzj = gjz - al_j * hz;
-      fx += gxi - gxj + rad * hx;
-      fy += gyi - gyj + rad * hy;
-      fz += gzi - gzj + rad * hz;
-      // W_i = -(dE_i/du . u), own descriptor only
-      if (EV) vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
+      if constexpr (EV) {
+        // own adjoint, direction +h
+        float al_i = 0, gix = 0, giy = 0, giz = 0;
+        sf_grad_ang(as, wv, dwv, ir, hx, hy, hz, al_i, gix, giy, giz);
+        // neighbour's adjoint, direction -h
+        float al_j = 0, gjx = 0, gjy = 0, gjz = 0;
+        sf_grad_ang(aj, wv, dwv, ir, -hx, -hy, -hz, al_j, gjx, gjy, gjz);
+        // F_i += dE_i/du - dE_j/du' with u' = -u; radial pieces are along h for i and -h for j
+        const float gxi = gix + (al_i)*hx, gyi = giy + (al_i)*hy, gzi = giz + (al_i)*hz;
+        const float gxj = gjx - al_j * hx, gyj = gjy - al_j * hy, gzj = gjz - al_j * hz;
+        fx += gxi - gxj + rad * hx;
+        fy += gyi - gyj + rad * hy;
+        fz += gzi - gzj + rad * hz;
+        // W_i = -(dE_i/du . u), own descriptor only
+        vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
+      } else {
+        // the angular term is linear in the adjoint and the neighbour's is taken at -h, where
+        // the moment components of odd degree in h (1..3) change sign: one evaluation on
+        // own + neighbour for the even components and own - neighbour for the odd ones gives
+        // dE_i/du - dE_j/du' of the two separate evaluations
+        float e[kNEta * kNMom];
+#pragma unroll
+        for (int m = 0; m < kNEta; ++m)
+#pragma unroll
+          for (int c = 0; c < kNMom; ++c) {
+            const int k = m * kNMom + c;
+            e[k] = (c >= 1 && c <= 3) ? as[k] - aj[k] : as[k] + aj[k];
+          }
+        float al = 0, gx = 0, gy = 0, gz = 0;
+        sf_grad_ang(e, wv, dwv, ir, hx, hy, hz, al, gx, gy, gz);
+        fx += gx + (al + rad) * hx;
+        fy += gy + (al + rad) * hy;
+        fz += gz + (al + rad) * hz;
+      }
     }
     }
     }
