@@ -66,7 +66,7 @@ def time_steps(e, steps, warmup):
 
 def cpu_baseline(sysm, par, norm):
     """The oracle's float32 restatement of the same step on the host (one thread), on a bounded
-    sample: 2 steps of the 24k-atom workload."""
+    sample: 5 steps of the 24k-atom workload (about 10 s)."""
     from nnpengine_amd import model as M
     from nnpengine_amd import system as S
     from oracle import pyoracle
@@ -79,7 +79,7 @@ def cpu_baseline(sysm, par, norm):
     o.mlp_set(WIDTHS, par, 0.5, *norm)
     o.gr_init(450, R_GR, RC_PAIR)
     o.set_dt(DT)
-    nsteps = 2
+    nsteps = 5
     t_reb = t_gr = t_step = 0.0
     for s in range(nsteps):
         t0 = time.perf_counter()
@@ -175,6 +175,23 @@ def serving(par, norm, nreq, steps, seed=2027):
                           "max": float(lat.max())},
             "atom_steps_per_s": float(work.sum() / span), "peak_resident_boxes": int(peak),
             "relayouts": int(st["relayouts"]), "chunks": int(st["chunks"]), "arena_bytes": int(st["arena_bytes"])}
+
+
+def hbm_traffic(stage):
+    """HBM bytes per launch of the stage's kernel from the committed FETCH_SIZE/WRITE_SIZE summary
+    (two rocprofv3 --pmc runs of this workload, scaled by a 64 MiB copy kernel's readings; the
+    file states both commands and the scale factors), or None when it lists no such stage."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_hbm_traffic_water24k_nnp_step.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[1] == stage:
+            try:
+                return float(cells[5])
+            except ValueError:
+                return None
+    return None
 
 
 def force_errors(sysm, par, norm):
@@ -317,7 +334,7 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": alg_bytes[dom] / (stages[dom] * 1e-3) / 1e9,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": alg_bytes[dom]}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = hbm_traffic(dom)
     roof["step_share"] = {k: round(v / sum(share.values()), 4) for k, v in share.items()}
     mlp_roof = {"achieved_tflops_bf16": flop_atom * N / (stages["mlp"] * 1e-3) / 1e12,
                 "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
