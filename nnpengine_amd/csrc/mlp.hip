@@ -164,6 +164,19 @@ __device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf
 
 constexpr int kMlpWaves = 4;
 
+// tanh: the FP32 form calls tanhf; the 16-bit forms store the activation rounded to 8 or 11
+// significant bits, so they take 1 - 2/(exp(2x) + 1) on the hardware exp2 and reciprocal
+// (absolute error about 1e-7, saturating to +-1 for large |x|)
+template <int KIND>
+__device__ __forceinline__ float act_tanh(float x) {
+  if constexpr (KIND == 0) {
+    return tanhf(x);
+  } else {
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // exp(2x) = 2^(2x / ln 2)
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+  }
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
                                            float* __restrict__ e_atom, const int* __restrict__ type_idx) {
@@ -229,15 +242,15 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
     const float* bias = P.b[e][l];
     const float un = 1.0f / P.s_w[l];
     for (int cb = wave; cb < P.H[l] / 32; cb += kMlpWaves) {
+      const int col = cb * 32 + (lane & 31);
+      const float bc = bias[col];  // issued before the MFMA chain
       f32x16 acc;
       if constexpr (KIND == 1)
         acc = gemm_block_pairs(src, nullptr, lds_src, wt, reinterpret_cast<const T*>(P.Wt_lo[e][l]), K, cb, lane);
       else
         acc = gemm_block<KIND>(src, lds_src, wt, K, cb, lane);
-      const int col = cb * 32 + (lane & 31);
-      const float bc = bias[col];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) dst[crow(lane, j) * ldd + col] = Prec<KIND>::cvt(tanhf(acc[j] * un + bc));
+      for (int j = 0; j < 16; ++j) dst[crow(lane, j) * ldd + col] = Prec<KIND>::cvt(act_tanh<KIND>(acc[j] * un + bc));
     }
     __syncthreads();
   }
@@ -428,11 +441,22 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
   const int pad = precision == 0 ? 4 : 8;  // LDS row padding (elements)
   d.ld[0] = d.Dp + pad;
   for (int l = 0; l < d.nh; ++l) d.ld[l + 1] = d.H[l] + pad;
-  int off = 0;
-  for (int s = 0; s <= d.nh; ++s) { d.slab_off[s] = off; off += 32 * d.ld[s]; }
+  // slabs: descriptors (layer-0 input), then one per hidden layer (activations, overwritten in
+  // place by the gradients).  BF16 form: the gradients' low parts of layer s are live only
+  // while layer s-1's gradient is formed, so they alternate between two buffers, the even
+  // layers' in the descriptor slab's place (dead after the first forward layer)
   for (int s = 0; s <= kMaxLayers; ++s) d.lo_off[s] = 0;
-  if (precision == 1)
-    for (int s = 1; s <= d.nh; ++s) { d.lo_off[s] = off; off += 32 * d.ld[s]; }
+  int ldmax = 0;
+  for (int s = 1; s <= d.nh; ++s) ldmax = std::max(ldmax, d.ld[s]);
+  int off = 32 * d.ld[0];
+  if (precision == 1) off = std::max(off, 32 * ldmax);
+  d.slab_off[0] = 0;
+  for (int s = 1; s <= d.nh; ++s) { d.slab_off[s] = off; off += 32 * d.ld[s]; }
+  if (precision == 1) {
+    const int odd = off;
+    off += 32 * ldmax;
+    for (int s = 1; s <= d.nh; ++s) d.lo_off[s] = (s % 2) ? odd : 0;
+  }
   S->lds_bytes = size_t(off) * esz;
   if (S->lds_bytes > 64 * 1024) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "MLP tile does not fit the 64 KiB LDS budget");
   // weights: per element, per layer W [in][out] then b [out]
