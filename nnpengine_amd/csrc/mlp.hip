@@ -102,11 +102,36 @@ template <> struct Prec<2> {  // FP16
 // C/D fragment of a 32x32 MFMA: lane l, element j -> row 8*(j/4) + 4*(l/32) + j%4, col l%32
 __device__ __forceinline__ int crow(int lane, int j) { return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3); }
 
-// out[32 x 32 block cb] = A[32 x K] (LDS, row stride lda) * B, B fragment rows from global
-// `bm` laid out [n][K] (k contiguous).  A/B fragments: lane l holds row/col l%32 and the
-// KS/2 consecutive k starting at ks*KS + (KS/2)*(l/32).  The B fragments of NB consecutive
-// MFMAs are loaded into registers before the MFMA chain, so a chain waits for one round trip
-// to L2 instead of one per MFMA.
+// four consecutive activations in LDS (8-byte aligned for 16-bit types, 16-byte for fp32)
+template <class T>
+__device__ __forceinline__ void store4(T* p, const T* v) {
+  if constexpr (sizeof(T) == 2) {
+    ushort4 u;
+    __builtin_memcpy(&u, v, 8);
+    *reinterpret_cast<ushort4*>(p) = u;
+  } else {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  }
+}
+template <class T>
+__device__ __forceinline__ void load4(const T* p, T* v) {
+  if constexpr (sizeof(T) == 2) {
+    const ushort4 u = *reinterpret_cast<const ushort4*>(p);
+    __builtin_memcpy(v, &u, 8);
+  } else {
+    const float4 f = *reinterpret_cast<const float4*>(p);
+    v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+  }
+}
+
+// Transposed output block: out^T[32 features of block cb x 32 rows] = W[cb rows of bm] * A^T,
+// the weights `bm` ([n][K], k contiguous, from global) as the MFMA A operand and the
+// activations A ([32 rows][K] in LDS, row stride lda) as the B operand.  Both fragments: lane l
+// holds row/column l%32 and the KS/2 consecutive k starting at ks*KS + (KS/2)*(l/32).  Lane l
+// of the result holds features crow(l, j) of activation row l%32, four consecutive features
+// per group of j, so the epilogues store 4 values at once.  The weight fragments of NB
+// consecutive MFMAs are loaded into registers before the MFMA chain, so a chain waits for one
+// round trip to L2 instead of one per MFMA.
 template <int KIND>
 __device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, int lda, const typename Prec<KIND>::T* bm,
                                              int K, int cb, int lane) {
@@ -125,7 +150,7 @@ __device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, in
       if (kb + s * KS < K) b[s] = P::ld(brow + kb + s * KS);
 #pragma unroll
     for (int s = 0; s < NB; ++s)
-      if (kb + s * KS < K) acc = P::mma(P::ld(arow + kb + s * KS), b[s], acc);
+      if (kb + s * KS < K) acc = P::mma(b[s], P::ld(arow + kb + s * KS), acc);
   }
   return acc;
 }
@@ -154,9 +179,9 @@ __device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf
     for (int s = 0; s < NB; ++s)
       if (kb + s * KS < K) {
         const F ah = P::ld(Ahi + ao + kb + s * KS);
-        acc = P::mma(ah, bh[s], acc);
-        if (Blo) acc = P::mma(ah, bl[s], acc);
-        if (Alo) acc = P::mma(P::ld(Alo + ao + kb + s * KS), bh[s], acc);
+        acc = P::mma(bh[s], ah, acc);
+        if (Blo) acc = P::mma(bl[s], ah, acc);
+        if (Alo) acc = P::mma(bh[s], P::ld(Alo + ao + kb + s * KS), acc);
       }
   }
   return acc;
@@ -242,15 +267,23 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
     const float* bias = P.b[e][l];
     const float un = 1.0f / P.s_w[l];
     for (int cb = wave; cb < P.H[l] / 32; cb += kMlpWaves) {
-      const int col = cb * 32 + (lane & 31);
-      const float bc = bias[col];  // issued before the MFMA chain
+      const int row = lane & 31, f0 = cb * 32 + 4 * (lane >> 5);  // features f0 + 8q + t
+      float4 bq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bq[q] = *reinterpret_cast<const float4*>(bias + f0 + 8 * q);  // before the chain
       f32x16 acc;
       if constexpr (KIND == 1)
         acc = gemm_block_pairs(src, nullptr, lds_src, wt, reinterpret_cast<const T*>(P.Wt_lo[e][l]), K, cb, lane);
       else
         acc = gemm_block<KIND>(src, lds_src, wt, K, cb, lane);
 #pragma unroll
-      for (int j = 0; j < 16; ++j) dst[crow(lane, j) * ldd + col] = Prec<KIND>::cvt(act_tanh<KIND>(acc[j] * un + bc));
+      for (int q = 0; q < 4; ++q) {
+        const float bv[4] = {bq[q].x, bq[q].y, bq[q].z, bq[q].w};
+        T o[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) o[t] = Prec<KIND>::cvt(act_tanh<KIND>(acc[4 * q + t] * un + bv[t]));
+        store4(dst + row * ldd + f0 + 8 * q, o);
+      }
     }
     __syncthreads();
   }
@@ -299,15 +332,21 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
                                lane);
       else
         acc = gemm_block<KIND>(dl, ldl2, w, P.H[l], cb, lane);
-      const int col = cb * 32 + (lane & 31);
+      const int row = lane & 31, f0 = cb * 32 + 4 * (lane >> 5);
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        T* p = prev + crow(lane, j) * ldp + col;
-        const float a = Prec<KIND>::back(*p);
-        const float v = acc[j] * un * (1.0f - a * a) * sd;
-        const T hi = Prec<KIND>::cvt(v);
-        *p = hi;
-        if constexpr (KIND == 1) lds[P.lo_off[l] + crow(lane, j) * ldp + col] = Prec<KIND>::cvt(v - Prec<KIND>::back(hi));
+      for (int q = 0; q < 4; ++q) {
+        T* p = prev + row * ldp + f0 + 8 * q;
+        T a4[4], hi[4], lo[4];
+        load4(p, a4);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float a = Prec<KIND>::back(a4[t]);
+          const float v = acc[4 * q + t] * un * (1.0f - a * a) * sd;
+          hi[t] = Prec<KIND>::cvt(v);
+          if constexpr (KIND == 1) lo[t] = Prec<KIND>::cvt(v - Prec<KIND>::back(hi[t]));
+        }
+        store4(p, hi);
+        if constexpr (KIND == 1) store4(lds + P.lo_off[l] + row * ldp + f0 + 8 * q, lo);
       }
     }
     __syncthreads();
@@ -324,13 +363,24 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
                                lane);
       else
         acc = gemm_block<KIND>(d1, P.ld[1], w, P.H[0], cb, lane);
-      const int col = cb * 32 + (lane & 31);
-      if (col < D) {
-        const float sc = scv[col] * un;
+      const int row = lane & 31, f0 = cb * 32 + 4 * (lane >> 5);
+      if (row < nrows) {
+        float* out = dG + (size_t)srow[row] * D;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const int r = crow(lane, j);
-          if (r < nrows) dG[(size_t)srow[r] * D + col] = acc[j] * sc;
+        for (int q = 0; q < 4; ++q) {
+          const int f = f0 + 8 * q;
+          if ((D & 3) == 0) {
+            if (f < D) {  // D a multiple of 4: the 4 features are all in or all out
+              const float4 sc = *reinterpret_cast<const float4*>(scv + f);
+              *reinterpret_cast<float4*>(out + f) =
+                  make_float4(acc[4 * q] * sc.x * un, acc[4 * q + 1] * sc.y * un, acc[4 * q + 2] * sc.z * un,
+                              acc[4 * q + 3] * sc.w * un);
+            }
+          } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+              if (f + t < D) out[f + t] = acc[4 * q + t] * (scv[f + t] * un);
+          }
         }
       }
     }
