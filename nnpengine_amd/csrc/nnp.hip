@@ -36,6 +36,7 @@ constexpr int kSlice = 64;
 constexpr int kSliceRad = kNEta * kNMom;
 constexpr float kPi = 3.14159265358979f;
 constexpr int kSfU = 4;  // list entries per lane in flight at once (descriptor and force passes)
+constexpr int kSfBuf = 128;  // in-range pairs staged per atom between the descriptor pass's scan and sums
 constexpr int kTblG = 16;      // lanes per atom, tabulated form
 constexpr int kSfG = 16;       // lanes per atom, symmetry functions
 
@@ -156,8 +157,10 @@ __global__ void __launch_bounds__(256) k_tbl_force(int n, int cap, const uint32_
 
 struct SfParams {
   float rc, rcsq, eta_r, rc_inv;
+  float eta_r_l2;         // -eta_r / ln 2: exp(-eta_r d^2) = 2^(eta_r_l2 d^2)
   float mu[kNRad];
   float eta[kNEta];
+  float eta_l2[kNEta];    // -eta / ln 2
 };
 
 // fc = (cos(pi r/rc) + 1)/2 and dfc/dr for 0 <= r <= rc.  With y = pi (r/rc - 1/2), |y| <= pi/2:
@@ -193,6 +196,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   constexpr int RAD0 = NT * kNEta * kNMom;
   constexpr int D = NT * kNRad + NP * kNEta * 4;
   __shared__ float s_mom[256 / G][NMT];
+  __shared__ float4 s_pair[256 / G][kSfBuf];
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
   const bool live = i < n;
@@ -219,50 +223,66 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
       const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (threadIdx.x & 63 & ~(G - 1));
       const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1ull;
       int na = 0;
-      // kSfU entries per lane and their position gathers are in flight together; lanes still
-      // accumulate their entries in list order and the ballots run in list order
-      for (int q0 = 0; q0 < cn; q0 += kSfU * G) {
-        uint32_t ec[kSfU];
-        float4 xr[kSfU];
+      float4* buf = s_pair[grp];
+      // Phase 1: all lanes scan the list (kSfU entries per lane and their position gathers in
+      // flight together), and a group-masked ballot compacts the pairs within the cutoff, in
+      // list order, into the active list and into the group's LDS buffer as (u, |u|^2).
+      // Phase 2: all lanes accumulate the buffered pairs, so the exp-heavy moment arithmetic
+      // runs on full lanes instead of under the scan's partial mask.
+      for (int q0 = 0; q0 < cn;) {
+        int nb = 0;
+        for (; q0 < cn && nb + kSfU * G <= kSfBuf; q0 += kSfU * G) {
+          uint32_t ec[kSfU];
+          float4 xr[kSfU];
 #pragma unroll
-        for (int u = 0; u < kSfU; ++u) {
-          const int q = q0 + u * G + lane;
-          ec[u] = q < cn ? row[q] : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < kSfU; ++u) {
-          const int q = q0 + u * G + lane;
-          xr[u] = xq[q < cn ? (int)NNPE_ENTRY_J(ec[u]) : i];
-        }
-#pragma unroll
-        for (int u = 0; u < kSfU; ++u) {
-          if (q0 + u * G >= cn) break;  // uniform across the group
-          const int q = q0 + u * G + lane;
-          const uint32_t ent = ec[u];
-          float ux = 0, uy = 0, uz = 0, rsq = 0;
-          bool in = false;
-          if (q < cn) {
-            const float4 xj = image_pos(xr[u], NNPE_ENTRY_CODE(ent), bl, m_bl);
-            ux = xj.x - xi.x; uy = xj.y - xi.y; uz = xj.z - xi.z;
-            rsq = dist2_rn(ux, uy, uz);  // rounded as written: the force pass makes the same cut
-            in = rsq < P.rcsq;
+          for (int u = 0; u < kSfU; ++u) {
+            const int q = q0 + u * G + lane;
+            ec[u] = q < cn ? row[q] : 0u;
           }
-          const unsigned long long bal = __ballot(in) & gmask;
-          if (in) arow[na + __popcll(bal & below)] = ent;
-          na += __popcll(bal);
-          if (!in) continue;
+#pragma unroll
+          for (int u = 0; u < kSfU; ++u) {
+            const int q = q0 + u * G + lane;
+            xr[u] = xq[q < cn ? (int)NNPE_ENTRY_J(ec[u]) : i];
+          }
+#pragma unroll
+          for (int u = 0; u < kSfU; ++u) {
+            if (q0 + u * G >= cn) break;  // uniform across the group
+            const int q = q0 + u * G + lane;
+            float ux = 0, uy = 0, uz = 0, rsq = 0;
+            bool in = false;
+            if (q < cn) {
+              const float4 xj = image_pos(xr[u], NNPE_ENTRY_CODE(ec[u]), bl, m_bl);
+              ux = xj.x - xi.x; uy = xj.y - xi.y; uz = xj.z - xi.z;
+              rsq = dist2_rn(ux, uy, uz);  // rounded as written: the force pass makes the same cut
+              in = rsq < P.rcsq;
+            }
+            const unsigned long long bal = __ballot(in) & gmask;
+            if (in) {
+              const int slot = __popcll(bal & below);
+              arow[na + slot] = ec[u];
+              buf[nb + slot] = make_float4(ux, uy, uz, rsq);
+            }
+            const int c = __popcll(bal);
+            na += c;
+            nb += c;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();  // a group lives in one wave: its LDS writes are ordered
+        for (int k = lane; k < nb; k += G) {
+          const float4 pu = buf[k];
+          const float rsq = pu.w;
           const float ir = __builtin_amdgcn_rsqf(rsq), r = rsq * ir;
-          const float hx = ux * ir, hy = uy * ir, hz = uz * ir;
+          const float hx = pu.x * ir, hy = pu.y * ir, hz = pu.z * ir;
           float fc, dfc;
           sf_cut(r, P.rc_inv, fc, dfc);
 #pragma unroll
-          for (int k = 0; k < kNRad; ++k) {
-            const float d = r - P.mu[k];
-            R[k] += __expf(-P.eta_r * d * d) * fc;
+          for (int k2 = 0; k2 < kNRad; ++k2) {
+            const float d = r - P.mu[k2];
+            R[k2] += __builtin_amdgcn_exp2f(P.eta_r_l2 * d * d) * fc;
           }
 #pragma unroll
           for (int m = 0; m < kNEta; ++m) {
-            const float w = __expf(-P.eta[m] * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
+            const float w = __builtin_amdgcn_exp2f(P.eta_l2[m] * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
             A[m][0] += w;
             A[m][1] += wx; A[m][2] += wy; A[m][3] += wz;
             A[m][4] += wx * hx; A[m][5] += wy * hy; A[m][6] += wz * hz;
@@ -270,6 +290,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
             A[m][10] += w * w;
           }
         }
+        __builtin_amdgcn_wave_barrier();  // buffer consumed before the next scan overwrites it
       }
       if (lane == 0) act_cnt[(size_t)a * stride + i] = na;
       group_sum_n<G, kNEta * kNMom>(&A[0][0]);
@@ -445,7 +466,7 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
       float wv[kNEta], dwv[kNEta];
 #pragma unroll
       for (int m = 0; m < kNEta; ++m) {
-        const float ex = __expf(-P.eta[m] * rsq);
+        const float ex = __builtin_amdgcn_exp2f(P.eta_l2[m] * rsq);
         wv[m] = ex * fc;
         dwv[m] = ex * (dfc - 2.0f * P.eta[m] * r * fc);
       }
@@ -458,7 +479,7 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
 #pragma unroll
       for (int k = 0; k < kNRad; ++k) {
         const float d = r - P.mu[k];
-        const float g = __expf(-P.eta_r * d * d) * (dfc - 2.0f * P.eta_r * d * fc);
+        const float g = __builtin_amdgcn_exp2f(P.eta_r_l2 * d * d) * (dfc - 2.0f * P.eta_r * d * fc);
         rad += (ri[k] + rj[k]) * g;
         if (EV) radi += ri[k] * g;
       }
@@ -683,8 +704,10 @@ extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, f
   if (ctx->ntype < 1 || ctx->ntype > 4) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "symmetry functions support 1-4 elements");
   NnpState& S = nnp_state(ctx);
   S.P.rc = rc; S.P.rcsq = rc * rc; S.P.eta_r = eta_r; S.P.rc_inv = 1.0f / rc;
+  const double l2e = 1.4426950408889634;  // 1 / ln 2
+  S.P.eta_r_l2 = float(-double(eta_r) * l2e);
   for (int k = 0; k < kNRad; ++k) S.P.mu[k] = mu[k];
-  for (int k = 0; k < kNEta; ++k) S.P.eta[k] = eta[k];
+  for (int k = 0; k < kNEta; ++k) { S.P.eta[k] = eta[k]; S.P.eta_l2[k] = float(-double(eta[k]) * l2e); }
   S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
   NNPE_CHECK_HIP(ctx, dev_alloc(S.M, size_t(ctx->natom) * S.nmt));
   NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->natom) * ctx->ntype * kSlice));
