@@ -81,7 +81,8 @@ __global__ void __launch_bounds__(256) k_gr_inner(int n, int cap, const uint32_t
 }
 
 // Full tally by home cell: persistent workgroups take the cells round-robin; for each home cell,
-// the candidates of its stencil cells (real rows, then ghost images, per stencil cell) are
+// the candidates of its stencil cells (real rows of the cells at or after it, then ghost
+// images, per stencil cell) are
 // staged in LDS in batches and each wave tests 64 of them per step against one home atom,
 // adding 2 for a real j with a larger row (each unordered real pair once) and 1 for a ghost
 // image, into the workgroup's LDS histogram (flushed once per workgroup).
@@ -125,7 +126,9 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
       int nr = 0, ng = 0, br = 0, bg = 0;
       if (ax >= 0 && ay >= 0 && az >= 0 && ax < num_gx && ay < num_gx && az < num_gx) {
         const int nc = B.cell_base + (ax * num_gx + ay) * num_gx + az;
-        br = cstart[nc]; nr = cstart[nc + 1] - br;
+        // real rows are sorted by cell, so a real partner with a larger row lies in this cell
+        // or a later one: the earlier cells' real rows are not staged at all
+        if (nc >= c) { br = cstart[nc]; nr = cstart[nc + 1] - br; }
         bg = gstart[nc]; ng = gstart[nc + 1] - bg;
       }
       r_begin[2 * s] = br; r_pref[2 * s] = nr; r_ghost[2 * s] = 0;
