@@ -144,13 +144,16 @@ __device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, in
   const typename P::T* arow = A + (lane & 31) * lda + HALF * (lane >> 5);
   const typename P::T* brow = bm + (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
   for (int kb = 0; kb < K; kb += NB * KS) {
-    F b[NB];
+    F b[NB], a[NB];
 #pragma unroll
     for (int s = 0; s < NB; ++s)
       if (kb + s * KS < K) b[s] = P::ld(brow + kb + s * KS);
 #pragma unroll
     for (int s = 0; s < NB; ++s)
-      if (kb + s * KS < K) acc = P::mma(b[s], P::ld(arow + kb + s * KS), acc);
+      if (kb + s * KS < K) a[s] = P::ld(arow + kb + s * KS);  // LDS reads issued together
+#pragma unroll
+    for (int s = 0; s < NB; ++s)
+      if (kb + s * KS < K) acc = P::mma(b[s], a[s], acc);
   }
   return acc;
 }
@@ -168,21 +171,30 @@ __device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf
   const size_t ao = (size_t)(lane & 31) * lda + HALF * (lane >> 5);
   const size_t bo = (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
   for (int kb = 0; kb < K; kb += NB * KS) {
-    F bh[NB], bl[NB];
+    F bh[NB], bl[NB], ah[NB], al[NB];
 #pragma unroll
     for (int s = 0; s < NB; ++s)
       if (kb + s * KS < K) {
         bh[s] = P::ld(Bhi + bo + kb + s * KS);
         if (Blo) bl[s] = P::ld(Blo + bo + kb + s * KS);
       }
+    // LDS reads of the A fragments issued together, half a batch at a time (register budget)
 #pragma unroll
-    for (int s = 0; s < NB; ++s)
-      if (kb + s * KS < K) {
-        const F ah = P::ld(Ahi + ao + kb + s * KS);
-        acc = P::mma(bh[s], ah, acc);
-        if (Blo) acc = P::mma(bl[s], ah, acc);
-        if (Alo) acc = P::mma(bh[s], P::ld(Alo + ao + kb + s * KS), acc);
-      }
+    for (int h = 0; h < NB; h += NB / 2) {
+#pragma unroll
+      for (int s = h; s < h + NB / 2; ++s)
+        if (kb + s * KS < K) {
+          ah[s] = P::ld(Ahi + ao + kb + s * KS);
+          if (Alo) al[s] = P::ld(Alo + ao + kb + s * KS);
+        }
+#pragma unroll
+      for (int s = h; s < h + NB / 2; ++s)
+        if (kb + s * KS < K) {
+          acc = P::mma(bh[s], ah[s], acc);
+          if (Blo) acc = P::mma(bl[s], ah[s], acc);
+          if (Alo) acc = P::mma(bh[s], al[s], acc);
+        }
+    }
   }
   return acc;
 }
@@ -203,7 +215,8 @@ __device__ __forceinline__ float act_tanh(float x) {
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
+// three waves per SIMD: the LDS of three tiles fits a CU, so registers are capped to match
+__global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(3))) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
                                            float* __restrict__ e_atom, const int* __restrict__ type_idx) {
   typedef typename Prec<KIND>::T T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -236,11 +249,13 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp(MlpDev P, const float* _
     for (int idx = tid; idx < 32 * Dp4; idx += NT) {
       const int r = idx / Dp4, c4 = idx - r * Dp4;
       const int row = srow[r];
-      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (row >= 0 && c4 < D4) {
-        const float4 g = G4[(size_t)row * D4 + c4], sh = sh4[c4], sc = sc4[c4];
-        v = make_float4((g.x - sh.x) * sc.x, (g.y - sh.y) * sc.y, (g.z - sh.z) * sc.z, (g.w - sh.w) * sc.w);
-      }
+      // loads at clamped in-bounds addresses, padding selected afterwards: no branch around
+      // the loads, so the unrolled iterations' loads are all in flight together
+      const bool ok = row >= 0 && c4 < D4;
+      const int rr = row >= 0 ? row : srow[0], cc = min(c4, D4 - 1);
+      const float4 g = G4[(size_t)rr * D4 + cc], sh = sh4[cc], sc = sc4[cc];
+      float4 v = make_float4((g.x - sh.x) * sc.x, (g.y - sh.y) * sc.y, (g.z - sh.z) * sc.z, (g.w - sh.w) * sc.w);
+      if (!ok) v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       T* d = Gs + r * ldg + 4 * c4;
       d[0] = Prec<KIND>::cvt(v.x); d[1] = Prec<KIND>::cvt(v.y); d[2] = Prec<KIND>::cvt(v.z); d[3] = Prec<KIND>::cvt(v.w);
     }
