@@ -177,6 +177,27 @@ def serving(par, norm, nreq, steps, seed=2027):
             "relayouts": int(st["relayouts"]), "chunks": int(st["chunks"]), "arena_bytes": int(st["arena_bytes"])}
 
 
+def sqd_timing(e, q_max=20.0, nq=1000, reps=3):
+    """End-of-run analysis: direct k-space S(q) of the current 24000-atom positions over every
+    reciprocal vector of the box with |G| <= 20 1/A (half space), 1000 bins."""
+    t0 = time.perf_counter()
+    ngv = e.sqd_init(nq, q_max)
+    t_init = time.perf_counter() - t0
+    e.sqd_compute()
+    e.sync()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        e.sqd_compute()
+        e.sync()
+        ts.append(time.perf_counter() - t0)
+    t = min(ts)
+    _, sq, cnt = e.sqd_get()
+    log(f"[bench] sqd: {ngv} vectors, {t * 1e3:.1f} ms per compute, init {t_init:.2f} s")
+    return {"q_max": q_max, "nq": nq, "n_gvec": int(ngv), "ms": t * 1e3, "init_s": t_init,
+            "atom_gvec_per_s": e.natom * ngv / t, "finite": bool(np.all(np.isfinite(sq[cnt > 0])))}
+
+
 def hbm_traffic(stage):
     """HBM bytes per launch of the stage's kernel from the committed FETCH_SIZE/WRITE_SIZE summary
     (two rocprofv3 --pmc runs of this workload, scaled by a 64 MiB copy kernel's readings; the
@@ -255,6 +276,7 @@ def main():
     ap.add_argument("--stage-reps", type=int, default=20, help="launches per stage in the stage profile")
     ap.add_argument("--serve-requests", type=int, default=256, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=500)
+    ap.add_argument("--no-sqd", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is "
@@ -340,6 +362,8 @@ def main():
                 "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
                 "flop_per_atom": flop_atom}
 
+    sqd = None if args.no_sqd else sqd_timing(eb)
+
     for e_ in engines.values():
         e_.close()
     serve = None
@@ -377,6 +401,7 @@ def main():
         "stage_ms": stages,
         "mlp": mlp_roof,
         "roofline": roof,
+        "sqd": sqd,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

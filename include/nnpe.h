@@ -121,6 +121,19 @@ int nnpe_gr_get(nnpe_ctx* ctx, long long* counts, double* h);
 int nnpe_sq_init(nnpe_ctx* ctx, int nq, double q_min, double q_max);
 int nnpe_sq_compute(nnpe_ctx* ctx, double* out);
 
+/* SQD ctor (src/grsq.cpp:362-491): the half space of reciprocal vectors G = 2 pi (h, k, l) / L
+ * with |h|,|k|,|l| <= int(q_max / g) and h^2 + k^2 + l^2 <= int((q_max / g)^2), g = 2 pi / L,
+ * binned by |G| into nq bins of width q_max / nq (the reference's float arithmetic decides the
+ * limits and the bins).  *n_gvec receives the number of vectors.  One box, at most 4 elements. */
+int nnpe_sqd_init(nnpe_ctx* ctx, int nq, float q_max, long long* n_gvec);
+/* SQD::compute (src/grsq.cpp:571-606) on the current positions: per bin the average over its
+ * vectors of S_a S_b + C_a C_b (S_a, C_a the sin / cos sums over the atoms of element a),
+ * times natom / (N_a N_b), minus natom / N_a for a == b.  Exact integer phases, FP64 sums. */
+int nnpe_sqd_compute(nnpe_ctx* ctx);
+/* Results of the last nnpe_sqd_compute: qq [nq] mean |G| of each bin, sq [nq][npair], count [nq]
+ * vectors per bin (SQD::dump, src/grsq.cpp:608-644, skips the empty bins); any may be NULL. */
+int nnpe_sqd_get(nnpe_ctx* ctx, double* qq, double* sq, long long* count);
+
 /* RMDF ctor (src/grsq.cpp:899-1060): refine the pair tables against nsq measured S(q) sets.
  * data [nsq][nq][3 + ntype] rows (q, S_exp(q), weight, form factor per element) on the S(q)
  * grid of nnpe_sq_init, which must have q_min = 0; the refined table's bins must be the inner

@@ -50,6 +50,9 @@ SIGNATURES = [
     ("nnpe_gr_get", _I, [_P, _PLL, _PD]),
     ("nnpe_sq_init", _I, [_P, _I, _D, _D]),
     ("nnpe_sq_compute", _I, [_P, _PD]),
+    ("nnpe_sqd_init", _I, [_P, _I, _F, _PLL]),
+    ("nnpe_sqd_compute", _I, [_P]),
+    ("nnpe_sqd_get", _I, [_P, _PD, _PD, _PLL]),
     ("nnpe_rmdf_init", _I, [_P, _I, _PF, _F, _F, _I]),
     ("nnpe_rmdf_set_delta", _I, [_P, _PF]),
     ("nnpe_rmdf_step", _I, [_P, _I]),

@@ -68,6 +68,7 @@ static void free_all(nnpe_ctx* c) {
   dev_free(c->gr_count); dev_free(c->gr_h); dev_free(c->gr_norm); dev_free(c->sq_out); dev_free(c->sq_ftmx);
   md_state_free(c);
   rmdf_state_free(c);
+  sqd_state_free(c);
   nnp_state_free(c);
   mlp_state_free(c);
   dev_free(c->G); dev_free(c->dG); dev_free(c->e_atom); dev_free(c->type_idx); dev_free(c->type_fill);

@@ -53,6 +53,7 @@ struct BoxPar {
 struct NnpState;  // descriptor tables and moment buffers (nnp.hip)
 struct MlpState;  // per-element network weights (mlp.hip)
 struct RmdfState; // S(q) refinement of the pair tables (rmdf.hip)
+struct SqdState;  // direct k-space S(q) (sqd.hip)
 
 struct nnpe_ctx {
   int device = 0;
@@ -147,6 +148,7 @@ struct nnpe_ctx {
   NnpState* nnp = nullptr;
   MlpState* mlp = nullptr;
   RmdfState* rmdf = nullptr;
+  SqdState* sqd = nullptr;
   float* G = nullptr;
   float* dG = nullptr;
   int D_cur = 0, D_cap = 0;
@@ -193,6 +195,9 @@ int type_index_enqueue(nnpe_ctx* ctx);
 int nnp_tbl_tables(nnpe_ctx* ctx, float** ftab, float** utab, int* nbin, float* dr);
 // symmetry-function active lists sized to the short lists' capacity
 int nnp_act_reserve(nnpe_ctx* ctx);
+
+// direct k-space S(q) (sqd.hip)
+void sqd_state_free(nnpe_ctx* ctx);
 
 // RMDF (rmdf.hip)
 void rmdf_state_free(nnpe_ctx* ctx);

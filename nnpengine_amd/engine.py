@@ -175,6 +175,24 @@ class Engine:
         self._check(self.L.nnpe_sq_compute(self.h, _p(out, C.c_double)))
         return out
 
+    def sqd_init(self, nq, q_max):
+        """SQD (direct k-space S(q)) vector set and bins; returns the number of vectors."""
+        n = C.c_longlong(0)
+        self.sqd_nq = nq
+        self._check(self.L.nnpe_sqd_init(self.h, nq, C.c_float(q_max), C.byref(n)))
+        return n.value
+
+    def sqd_compute(self):
+        self._check(self.L.nnpe_sqd_compute(self.h))
+
+    def sqd_get(self):
+        """qq [nq] (mean |G| per bin), sq [nq][npair], count [nq] of the last sqd_compute."""
+        qq = np.zeros(self.sqd_nq)
+        sq = np.zeros((self.sqd_nq, self.npair))
+        cnt = np.zeros(self.sqd_nq, np.int64)
+        self._check(self.L.nnpe_sqd_get(self.h, _p(qq, C.c_double), _p(sq, C.c_double), _p(cnt, C.c_longlong)))
+        return qq, sq, cnt
+
     # RMDF ----------------------------------------------------------------------------------
     def rmdf_init(self, data, strength, gamma, target=0):
         """data [nsq][nq][3 + ntype] (q, S_exp, weight, form factors) on the S(q) grid."""

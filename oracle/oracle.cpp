@@ -959,6 +959,98 @@ struct Eng {
 }  // namespace
 
 // ============================================================================================
+// SQD: direct k-space S(q) (src/grsq.cpp:362-606).
+//  * The vector set, |G| and the bins follow the reference's float arithmetic (numtype = float):
+//    g = two_pi / bl, kcutoff = int(q_max / g), cuti2 = int(sqr(q_max / g)) (:370-376);
+//    the three loops over the half space (:378-423); gscl = sqrt(float(n^2)) * g; the vectors
+//    sorted by gscl (:424, any order among equal values); vbegin/vend from gx = i * dq,
+//    dq = q_max / nq (:436-455).  qq_ref is the reference's float mean in sorted order
+//    (:440-455); qq is the FP64 mean of the same float gscl values.
+//  * SQD::compute (:571-606) in FP64: G = 2 pi (h, k, l) / L, kr = G . x, S_a / C_a = sums of
+//    sin / cos over the atoms of element a; per bin avg(S_a S_b + C_a C_b) * natom / (N_a N_b)
+//    - [a == b] natom / N_a (:476-489).  Empty bins get 0 (the reference divides by zero there
+//    and SQD::dump skips them, :629).
+void sqd_restate(int natom, int ntype, float bl, const float* x, const int* typ, int nq, float q_max, double* qq,
+                 double* qq_ref, double* sq, long long* count) {
+  const float pi_f = 3.14159265358979323846f, two_pi_f = pi_f * 2;
+  const float g = two_pi_f / bl;
+  const int kc = (int)(q_max / g);
+  const int cuti2 = (int)sqr(q_max / g);
+  struct V { int h, k, l; float s; };
+  std::vector<V> vs;
+  for (int i = -kc; i <= kc; ++i)
+    for (int j = -kc; j <= kc; ++j)
+      for (int k = 1; k <= kc; ++k) {
+        const int n2 = i * i + j * j + k * k;
+        if (n2 > cuti2) continue;
+        vs.push_back({i, j, k, std::sqrt((float)n2) * g});
+      }
+  for (int i = -kc; i <= kc; ++i)
+    for (int j = 1; j <= kc; ++j) {
+      const int n2 = i * i + j * j;
+      if (n2 > cuti2) continue;
+      vs.push_back({i, j, 0, std::sqrt((float)n2) * g});
+    }
+  for (int i = 1; i <= kc; ++i) {
+    if (i * i > cuti2) continue;
+    vs.push_back({i, 0, 0, std::sqrt((float)(i * i)) * g});
+  }
+  std::stable_sort(vs.begin(), vs.end(), [](const V& a, const V& b) { return a.s < b.s; });
+  const int ngv = (int)vs.size();
+  const float dq = q_max / nq;
+  std::vector<int> vb(nq), ve(nq);
+  int i2 = 0;
+  vb[0] = 0;
+  for (int i = 1; i < nq; ++i) {
+    const float gx = i * dq;
+    float gy = 0;
+    const int b0 = i2;
+    while (i2 < ngv && vs[i2].s < gx) { gy += vs[i2].s; ++i2; }
+    vb[i] = ve[i - 1] = i2;
+    qq_ref[i - 1] = ve[i - 1] > b0 ? double(gy / (ve[i - 1] - b0)) : 0.0;
+  }
+  {
+    float gy = 0;
+    const int b0 = i2;
+    while (i2 < ngv) { gy += vs[i2].s; ++i2; }
+    ve[nq - 1] = ngv;
+    qq_ref[nq - 1] = ngv > b0 ? double(gy / (ngv - b0)) : 0.0;
+  }
+  std::vector<int> tc(ntype, 0);
+  for (int a = 0; a < natom; ++a) ++tc[typ[a]];
+  const int npair = ntype * (ntype + 1) / 2;
+  const double two_pi_L = 2 * 3.14159265358979323846 / double(bl);
+  std::vector<double> ss(ntype), cc(ntype);
+  for (int b = 0; b < nq; ++b) {
+    count[b] = ve[b] - vb[b];
+    double qs = 0;
+    std::vector<double> acc(npair, 0.0);
+    for (int v = vb[b]; v < ve[b]; ++v) {
+      qs += vs[v].s;
+      const double qx = two_pi_L * vs[v].h, qy = two_pi_L * vs[v].k, qz = two_pi_L * vs[v].l;
+      std::fill(ss.begin(), ss.end(), 0.0);
+      std::fill(cc.begin(), cc.end(), 0.0);
+      for (int a = 0; a < natom; ++a) {
+        const double kr = qx * double(x[3 * a]) + qy * double(x[3 * a + 1]) + qz * double(x[3 * a + 2]);
+        ss[typ[a]] += std::sin(kr);
+        cc[typ[a]] += std::cos(kr);
+      }
+      int p = 0;
+      for (int t = 0; t < ntype; ++t)
+        for (int u = t; u < ntype; ++u, ++p) acc[p] += ss[t] * ss[u] + cc[t] * cc[u];
+    }
+    qq[b] = count[b] ? qs / double(count[b]) : 0.0;
+    int p = 0;
+    for (int t = 0; t < ntype; ++t)
+      for (int u = t; u < ntype; ++u, ++p) {
+        const double factor = double(natom) / (double(tc[t]) * tc[u]);
+        const double base = t == u ? double(natom) / tc[u] : 0.0;
+        sq[size_t(b) * npair + p] = count[b] ? acc[p] / double(count[b]) * factor - base : 0.0;
+      }
+  }
+}
+
+// ============================================================================================
 // C ABI (ctypes).  Handles are Eng<float> (prec 0) or Eng<double> (prec 1).
 extern "C" {
 
@@ -1216,6 +1308,14 @@ int orc_sf_force(OrcHandle* h, const double* dG, int ev, double* vir) {
     e->sf_force(ev != 0, vir);
   };
   if (h->prec) run(static_cast<Eng<double>*>(h->e)); else run(static_cast<Eng<float>*>(h->e));
+  return 0;
+}
+
+// SQD on positions x [natom][3] (any atom order) and elements typ [natom]; qq, qq_ref [nq],
+// sq [nq][npair], count [nq]
+int orc_sqd(int natom, int ntype, float bl, const float* x, const int* typ, int nq, float q_max, double* qq,
+            double* qq_ref, double* sq, long long* count) {
+  sqd_restate(natom, ntype, bl, x, typ, nq, q_max, qq, qq_ref, sq, count);
   return 0;
 }
 

@@ -48,7 +48,7 @@ def lib():
                    "orc_compute_ke", "orc_gr_init", "orc_gr_tally", "orc_gr_get", "orc_sq",
                    "orc_nnp_tbl_init", "orc_nnp_tbl_tables", "orc_nnp_tbl_descriptor",
                    "orc_nnp_tbl_set_dedg", "orc_nnp_tbl_force", "orc_mlp_set", "orc_mlp_eval",
-                   "orc_sf_set", "orc_sf_descriptor", "orc_sf_force"):
+                   "orc_sf_set", "orc_sf_descriptor", "orc_sf_force", "orc_sqd"):
             getattr(L, fn)
         _lib = L
     return _lib
@@ -288,6 +288,21 @@ def sq_transform(h, dr, rho, nq, q_min, q_max):
     lib().orc_sq_transform(len(h), C.c_double(dr), C.c_double(rho), _p(h, C.c_double), nq, C.c_double(q_min),
                            C.c_double(q_max), _p(out, C.c_double))
     return out
+
+
+def sqd(x, typ, ntype, bl, nq, q_max):
+    """SQD (direct k-space S(q)) of positions x [natom][3] and elements typ: returns qq (FP64 mean
+    |G| per bin), qq_ref (the reference's float mean), sq [nq][npair] (FP64) and count [nq]."""
+    x = _f32(x).reshape(-1, 3)
+    t = _i32(typ)
+    npair = ntype * (ntype + 1) // 2
+    qq = np.zeros(nq)
+    qq_ref = np.zeros(nq)
+    sq = np.zeros((nq, npair))
+    cnt = np.zeros(nq, np.int64)
+    lib().orc_sqd(len(t), ntype, C.c_float(bl), _p(x, C.c_float), _p(t, C.c_int), nq, C.c_float(q_max),
+                  _p(qq, C.c_double), _p(qq_ref, C.c_double), _p(sq, C.c_double), _p(cnt, C.c_longlong))
+    return qq, qq_ref, sq, cnt
 
 
 def canonical(entries):
