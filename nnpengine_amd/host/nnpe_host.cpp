@@ -1,9 +1,11 @@
 // nnpe_host.cpp -- host classes over the nnpe C ABI (see nnpe_host.h).
 #include "nnpe_host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <iostream>
 #include <random>
 
 namespace nnpe_host {
@@ -274,6 +276,8 @@ template void LJTableCoulDSF::compute<true, false>(NEIGHLIST*, numtype*, numtype
 template void LJTableCoulDSF::compute<false, false>(NEIGHLIST*, numtype*, numtype*);
 
 void LJTableCoulDSF::dumpForce(const char* fname, bool, bool append) {
+  // the refined table as it is now on the device (RMDF rewrites it in place, src/grsq.cpp:1246-1300)
+  if (refined) check(nnpe_rmdf_tables(nnpe_host::ctx(), force.data(), potential.data()), "LJTableCoulDSF::dumpForce");
   FILE* fp = std::fopen(fname, append ? "a+" : "w");
   if (!fp) NNPE_END_PROGRAM("write error");
   for (int i = 0; i < nbin_r; ++i) {
@@ -493,3 +497,158 @@ SQ::SQ(int nq, numtype qmax, GR* ggr) : nbin_q(nq), q_max(qmax), gr(ggr) {
 }
 void SQ::compute(int) { check(nnpe_sq_compute(nnpe_host::ctx(), sq.data()), "SQ::compute"); }
 void SQ::dump(const char* fname, bool append) { dump_table(fname, append, nbin_q, ENVIRON::npair, sq, double(q_max) / nbin_q); }
+
+// ---- SQD (src/grsq.cpp:362-644) ----------------------------------------------------------------------
+SQD::SQD(int nq, numtype qmax) : q_max(qmax), nbin_q(nq) {
+  check(nnpe_sqd_init(nnpe_host::ctx(), nq, qmax, &ngv), "SQD");
+  std::cerr << "SQDIRECT::SQDIRECT " << ngv << " G vecs\n";
+  qq.assign(nq, 0);
+  sq.assign(size_t(nq) * ENVIRON::npair, 0);
+  count.assign(nq, 0);
+}
+void SQD::compute(int) {
+  check(nnpe_sqd_compute(nnpe_host::ctx()), "SQD::compute");
+  check(nnpe_sqd_get(nnpe_host::ctx(), qq.data(), sq.data(), count.data()), "SQD::compute");
+}
+// running average over appended runs (# n header, at most 100); empty bins are not written
+void SQD::dump(const char* fname, bool append) {
+  int num = 1;
+  FILE* fp;
+  if (append && (fp = std::fopen(fname, "r"))) {
+    if (std::fscanf(fp, "# %d ", &num) != 1) num = 0;
+    num = std::min(num + 1, 100);  // MAXACC
+    const double factor = 1.0 / num, cofactor = 1 - factor;
+    std::cerr << "sq_acc = " << num << "\n";
+    for (int i = 0; i < nbin_q; ++i) {
+      if (!count[i]) continue;
+      float v;
+      if (std::fscanf(fp, "%f ", &v) != 1) break;
+      if (std::fabs(v - qq[i]) > 1e-5f) std::cerr << "warning: large diff: " << v << " " << qq[i] << "\n";
+      for (int p = 0; p < ENVIRON::npair; ++p) {
+        if (std::fscanf(fp, "%f ", &v) != 1) break;
+        double& s = sq[size_t(i) * ENVIRON::npair + p];
+        s = s * factor + cofactor * v;
+      }
+    }
+    std::fclose(fp);
+  }
+  fp = std::fopen(fname, "w");
+  if (!fp) NNPE_END_PROGRAM("write error");
+  std::fprintf(fp, "# %d\n", num);
+  for (int i = 0; i < nbin_q; ++i) {
+    if (!count[i]) continue;
+    std::fprintf(fp, "%.15e", qq[i]);
+    for (int p = 0; p < ENVIRON::npair; ++p) std::fprintf(fp, " %.15e", sq[size_t(i) * ENVIRON::npair + p]);
+    std::fprintf(fp, "\n");
+  }
+  std::fclose(fp);
+}
+
+// ---- RMDF (src/grsq.cpp:899-1443) --------------------------------------------------------------------
+RMDF::RMDF(int nexp, char** fexp, SQ* c_sq, numtype strength, numtype _gamma, LJTableCoulDSF* pair, NNPOTENTIAL_TBL*)
+    : n_sq(nexp), nbin_q(c_sq->nbin_q), gamma(_gamma), q_max(c_sq->q_max) {
+  if (gamma == 1) std::cerr << "DA/RMD; ";
+  else if (gamma > 0 && gamma < 1) std::cerr << "FMIRL training; ";
+  else if (gamma == 0) std::cerr << "FMIRL inference; ";
+  else NNPE_END_PROGRAM("invalid gamma");
+  std::cerr << "strength = " << strength << "\n";
+  const int nt = ENVIRON::ntype, row = 3 + nt;
+  std::vector<float> data(size_t(n_sq) * nbin_q * row, 0.0f);
+  const double dq = double(q_max) / nbin_q;
+  for (int n = 0; n < n_sq; ++n) {
+    std::cerr << "reading " << fexp[n] << "\n";
+    FILE* fp = std::fopen(fexp[n], "r");
+    if (!fp) NNPE_END_PROGRAM("read error");
+    float* d = data.data() + size_t(n) * nbin_q * row;
+    for (int i = 0; i < nbin_q; ++i)
+      for (int c = 0; c < row; ++c)
+        if (std::fscanf(fp, "%f ", d + size_t(i) * row + c) != 1) NNPE_END_PROGRAM("read error");
+    for (int i = 0; i < nbin_q; ++i)
+      if (std::fabs(d[size_t(i) * row] - (i + 0.5) * dq) > 1e-5)
+        std::cerr << "warning: large diff: " << d[size_t(i) * row] << " " << (i + 0.5) * dq << "\n";
+    std::fclose(fp);
+    std::cerr << "last entry: " << d[size_t(nbin_q - 1) * row] << " " << d[size_t(nbin_q - 1) * row + 1] << " "
+              << d[size_t(nbin_q - 1) * row + 2] * strength << "\n";
+  }
+  sq.assign(size_t(n_sq) * nbin_q, 0.0f);
+  deltasq.assign(sq.size(), 0.0f);
+  deltasq_accu.assign(sq.size(), 0.0f);
+  if (n_sq > 0) {
+    check(nnpe_rmdf_init(nnpe_host::ctx(), n_sq, data.data(), strength, gamma, pair ? 0 : 1), "RMDF");
+    if (pair) pair->refined = true;
+  }
+}
+// the relaxed residual to resume from: column 2 (gamma != 0) or column 3 (FMIRL inference) of a
+// dumpSQ file; a missing file is not an error
+void RMDF::loadDeltaSQ(const char* fname) {
+  std::cerr << "reading " << fname << "\n";
+  FILE* fp = std::fopen(fname, "r");
+  if (!fp) {
+    std::cerr << "cannot open file\n";
+    return;
+  }
+  if (std::fscanf(fp, "# %*d ") != 0) NNPE_END_PROGRAM("read error");
+  const double dq = double(q_max) / nbin_q;
+  float qi = 0;
+  for (int i = 0; i < nbin_q; ++i) {
+    if (std::fscanf(fp, "%f ", &qi) != 1) NNPE_END_PROGRAM("read error");
+    if (std::fabs(qi - (i + 0.5) * dq) > 1e-5) std::cerr << "warning: large diff: " << qi << " " << (i + 0.5) * dq << "\n";
+    for (int n = 0; n < n_sq; ++n) {
+      float a, b, c;
+      if (std::fscanf(fp, "%f %f %f ", &a, &b, &c) != 3) NNPE_END_PROGRAM("read error");
+      deltasq[size_t(n) * nbin_q + i] = gamma == 0 ? c : b;
+    }
+  }
+  std::fclose(fp);
+  if (n_sq > 0) {
+    std::cerr << "last entry: " << qi << " " << deltasq.back() << "\n";
+    check(nnpe_rmdf_set_delta(nnpe_host::ctx(), deltasq.data()), "RMDF::loadDeltaSQ");
+  }
+}
+numtype RMDF::step(bool potential) {
+  if (n_sq == 0) return 0;
+  check(nnpe_rmdf_step(nnpe_host::ctx(), potential ? 1 : 0), "RMDF");
+  double e[2 * NNPE_NTERM];
+  check(nnpe_energy_get(nnpe_host::ctx(), e), "RMDF");
+  return numtype(e[2 * NNPE_TERM_RMDF]);
+}
+// format: q, then per data set S_n(q) (running average), deltasq, deltasq running average
+void RMDF::dumpSQ(const char* fname, bool append) {
+  if (n_sq > 0) check(nnpe_rmdf_get(nnpe_host::ctx(), sq.data(), deltasq.data(), nullptr), "RMDF::dumpSQ");
+  int num = 1;
+  double factor = 1, cofactor = 0;
+  FILE* fp;
+  const double dq = double(q_max) / nbin_q;
+  if (append && (fp = std::fopen(fname, "r"))) {
+    if (std::fscanf(fp, "# %d ", &num) != 1) num = 0;
+    num = std::min(num + 1, 100);  // MAXACC
+    cofactor = 1 - (factor = 1.0 / num);
+    std::cerr << "rmdf_acc = " << num << "\n";
+    for (int i = 0; i < nbin_q; ++i) {
+      float v;
+      if (std::fscanf(fp, "%f ", &v) != 1) break;
+      if (std::fabs(v - (i + 0.5) * dq) > 1e-5) std::cerr << "warning: large diff: " << v << " " << (i + 0.5) * dq << "\n";
+      for (int n = 0; n < n_sq; ++n) {
+        float a, b, c;
+        if (std::fscanf(fp, "%f %f %f ", &a, &b, &c) != 3) break;
+        float& s = sq[size_t(n) * nbin_q + i];
+        s = float(s * factor + cofactor * a);
+        deltasq_accu[size_t(n) * nbin_q + i] = c;
+      }
+    }
+    std::fclose(fp);
+  }
+  fp = std::fopen(fname, "w");
+  if (!fp) NNPE_END_PROGRAM("write error");
+  std::fprintf(fp, "# %d\n", num);
+  for (int i = 0; i < nbin_q; ++i) {
+    std::fprintf(fp, "%.15e", (i + 0.5) * dq);
+    for (int n = 0; n < n_sq; ++n) {
+      const size_t k = size_t(n) * nbin_q + i;
+      std::fprintf(fp, " %.15e %.15e %.15e", double(sq[k]), double(deltasq[k]),
+                   deltasq[k] * factor + cofactor * deltasq_accu[k]);
+    }
+    std::fprintf(fp, "\n");
+  }
+  std::fclose(fp);
+}

@@ -74,6 +74,7 @@ class LJTableCoulDSF {
   int nbin_r;
   numtype cutoff, dr;
   std::vector<numtype> force, potential;  // [npair][nbin+1]
+  bool refined = false;                   // an RMDF refines this table on the device
   LJTableCoulDSF(numtype cf, int nbin);
   void loadPotential(int kind, const char* potfile = nullptr);  // src/pair.cpp:197-263
   template <bool ev, bool nsp> void compute(NEIGHLIST* list, numtype* erg, numtype* virial);
@@ -161,4 +162,34 @@ class SQ {
   SQ(int nq, numtype qmax, GR* ggr);
   void compute(int thread_id);
   void dump(const char* fname, bool append = false);  // src/grsq.cpp:324-359
+};
+
+// direct k-space S(q) (src/grsq.h:40-63): the half space of reciprocal vectors up to q_max,
+// binned by |G|; sq [nq][npair]
+class SQD {
+ public:
+  numtype q_max;
+  int nbin_q;
+  long long ngv = 0;
+  std::vector<double> qq, sq;
+  std::vector<long long> count;
+  SQD(int nq, numtype qmax);
+  void compute(int thread_id);
+  void dump(const char* fname, bool append = false);  // src/grsq.cpp:608-644
+};
+
+// S(q) refinement of the pair tables (src/grsq.h:65-112).  The experimental data files hold
+// nq rows (q, S_exp, weight, one form factor per element) on the SQ grid; pair != nullptr
+// refines the LJTableCoulDSF table, else the NNPOTENTIAL_TBL pair tables.
+class RMDF {
+ public:
+  int n_sq, nbin_q;
+  numtype gamma, q_max;
+  std::vector<float> sq, deltasq, deltasq_accu;  // [n_sq][nq], host copies for dumpSQ
+  RMDF(int nexp, char** fexp, SQ* c_sq, numtype strength, numtype gamma, LJTableCoulDSF* pair, NNPOTENTIAL_TBL* pair_nnp);
+  void loadDeltaSQ(const char* fname);                   // src/grsq.cpp:1419-1443
+  // compute_Qspace + compute_Rspace[_NNP] (+ compute_Potential[_NNP] when potential) on the
+  // current S(q) (src/grsq.cpp:1079-1377); returns the Q-space energy
+  numtype step(bool potential);
+  void dumpSQ(const char* fname, bool append = false);  // src/grsq.cpp:1378-1415
 };

@@ -6,8 +6,6 @@
 // One owner thread: the reference's omp barriers/singles/criticals disappear; every class method
 // is one stream-ordered call into libnnpe.so.  Differences from the reference (all explicit):
 //  * the time step comes from the environment (NNPE_DT, default 1 fs as the reference);
-//  * RMDF refinement is not available yet: the RMDF input lines are read and must ask for no
-//    experimental data (0 files);
 //  * nnp_run evaluates our per-element MLP (network file = first token of the NNP param file)
 //    instead of a TensorFlow SavedModel;
 //  * NNPE_SEED fixes the Andersen thermostat's random stream (the reference seeds from
@@ -42,32 +40,53 @@ void log_header() {
             << "erg [kcal/mol]" << std::setw(18) << "dif [kcal/mol]" << std::setw(10) << "nbuild" << "\n";
 }
 
-void log_row(int md_step, numtype ekin, numtype virial, numtype erg, unsigned nbuild) {
+void log_row(int md_step, numtype ekin, numtype virial, numtype erg, numtype erg_exp, unsigned nbuild) {
   const double ek_T = 335.47968899917714, ev_GPa = 6.9476954570553735;
   const numtype temperature = ekin / ENVIRON::natom * ek_T;
   const numtype press = (virial + ekin * 2) / 3 * ev_GPa / std::pow(ENVIRON::bl, 3);
   std::cout << std::setw(7) << md_step << std::setw(18) << temperature << std::setw(18) << press << std::setw(18) << erg
-            << std::setw(18) << 0.0 << std::setw(10) << nbuild << std::endl;
+            << std::setw(18) << erg_exp << std::setw(10) << nbuild << std::endl;
 }
 
-void read_rmdf(Input& in) {
-  float strength, gamma;
-  int nexp;
-  if (std::sscanf(in.line(), "%f %f", &strength, &gamma) != 2) NNPE_END_PROGRAM("read error");
-  if (std::sscanf(in.line(), "%d", &nexp) != 1) NNPE_END_PROGRAM("read error");
-  if (nexp > 2) NNPE_END_PROGRAM("max n_sq is 2");
-  if (nexp != 0) NNPE_END_PROGRAM("RMDF refinement is not available in this build (use 0 experimental files)");
-  in.line();  // rmdf initial weights file (loadDeltaSQ)
-}
+// the RMDF lines of the input (src/run.cpp:155-170): "strength gamma", "n_sq file...", the
+// initial weights file (RMDF::loadDeltaSQ)
+struct RmdfInput {
+  float strength = 0, gamma = 1;
+  int nexp = 0;
+  char files[2][256];
+  char* fptr[2] = {files[0], files[1]};
+  char delta[256];
+  void read(Input& in) {
+    if (std::sscanf(in.line(), "%f %f", &strength, &gamma) != 2) NNPE_END_PROGRAM("read error");
+    const char* buf = in.line();
+    int used = 0, k = 0;
+    if (std::sscanf(buf, "%d %n", &nexp, &used) != 1) NNPE_END_PROGRAM("read error");
+    if (nexp > 2) NNPE_END_PROGRAM("max n_sq is 2");
+    for (int i = 0; i < nexp; ++i) {
+      if (std::sscanf(buf + used, "%255s %n", files[i], &k) != 1) NNPE_END_PROGRAM("read error");
+      used += k;
+    }
+    if (std::sscanf(in.line(), "%255s", delta) != 1) NNPE_END_PROGRAM("read error");
+  }
+};
 
-void dumps(Input& in, GR& gr, SQ& sq, LJTableCoulDSF* pair) {
+void dumps(Input& in, GR& gr, SQ& sq, RMDF& rmdf, LJTableCoulDSF* pair) {
   char name[256];
-  int app;
+  int app, nq;
+  float qmax;
   if (std::sscanf(in.line(), "%255s %d", name, &app) == 2 && *name != '*') ENVIRON::dump(name, app);
   if (std::sscanf(in.line(), "%255s %d", name, &app) == 2 && *name != '*') gr.dump(name, app);
   if (std::sscanf(in.line(), "%255s %d", name, &app) == 2 && *name != '*') sq.dump(name, app);
-  in.line();  // sqd: direct k-space S(q) is not on the MD path
-  in.line();  // sqx: RMDF S(q)
+  // sqd: direct k-space S(q) of the final positions (src/run.cpp:421-432)
+  if (std::sscanf(in.line(), "%255s %d %f %d", name, &app, &qmax, &nq) != 4) NNPE_END_PROGRAM("read error");
+  if (*name != '*') {
+    std::cerr << "sqd = " << name << "; append = " << bool(app) << "\n";
+    SQD sqd(nq, qmax);
+    sqd.compute(0);
+    sqd.dump(name, app);
+  }
+  // sqx: the RMDF S(q) and residuals (src/run.cpp:434-440)
+  if (std::sscanf(in.line(), "%255s %d", name, &app) == 2 && *name != '*') rmdf.dumpSQ(name, app);
   if (pair && std::sscanf(in.line(), "%255s %d", name, &app) == 2 && *name != '*') pair->dumpForce(name, false, app);
 }
 
@@ -130,14 +149,17 @@ int classical(const char* fin) {
   if (std::sscanf(in.line(), "%d %d %f", &nbin_gr, &nq, &qmax) != 3) NNPE_END_PROGRAM("read error");
   GR gr(nbin_gr, gr_cutoff, pair_cutoff);
   SQ sq(nq, qmax, &gr);
-  read_rmdf(in);
+  RmdfInput ri;
+  ri.read(in);
+  RMDF rmdf(ri.nexp, ri.fptr, &sq, ri.strength, ri.gamma, &pair, nullptr);
+  rmdf.loadDeltaSQ(ri.delta);
   Run R;
   R.read(in);
   ENVIRON::initV(R.temp_start);
   VERLET verlet(0);
   VERLET::setDt(env_dt());
   log_header();
-  numtype erg = 0, virial = 0, ekin = 0;
+  numtype erg = 0, virial = 0, ekin = 0, erg_exp = 0;
   for (int md_step = 0; md_step <= R.total_steps; ++md_step) {
     const bool f_log = !(md_step % R.log_interval);
     if (md_step > 0) verlet.updateX();
@@ -147,6 +169,7 @@ int classical(const char* fin) {
       else gr.tally(nullptr, nullptr);
       gr.reduce_local(0);
       sq.compute(0);
+      erg_exp = rmdf.step(f_log && ri.gamma != 1);  // compute_Qspace, compute_Rspace (+ compute_Potential)
     }
     erg = virial = 0;
     if (f_log) {
@@ -161,9 +184,9 @@ int classical(const char* fin) {
       ekin = verlet.computeKE();
       ENVIRON::initV(R.temp_start + (R.temp_end - R.temp_start) * md_step / R.total_steps, 0.1);  // THERMO_ANDERSEN_FRAC
     }
-    if (f_log) log_row(md_step, ekin, virial, erg, list.nbuild);
+    if (f_log) log_row(md_step, ekin, virial, erg, erg_exp, list.nbuild);
   }
-  dumps(in, gr, sq, &pair);
+  dumps(in, gr, sq, rmdf, &pair);
   return 0;
 }
 
@@ -199,14 +222,17 @@ int nnp_run(const char* fin) {
   if (std::sscanf(in.line(), "%d %d %f", &nbin_gr, &nq, &qmax) != 3) NNPE_END_PROGRAM("read error");
   GR gr(nbin_gr, gr_cutoff, pair_cutoff);
   SQ sq(nq, qmax, &gr);
-  read_rmdf(in);
+  RmdfInput ri;
+  ri.read(in);
+  RMDF rmdf(ri.nexp, ri.fptr, &sq, ri.strength, ri.gamma, nullptr, &nnp);
+  rmdf.loadDeltaSQ(ri.delta);
   Run R;
   R.read(in);
   ENVIRON::initV(R.temp_start);
   VERLET verlet(0);
   VERLET::setDt(env_dt());
   log_header();
-  numtype erg = 0, virial = 0, ekin = 0;
+  numtype erg = 0, virial = 0, ekin = 0, erg_exp = 0;
   for (int md_step = 0; md_step <= R.total_steps; ++md_step) {
     const bool f_log = !(md_step % R.log_interval);
     if (md_step > 0) verlet.updateX();
@@ -216,6 +242,7 @@ int nnp_run(const char* fin) {
       else gr.tally(nullptr, nullptr);
       gr.reduce_local(0);
       sq.compute(0);
+      erg_exp = rmdf.step(f_log && ri.gamma != 1);  // compute_Qspace, compute_Rspace_NNP (+ compute_Potential_NNP)
     }
     // forces: the NNP force kernel and the bonded terms add to a cleared force array
     erg = virial = 0;
@@ -235,9 +262,9 @@ int nnp_run(const char* fin) {
       ekin = verlet.computeKE();
       ENVIRON::initV(R.temp_start + (R.temp_end - R.temp_start) * md_step / R.total_steps, 0.1);
     }
-    if (f_log) log_row(md_step, ekin, virial, erg, list.nbuild);
+    if (f_log) log_row(md_step, ekin, virial, erg, erg_exp, list.nbuild);
   }
-  dumps(in, gr, sq, nullptr);
+  dumps(in, gr, sq, rmdf, nullptr);
   return 0;
 }
 

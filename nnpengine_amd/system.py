@@ -387,9 +387,11 @@ def write_config(path, sys: System, mol_files):
 
 
 def write_classical_input(d, sysm, pair_cutoff, skin, gr_cutoff, force_tbl, pot_tbl, nbin_gr, nq, q_max,
-                          temp=(300.0, 300.0), steps=(100, 10, 10, 10, 10), outputs=None):
+                          temp=(300.0, 300.0), steps=(100, 10, 10, 10, 10), outputs=None, rmdf=None, sqd=(10.0, 10)):
     """The positional input file of ``classical`` (src/run.cpp:25-50) plus the configuration,
-    molecule and table files it names, written into directory d.  Returns the input path."""
+    molecule and table files it names, written into directory d.  Returns the input path.
+    rmdf = (strength, gamma, [experimental S(q) files], initial-weights file) or None (no data
+    sets); sqd = (q_max, nq) of the end-of-run direct S(q) (written when outputs names 'sqd')."""
     import os
     mols = []
     for k, mt in enumerate(sysm.moltypes):
@@ -407,11 +409,12 @@ def write_classical_input(d, sysm, pair_cutoff, skin, gr_cutoff, force_tbl, pot_
     lines += [str(len(ap))] + [" ".join(f"{v:.9g}" for v in r) for r in ap]
     dp = sysm.dihedral_params or []
     lines += [str(len(dp))] + [" ".join(f"{v:.9g}" for v in r) for r in dp]
-    lines += [f"{nbin_gr} {nq} {q_max:.9g}", "0 1", "0", "*", f"{temp[0]:.9g} {temp[1]:.9g}",
-              " ".join(str(s) for s in steps)]
+    strength, gamma, files, delta = rmdf if rmdf else (0.0, 1.0, [], "*")
+    lines += [f"{nbin_gr} {nq} {q_max:.9g}", f"{strength:.9g} {gamma:.9g}", " ".join([str(len(files))] + list(files)),
+              delta, f"{temp[0]:.9g} {temp[1]:.9g}", " ".join(str(s) for s in steps)]
     o = outputs or {}
-    lines += [f"{o.get('xyz', '*')} 0", f"{o.get('gr', '*')} 0", f"{o.get('sq', '*')} 0", "* 0 10 10", "* 0",
-              f"{o.get('force', '*')} 0"]
+    lines += [f"{o.get('xyz', '*')} 0", f"{o.get('gr', '*')} 0", f"{o.get('sq', '*')} 0",
+              f"{o.get('sqd', '*')} 0 {sqd[0]:.9g} {sqd[1]}", f"{o.get('sqx', '*')} 0", f"{o.get('force', '*')} 0"]
     path = os.path.join(d, "in.classical")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
