@@ -18,7 +18,7 @@ namespace {
 // special-pair factors (src/molecule.h:11): class 0 full, classes 1 and 2 excluded, class 3 scaled
 constexpr float kSpecialLj3 = 0.5f, kSpecialCoul3 = float(5 / 6.);
 constexpr float kElectricK = 332.06371f;                                      // src/environ.h:77
-constexpr int kPairG = 16;  // lanes per atom
+constexpr int kPairG = 32;  // lanes per atom
 constexpr int kPairU = 4;   // list entries per lane in flight at once
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {

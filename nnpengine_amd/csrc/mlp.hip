@@ -42,6 +42,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 namespace {
 
 constexpr int kMaxLayers = 6;   // weight layers including the output layer
+constexpr int kMaxWidth = 512;  // widest hidden layer
 constexpr int kMaxType = 4;
 
 struct MlpDev {
@@ -124,14 +125,25 @@ __device__ __forceinline__ void load4(const T* p, T* v) {
   }
 }
 
+// Weight matrices W [n][K] (n a multiple of 32) are stored in MFMA fragment order: fragment
+// (cb, t) -- rows cb*32 .. cb*32+31, k = t*KS .. t*KS+KS-1 -- is 64 lanes x KS/2 consecutive
+// elements, lane l holding row cb*32 + l%32 and k = t*KS + (KS/2)*(l/32) + h.  One wave's
+// fragment load is then one contiguous 1 KiB (16-bit) or 256 B (fp32) run: 8 (or 2) whole
+// cache lines instead of 32 partial ones, which is what bounds a layer when every tile
+// re-streams the weights through its CU's L1.
+template <int KS, class T>
+__device__ __forceinline__ const T* frag_ptr(const T* W, int K, int cb, int t, int lane) {
+  return W + ((size_t)cb * (K / KS) + t) * (64 * (KS / 2)) + lane * (KS / 2);
+}
+
 // Transposed output block: out^T[32 features of block cb x 32 rows] = W[cb rows of bm] * A^T,
-// the weights `bm` ([n][K], k contiguous, from global) as the MFMA A operand and the
-// activations A ([32 rows][K] in LDS, row stride lda) as the B operand.  Both fragments: lane l
-// holds row/column l%32 and the KS/2 consecutive k starting at ks*KS + (KS/2)*(l/32).  Lane l
-// of the result holds features crow(l, j) of activation row l%32, four consecutive features
-// per group of j, so the epilogues store 4 values at once.  The weight fragments of NB
-// consecutive MFMAs are loaded into registers before the MFMA chain, so a chain waits for one
-// round trip to L2 instead of one per MFMA.
+// the weights `bm` (fragment order, from global) as the MFMA A operand and the activations A
+// ([32 rows][K] in LDS, row stride lda) as the B operand.  Both fragments: lane l holds
+// row/column l%32 and the KS/2 consecutive k starting at ks*KS + (KS/2)*(l/32).  Lane l of the
+// result holds features crow(l, j) of activation row l%32, four consecutive features per group
+// of j, so the epilogues store 4 values at once.  The weight fragments of NB consecutive MFMAs
+// are loaded into registers before the MFMA chain, so a chain waits for one round trip to L2
+// instead of one per MFMA.
 template <int KIND>
 __device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, int lda, const typename Prec<KIND>::T* bm,
                                              int K, int cb, int lane) {
@@ -142,12 +154,11 @@ __device__ __forceinline__ f32x16 gemm_block(const typename Prec<KIND>::T* A, in
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
   const typename P::T* arow = A + (lane & 31) * lda + HALF * (lane >> 5);
-  const typename P::T* brow = bm + (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
   for (int kb = 0; kb < K; kb += NB * KS) {
     F b[NB], a[NB];
 #pragma unroll
     for (int s = 0; s < NB; ++s)
-      if (kb + s * KS < K) b[s] = P::ld(brow + kb + s * KS);
+      if (kb + s * KS < K) b[s] = P::ld(frag_ptr<KS>(bm, K, cb, kb / KS + s, lane));
 #pragma unroll
     for (int s = 0; s < NB; ++s)
       if (kb + s * KS < K) a[s] = P::ld(arow + kb + s * KS);  // LDS reads issued together
@@ -169,14 +180,13 @@ __device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
   const size_t ao = (size_t)(lane & 31) * lda + HALF * (lane >> 5);
-  const size_t bo = (size_t)(cb * 32 + (lane & 31)) * K + HALF * (lane >> 5);
   for (int kb = 0; kb < K; kb += NB * KS) {
     F bh[NB], bl[NB], ah[NB], al[NB];
 #pragma unroll
     for (int s = 0; s < NB; ++s)
       if (kb + s * KS < K) {
-        bh[s] = P::ld(Bhi + bo + kb + s * KS);
-        if (Blo) bl[s] = P::ld(Blo + bo + kb + s * KS);
+        bh[s] = P::ld(frag_ptr<KS>(Bhi, K, cb, kb / KS + s, lane));
+        if (Blo) bl[s] = P::ld(frag_ptr<KS>(Blo, K, cb, kb / KS + s, lane));
       }
     // LDS reads of the A fragments issued together, half a batch at a time (register budget)
 #pragma unroll
@@ -236,7 +246,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
   // stage the descriptor rows standardised in fp32, zero padding rows/cols; row ids go through
   // LDS so every global load of the loop is independent of the others (they overlap)
   __shared__ int srow[32];
+  __shared__ float s_wout[kMaxWidth];  // output-layer weights of the element
   if (tid < 32) srow[tid] = tid < nrows ? rows[tid] : -1;
+  for (int c = tid; c < P.H[nh - 1]; c += NT) s_wout[c] = P.wout[e][c];
   __syncthreads();
   const float* shv = P.shift[e];
   const float* scv = P.scale[e];
@@ -311,7 +323,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
     const int HL = P.H[nh - 1];
     T* dL = lds + P.slab_off[nh];
     const int ldl = P.ld[nh];
-    const float* wo = P.wout[e];
+    const float* wo = s_wout;
     const float sc = P.out_scale * P.s_d[nh - 1];
     const int r = tid >> 3, sub = tid & 7;
     float s = 0.0f;
@@ -491,7 +503,8 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
   if (ctx->ntype > kMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "at most 4 elements");
   if (widths[nlayer] != 1) return ctx_fail(ctx, NNPE_ERR_ARG, "the output layer must have width 1");
   for (int l = 1; l < nlayer; ++l)
-    if (widths[l] % 32 || widths[l] <= 0) return ctx_fail(ctx, NNPE_ERR_ARG, "hidden widths must be multiples of 32");
+    if (widths[l] % 32 || widths[l] <= 0 || widths[l] > kMaxWidth)
+      return ctx_fail(ctx, NNPE_ERR_ARG, "hidden widths must be multiples of 32, at most 512");
   mlp_state_free(ctx);
   MlpState* S = ctx->mlp = new MlpState;
   S->kind = precision; S->nl = nlayer;
@@ -578,8 +591,18 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
           wn[size_t(i) * out + k] = W[size_t(i) * out + k] * sw;
         }
       // lo = false: the value rounded to the storage type; lo = true (BF16 form): the rounding
-      // remainder x - hi rounded to bf16
-      auto upload = [&](const std::vector<float>& h, const void*& dst, bool lo) -> int {
+      // remainder x - hi rounded to bf16.  h is [n][K] row-major; it is stored in MFMA fragment
+      // order (frag_ptr): fragment (cb, t) = 64 lanes x KS/2 elements, lane l holding row
+      // cb*32 + l%32, k = t*KS + (KS/2)*(l/32) + e
+      const int KS = precision == 0 ? 2 : 16, HALF = KS / 2;
+      auto upload = [&](const std::vector<float>& hrow, int n, int K, const void*& dst, bool lo) -> int {
+        std::vector<float> h(hrow.size());
+        for (int cb = 0; cb < n / 32; ++cb)
+          for (int t = 0; t < K / KS; ++t)
+            for (int ln = 0; ln < 64; ++ln)
+              for (int e2 = 0; e2 < HALF; ++e2)
+                h[((size_t(cb) * (K / KS) + t) * 64 + ln) * HALF + e2] =
+                    hrow[size_t(cb * 32 + ln % 32) * K + t * KS + HALF * (ln / 32) + e2];
         void* p = nullptr;
         NNPE_CHECK_HIP(ctx, hipMalloc(&p, h.size() * esz));
         S->bufs.push_back(p);
@@ -601,14 +624,14 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
         dst = p;
         return NNPE_OK;
       };
-      int rc = upload(wt, d.Wt[t][l], false);
+      int rc = upload(wt, out, inp, d.Wt[t][l], false);
       if (rc) return rc;
-      rc = upload(wn, d.W[t][l], false);
+      rc = upload(wn, inp, out, d.W[t][l], false);
       if (rc) return rc;
       if (precision == 1) {
-        rc = upload(wt, d.Wt_lo[t][l], true);
+        rc = upload(wt, out, inp, d.Wt_lo[t][l], true);
         if (rc) return rc;
-        rc = upload(wn, d.W_lo[t][l], true);
+        rc = upload(wn, inp, out, d.W_lo[t][l], true);
         if (rc) return rc;
       }
       float* db = nullptr;

@@ -20,6 +20,7 @@
 //     moments: every atom turns dE/dG into dE/d(moments) once, then each pair's force is a
 //     function of the pair vector and the two atoms' adjoints, gathered over the full list, so
 //     each atom's force has one writer.
+#include <algorithm>
 #include <cmath>
 
 #include "internal.h"
@@ -155,13 +156,45 @@ __global__ void __launch_bounds__(256) k_tbl_force(int n, int cap, const uint32_
 // ---------------------------------------------------------------------------------------------
 // (2) symmetry functions
 
+constexpr int kRadS = 8;  // radial recurrence: the centre evaluated directly
+
 struct SfParams {
   float rc, rcsq, eta_r, rc_inv;
   float eta_r_l2;         // -eta_r / ln 2: exp(-eta_r d^2) = 2^(eta_r_l2 d^2)
   float mu[kNRad];
   float eta[kNEta];
   float eta_l2[kNEta];    // -eta / ln 2
+  // equally spaced centres mu_k = mu_s + (k - s) delta (s = kRadS):
+  float rad_step_l2;      // 2 eta_r delta / ln 2
+  float rad_rho[kRadS];   // exp(-eta_r delta^2 (2j + 1)), j = 0 .. kRadS-1
 };
+
+// g[k] = exp(-eta_r (r - mu_k)^2), k = 0 .. kNRad-1.  RR (equally spaced centres): g at the
+// centre s and the step ratio R = exp(2 eta_r delta (r - mu_s)) from the hardware exp2, the rest
+// from the exact identities g_{s+j+1} = g_{s+j} R rho_j and g_{s-j-1} = g_{s-j} rho_j / R,
+// rho_j = exp(-eta_r delta^2 (2j + 1)): three exponentials instead of sixteen.  Walking outwards
+// from the middle keeps every intermediate within [0, R] (no overflow); a product that
+// underflows to 0 is followed only by smaller true values.
+template <bool RR>
+__device__ __forceinline__ void sf_gauss(float r, const SfParams& P, float* g) {
+#pragma clang fp contract(fast)
+  if constexpr (RR) {
+    const float e = r - P.mu[kRadS];
+    g[kRadS] = __builtin_amdgcn_exp2f(P.eta_r_l2 * e * e);
+    const float l = P.rad_step_l2 * e;
+    const float R = __builtin_amdgcn_exp2f(l), Ri = __builtin_amdgcn_exp2f(-l);
+#pragma unroll
+    for (int j = 0; kRadS + j + 1 < kNRad; ++j) g[kRadS + j + 1] = g[kRadS + j] * R * P.rad_rho[j];
+#pragma unroll
+    for (int j = 0; j < kRadS; ++j) g[kRadS - j - 1] = g[kRadS - j] * Ri * P.rad_rho[j];
+  } else {
+#pragma unroll
+    for (int k = 0; k < kNRad; ++k) {
+      const float d = r - P.mu[k];
+      g[k] = __builtin_amdgcn_exp2f(P.eta_r_l2 * d * d);
+    }
+  }
+}
 
 // fc = (cos(pi r/rc) + 1)/2 and dfc/dr for 0 <= r <= rc.  With y = pi (r/rc - 1/2), |y| <= pi/2:
 // cos(pi r/rc) = -sin(y), sin(pi r/rc) = cos(y), both from their Taylor series about 0 (sin to
@@ -183,7 +216,7 @@ __device__ __forceinline__ void sf_cut(float r, float rc_inv, float& fc, float& 
 // group butterfly-sums them, and the lanes split the stores (lane l writes the values whose
 // index is l mod G) to the moment rows and to an LDS copy, from which the angular descriptors
 // are formed at the end.
-template <int NT, int G>
+template <int NT, int G, bool RR>
 __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
                                                  const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                  const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
@@ -275,11 +308,10 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
           const float hx = pu.x * ir, hy = pu.y * ir, hz = pu.z * ir;
           float fc, dfc;
           sf_cut(r, P.rc_inv, fc, dfc);
+          float gk[kNRad];
+          sf_gauss<RR>(r, P, gk);
 #pragma unroll
-          for (int k2 = 0; k2 < kNRad; ++k2) {
-            const float d = r - P.mu[k2];
-            R[k2] += __builtin_amdgcn_exp2f(P.eta_r_l2 * d * d) * fc;
-          }
+          for (int k2 = 0; k2 < kNRad; ++k2) R[k2] += gk[k2] * fc;
 #pragma unroll
           for (int m = 0; m < kNEta; ++m) {
             const float w = __builtin_amdgcn_exp2f(P.eta_l2[m] * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
@@ -332,60 +364,52 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   }
 }
 
-// dE/d(moments) of each atom from dE/dG and its moments
+// dE/d(moments) of each atom from dE/dG and its moments.  Lane (atom i, element a, exponent m)
+// forms the 11 adjoints of moment group (a, m) -- summing the partner elements b in order, each
+// pair's terms in the order of the pair loop (x <= y) of the descriptor definition, the
+// self pair twice -- and copies four of element a's sixteen radial dE/dG into the slice.
 template <int NT>
 __global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restrict__ M, const float* __restrict__ dG,
                                                     float* __restrict__ adj) {
 #pragma clang fp contract(fast)
-  constexpr int NP = NT * (NT + 1) / 2;
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
-  constexpr int D = NT * kNRad + NP * kNEta * 4;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  constexpr int D = NT * kNRad + NT * (NT + 1) / 2 * kNEta * 4;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NT * kNEta) return;
+  const int m = t % kNEta, a = (t / kNEta) % NT, i = t / (kNEta * NT);
   const float* mi = M + (size_t)i * NMT;
   const float* g = dG + (size_t)i * D;
-  float* ad = adj + (size_t)i * NT * kSlice;
-  float dA[NT][kNEta][kNMom];
+  float dA[kNMom];
 #pragma unroll
-  for (int a = 0; a < NT; ++a)
+  for (int c = 0; c < kNMom; ++c) dA[c] = 0.0f;
 #pragma unroll
-    for (int m = 0; m < kNEta; ++m)
+  for (int b = 0; b < NT; ++b) {
+    const int x = min(a, b), y = max(a, b);
+    const int pp = x * NT - (x * (x - 1)) / 2 + (y - x);  // index of the pair (x, y), x <= y
+    const float* gg = g + NT * kNRad + (pp * kNEta + m) * 4;
+    const float u0 = gg[0] + gg[1] + 0.5f * (gg[2] + gg[3]);
+    const float u1 = gg[0] - gg[1] + gg[2] - gg[3];
+    const float u2 = 0.5f * (gg[2] + gg[3]);
+    const float* Z = mi + (b * kNEta + m) * kNMom;  // the partner's moments
+    for (int rep = 0; rep < (a == b ? 2 : 1); ++rep) {
+      dA[0] += u0 * Z[0];
 #pragma unroll
-      for (int c = 0; c < kNMom; ++c) dA[a][m][c] = 0.0f;
-  int pp = 0;
+      for (int c = 1; c <= 3; ++c) dA[c] += u1 * Z[c];
 #pragma unroll
-  for (int a = 0; a < NT; ++a)
+      for (int c = 4; c <= 6; ++c) dA[c] += u2 * Z[c];
 #pragma unroll
-    for (int b = a; b < NT; ++b, ++pp)
+      for (int c = 7; c <= 9; ++c) dA[c] += 2.0f * u2 * Z[c];
+    }
+    if (a == b) dA[10] -= u0 + u1 + u2;
+  }
+  float* sl = adj + (size_t)i * NT * kSlice + a * kSlice;
 #pragma unroll
-      for (int m = 0; m < kNEta; ++m) {
-        const float* gg = g + NT * kNRad + (pp * kNEta + m) * 4;
-        const float u0 = gg[0] + gg[1] + 0.5f * (gg[2] + gg[3]);
-        const float u1 = gg[0] - gg[1] + gg[2] - gg[3];
-        const float u2 = 0.5f * (gg[2] + gg[3]);
-        const float* X = mi + (a * kNEta + m) * kNMom;
-        const float* Y = mi + (b * kNEta + m) * kNMom;
-        dA[a][m][0] += u0 * Y[0]; dA[b][m][0] += u0 * X[0];
+  for (int c = 0; c < kNMom; ++c) sl[m * kNMom + c] = dA[c];
 #pragma unroll
-        for (int c = 1; c <= 3; ++c) { dA[a][m][c] += u1 * Y[c]; dA[b][m][c] += u1 * X[c]; }
-#pragma unroll
-        for (int c = 4; c <= 6; ++c) { dA[a][m][c] += u2 * Y[c]; dA[b][m][c] += u2 * X[c]; }
-#pragma unroll
-        for (int c = 7; c <= 9; ++c) { dA[a][m][c] += 2.0f * u2 * Y[c]; dA[b][m][c] += 2.0f * u2 * X[c]; }
-        if (a == b) dA[a][m][10] -= u0 + u1 + u2;
-      }
-#pragma unroll
-  for (int a = 0; a < NT; ++a) {
-    float* sl = ad + a * kSlice;
-#pragma unroll
-    for (int m = 0; m < kNEta; ++m)
-#pragma unroll
-      for (int c = 0; c < kNMom; ++c) sl[m * kNMom + c] = dA[a][m][c];
-#pragma unroll
-    for (int k = 0; k < kNRad; ++k) sl[kSliceRad + k] = g[a * kNRad + k];
+  for (int k = 0; k < kNRad / kNEta; ++k) sl[kSliceRad + m * (kNRad / kNEta) + k] = g[a * kNRad + m * (kNRad / kNEta) + k];
+  if (m == 0)
 #pragma unroll
     for (int k = kSliceRad + kNRad; k < kSlice; ++k) sl[k] = 0.0f;
-  }
 }
 
 // dE_owner/du for one neighbour at unit vector h (u = r h) of the given element slice of an
@@ -413,7 +437,7 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
 // list per neighbour element, compacted by the descriptor pass); the atom's own adjoint row is
 // staged in LDS (one row per group), each lane gathers the neighbour's element slice of its
 // pairs from global memory.
-template <int NT, bool EV, int G>
+template <int NT, bool EV, int G, bool RR>
 __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                   const int* __restrict__ typ, const BoxPar* __restrict__ boxes,
@@ -476,10 +500,13 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
       float rad = 0.0f, radi = 0.0f;
       const float* ri = as + kSliceRad;
       const float* rj = aj + kSliceRad;
+      float gk[kNRad];
+      sf_gauss<RR>(r, P, gk);
+      const float m2efc = -2.0f * P.eta_r * fc;
 #pragma unroll
       for (int k = 0; k < kNRad; ++k) {
         const float d = r - P.mu[k];
-        const float g = __builtin_amdgcn_exp2f(P.eta_r_l2 * d * d) * (dfc - 2.0f * P.eta_r * d * fc);
+        const float g = gk[k] * (m2efc * d + dfc);  // d/dr of g_k fc
         rad += (ri[k] + rj[k]) * g;
         if (EV) radi += ri[k] * g;
       }
@@ -544,6 +571,7 @@ struct NnpState {
   float *tdesc = nullptr, *tfdesc = nullptr, *tpair = nullptr, *tfpair = nullptr;
   // symmetry functions
   bool sf = false;
+  bool rad_recur = false;  // equally spaced radial centres: Gaussians by recurrence
   SfParams P{};
   float* M = nullptr;    // moments [N][NMT]
   float* adj = nullptr;  // [N][ntype][kSlice]
@@ -708,6 +736,13 @@ extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, f
   S.P.eta_r_l2 = float(-double(eta_r) * l2e);
   for (int k = 0; k < kNRad; ++k) S.P.mu[k] = mu[k];
   for (int k = 0; k < kNEta; ++k) { S.P.eta[k] = eta[k]; S.P.eta_l2[k] = float(-double(eta[k]) * l2e); }
+  // equally spaced centres (to 1e-6 of their range): the radial Gaussians by recurrence
+  const double delta = (double(mu[kNRad - 1]) - double(mu[0])) / (kNRad - 1);
+  double dev = 0;
+  for (int k = 0; k < kNRad; ++k) dev = std::max(dev, std::fabs(double(mu[k]) - (double(mu[0]) + k * delta)));
+  S.rad_recur = delta > 0 && dev <= 1e-6 * std::max(1.0, std::fabs(double(mu[kNRad - 1])));
+  S.P.rad_step_l2 = float(2.0 * double(eta_r) * delta * l2e);
+  for (int j = 0; j < kRadS; ++j) S.P.rad_rho[j] = float(std::exp(-double(eta_r) * delta * delta * (2 * j + 1)));
   S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
   NNPE_CHECK_HIP(ctx, dev_alloc(S.M, size_t(ctx->natom) * S.nmt));
   NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->natom) * ctx->ntype * kSlice));
@@ -735,10 +770,11 @@ int sf_dim(int ntype) { return ntype * kNRad + ntype * (ntype + 1) / 2 * kNEta *
 int sf_desc_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
   const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
-#define NNPE_SF_DESC(NT)                                                                                           \
-  hipLaunchKernelGGL((k_sf_desc<NT, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride,         \
+#define NNPE_SF_DESC1(NT, RR)                                                                                      \
+  hipLaunchKernelGGL((k_sf_desc<NT, kSfG, RR>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride,     \
                      ctx->cap_short, ctx->nbr_short, ctx->cnt_short, ctx->xq, ctx->d_box, ctx->rbox, S.P, S.M, nnp_G(ctx), \
                      S.act, S.act_cnt)
+#define NNPE_SF_DESC(NT) if (S.rad_recur) NNPE_SF_DESC1(NT, true); else NNPE_SF_DESC1(NT, false)
   switch (ctx->ntype) {
     case 1: NNPE_SF_DESC(1); break;
     case 2: NNPE_SF_DESC(2); break;
@@ -746,6 +782,7 @@ int sf_desc_enqueue(nnpe_ctx* ctx) {
     default: NNPE_SF_DESC(4); break;
   }
 #undef NNPE_SF_DESC
+#undef NNPE_SF_DESC1
   NNPE_CHECK_LAUNCH(ctx);
   nnp_set_current_D(ctx, sf_dim(ctx->ntype));
   return NNPE_OK;
@@ -753,7 +790,7 @@ int sf_desc_enqueue(nnpe_ctx* ctx) {
 
 int sf_adjoint_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
-  const int nblk = blocks_for(ctx->natom, 256);
+  const int nblk = blocks_for((long long)ctx->natom * ctx->ntype * kNEta, 256);
 #define NNPE_SF_ADJ(NT) \
   hipLaunchKernelGGL(k_sf_adjoint<NT>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, S.M, nnp_dG(ctx), S.adj)
   switch (ctx->ntype) {
@@ -773,9 +810,10 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
   const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
   double* pe = ev_part(ctx, NNPE_TERM_SF, 0);
   double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
-#define NNPE_SF_FORCE(NT, EV)                                                                                     \
-  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
+#define NNPE_SF_FORCE1(NT, EV, RR)                                                                                \
+  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG, RR>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
                      S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, S.P, S.adj, ctx->f, pe, pw)
+#define NNPE_SF_FORCE(NT, EV) if (S.rad_recur) NNPE_SF_FORCE1(NT, EV, true); else NNPE_SF_FORCE1(NT, EV, false)
   switch (ctx->ntype) {
     case 1: if (ev) NNPE_SF_FORCE(1, true); else NNPE_SF_FORCE(1, false); break;
     case 2: if (ev) NNPE_SF_FORCE(2, true); else NNPE_SF_FORCE(2, false); break;
@@ -783,6 +821,7 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
     default: if (ev) NNPE_SF_FORCE(4, true); else NNPE_SF_FORCE(4, false); break;
   }
 #undef NNPE_SF_FORCE
+#undef NNPE_SF_FORCE1
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_SF, nblk);
   return NNPE_OK;
