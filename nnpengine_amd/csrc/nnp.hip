@@ -369,8 +369,8 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
 // pair's terms in the order of the pair loop (x <= y) of the descriptor definition, the
 // self pair twice -- and copies four of element a's sixteen radial dE/dG into the slice.
 template <int NT>
-__global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restrict__ M, const float* __restrict__ dG,
-                                                    float* __restrict__ adj) {
+__global__ void __launch_bounds__(256) k_sf_adjoint(int n, int stride, const float* __restrict__ M,
+                                                    const float* __restrict__ dG, float* __restrict__ adj) {
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int D = NT * kNRad + NT * (NT + 1) / 2 * kNEta * 4;
@@ -402,14 +402,17 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, const float* __restri
     }
     if (a == b) dA[10] -= u0 + u1 + u2;
   }
-  float* sl = adj + (size_t)i * NT * kSlice + a * kSlice;
+  // slice float f of element a of row i at adj[((a * 16 + f / 4) * stride + i) * 4 + f % 4]
+  auto at = [&](int f) -> float& { return adj[((size_t)(a * (kSlice / 4) + (f >> 2)) * stride + i) * 4 + (f & 3)]; };
 #pragma unroll
-  for (int c = 0; c < kNMom; ++c) sl[m * kNMom + c] = dA[c];
-#pragma unroll
-  for (int k = 0; k < kNRad / kNEta; ++k) sl[kSliceRad + m * (kNRad / kNEta) + k] = g[a * kNRad + m * (kNRad / kNEta) + k];
+  for (int c = 0; c < kNMom; ++c) at(m * kNMom + c) = dA[c];
+  static_assert(kSliceRad % 4 == 0 && kNRad == 4 * kNEta, "one radial chunk per exponent lane");
+  const float* gr = g + a * kNRad + 4 * m;
+  reinterpret_cast<float4*>(adj)[(size_t)(a * (kSlice / 4) + kSliceRad / 4 + m) * stride + i] =
+      make_float4(gr[0], gr[1], gr[2], gr[3]);
   if (m == 0)
-#pragma unroll
-    for (int k = kSliceRad + kNRad; k < kSlice; ++k) sl[k] = 0.0f;
+    reinterpret_cast<float4*>(adj)[(size_t)(a * (kSlice / 4) + (kSliceRad + kNRad) / 4) * stride + i] =
+        make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // dE_owner/du for one neighbour at unit vector h (u = r h) of the given element slice of an
@@ -446,11 +449,12 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
                                                   double* pw) {
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kSlice;
-  __shared__ float s_adj[256 / G][NMT];
+  __shared__ __align__(16) float s_adj[256 / G][NMT];
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
+  const float4* __restrict__ adj4 = reinterpret_cast<const float4*>(adj);
   if (i < n)
-    for (int k = lane; k < NMT; k += G) s_adj[grp][k] = adj[(size_t)i * NMT + k];
+    for (int q = lane; q < NMT / 4; q += G) reinterpret_cast<float4*>(s_adj[grp])[q] = adj4[(size_t)q * stride + i];
   __syncthreads();
   double wacc = 0;
   if (i < n) {
@@ -494,7 +498,14 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
         wv[m] = ex * fc;
         dwv[m] = ex * (dfc - 2.0f * P.eta[m] * r * fc);
       }
-      const float* aj = adj + (size_t)j * NMT + ti * kSlice;  // neighbour's slice for this atom's element
+      // the neighbour's slice for this atom's element, 16-byte chunk q at adj4[(ti*16 + q) * stride
+      // + j]: the lanes' neighbours are rows of a few cells, so a wave's chunk loads share lines
+      float aj[kSliceRad + kNRad];
+#pragma unroll
+      for (int q = 0; q < (kSliceRad + kNRad) / 4; ++q) {
+        const float4 v = adj4[(size_t)(ti * (kSlice / 4) + q) * stride + j];
+        aj[4 * q] = v.x; aj[4 * q + 1] = v.y; aj[4 * q + 2] = v.z; aj[4 * q + 3] = v.w;
+      }
       const float* as = ai + tj * kSlice;                      // own slice for the neighbour's element
       // radial: sum_n g_n'(r) (dR_i[tj][n] + dR_j[ti][n])
       float rad = 0.0f, radi = 0.0f;
@@ -745,7 +756,7 @@ extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, f
   for (int j = 0; j < kRadS; ++j) S.P.rad_rho[j] = float(std::exp(-double(eta_r) * delta * delta * (2 * j + 1)));
   S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
   NNPE_CHECK_HIP(ctx, dev_alloc(S.M, size_t(ctx->natom) * S.nmt));
-  NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->natom) * ctx->ntype * kSlice));
+  NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->stride) * ctx->ntype * kSlice));
   S.sf = true;
   S.act_cap = 0;
   int rc0 = nnp_act_reserve(ctx);
@@ -792,7 +803,8 @@ int sf_adjoint_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
   const int nblk = blocks_for((long long)ctx->natom * ctx->ntype * kNEta, 256);
 #define NNPE_SF_ADJ(NT) \
-  hipLaunchKernelGGL(k_sf_adjoint<NT>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, S.M, nnp_dG(ctx), S.adj)
+  hipLaunchKernelGGL(k_sf_adjoint<NT>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, S.M, nnp_dG(ctx), \
+                     S.adj)
   switch (ctx->ntype) {
     case 1: NNPE_SF_ADJ(1); break;
     case 2: NNPE_SF_ADJ(2); break;
