@@ -177,6 +177,35 @@ def serving(par, norm, nreq, steps, seed=2027):
             "relayouts": int(st["relayouts"]), "chunks": int(st["chunks"]), "arena_bytes": int(st["arena_bytes"])}
 
 
+E_TERMS = (0, 1, 2, 3, 4, 5, 7)  # pair, bond, angle, dihedral, NNP table pair, network, kinetic
+
+
+def nve_drift(sysm, par, norm, prec, ps, log_every=200):
+    """NVE energy drift of one precision form: the workload's MD step (FP64 Verlet) for `ps`
+    picoseconds from the starting snapshot, total energy logged every `log_every` steps; the
+    drift is the slope of a linear fit of E_tot / N against time.  Returns the drift record and
+    the final positions."""
+    e = make_engine(sysm, par, norm, prec)
+    steps = int(round(ps * 1000 / DT))
+    steps -= steps % log_every
+    t0 = time.perf_counter()
+    # chunks of 1000 steps; a chunk that overflows a list as the lattice melts is repeated from
+    # its start with the grown capacities
+    log, redo = e.md_run_checkpointed(steps, chunk=1000, log_every=log_every, **MD_KW)
+    wall = time.perf_counter() - t0
+    st = e.md_stats()
+    x = e.get("x")
+    e.close()
+    et = log[:, [2 * k for k in E_TERMS]].sum(1) / sysm.natom
+    t_ns = np.arange(len(et)) * log_every * DT * 1e-6
+    slope = float(np.polyfit(t_ns, et, 1)[0]) if len(et) > 2 else float("nan")
+    ke = log[:, 14] / sysm.natom
+    return {"ps": steps * DT / 1000, "steps": steps, "drift_kcal_mol_atom_per_ns": slope,
+            "e_tot_per_atom_first_last": [float(et[0]), float(et[-1])], "e_tot_per_atom_std": float(et.std()),
+            "ke_per_atom_mean": float(ke.mean()), "skin_violations": int(st["skin_violations"]),
+            "repeated_chunks": int(redo), "wall_s": wall}, x
+
+
 def sqd_timing(e, q_max=20.0, nq=1000, reps=3):
     """End-of-run analysis: direct k-space S(q) of the current 24000-atom positions over every
     reciprocal vector of the box with |G| <= 20 1/A (half space), 1000 bins."""
@@ -216,7 +245,7 @@ def hbm_traffic(stage):
 
 
 def force_errors(sysm, par, norm):
-    """NNP and total force relative RMS vs the FP64 oracle on the starting snapshot."""
+    """NNP and total force relative RMS vs the FP64 oracle on one snapshot."""
     from nnpengine_amd import model as M
     from nnpengine_amd import system as S
     from oracle import pyoracle
@@ -277,6 +306,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--no-sqd", action="store_true")
+    ap.add_argument("--drift-ps", type=float, default=20.0, help="NVE run per precision form for the drift (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is "
@@ -370,7 +400,21 @@ def main():
     if args.serve_requests > 0:
         serve = serving(par, norm, args.serve_requests, args.serve_steps)
         log("[bench] serving:", {k: v for k, v in serve.items() if k != "latency_s"}, serve["latency_s"])
-    errs = None if args.no_err else force_errors(sysm, par, norm)
+    drift, x_end = {}, None
+    if args.drift_ps > 0:
+        for prec in ("bf16", "fp16", "fp32"):
+            drift[prec], xe = nve_drift(sysm, par, norm, prec, args.drift_ps)
+            if prec == "bf16":
+                x_end = xe
+            log(f"[bench] NVE {prec}: {drift[prec]}")
+    errs = None
+    if not args.no_err:
+        errs = {"start": force_errors(sysm, par, norm)}
+        if x_end is not None:
+            # the liquid after the BF16 NVE run (positions as the engine holds them)
+            snap = sysm.copy()
+            snap.x = np.ascontiguousarray(x_end, np.float32)
+            errs[f"after_{drift['bf16']['ps']:g}ps_bf16_nve"] = force_errors(snap, par, norm)
     cpu = None if args.no_cpu else cpu_baseline(sysm, par, norm)
 
     t_b, t_f = res["bf16"], res["fp32"]
@@ -396,6 +440,7 @@ def main():
         "fp32_form": {"value": N * args.steps / t_f, "ms_per_step": t_f / args.steps * 1e3},
         "bf16_speedup_vs_fp32": t_f / t_b,
         "force_error_vs_fp64": errs,
+        "nve_drift": drift or None,
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "serving": serve,
         "stage_ms": stages,

@@ -214,6 +214,11 @@ int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* params, int nsteps, double*
 int nnpe_md_stats(nnpe_ctx* ctx, long long* out);
 /* Restart the step counter (the next step evaluates forces on the positions as they are). */
 int nnpe_md_reset_step(nnpe_ctx* ctx);
+/* Continue at step `step` (> 0: the next step starts with the position update, as inside a
+ * run).  With nnpe_state_set of X, V and FPREV (the accelerations) saved at that step, a run
+ * that stopped on a list or ghost overflow (capacities grown) is repeated from a checkpoint.
+ * No reference counterpart (the reference ends the program on an overflow). */
+int nnpe_md_set_step(nnpe_ctx* ctx, long long step);
 
 /* ---- Serving: a request loop over independent boxes -----------------------------------------
  * No reference counterpart (the reference runs one system per process).  Queued boxes are

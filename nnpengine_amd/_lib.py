@@ -71,6 +71,7 @@ SIGNATURES = [
     ("nnpe_md_run", _I, [_P, _P, _I, _PD, _I, _PI]),
     ("nnpe_md_stats", _I, [_P, _PLL]),
     ("nnpe_md_reset_step", _I, [_P]),
+    ("nnpe_md_set_step", _I, [_P, C.c_longlong]),
     ("nnpe_profile_stage", _I, [_P, _I, _I, _PD]),
     ("nnpe_serve_create", _I, [_P, C.POINTER(_P)]),
     ("nnpe_serve_destroy", _I, [_P]),

@@ -316,3 +316,9 @@ extern "C" int nnpe_md_reset_step(nnpe_ctx* ctx) {
   ctx->md_step = 0;
   return NNPE_OK;
 }
+
+extern "C" int nnpe_md_set_step(nnpe_ctx* ctx, long long step) {
+  if (!ctx || step < 0) return NNPE_ERR_ARG;
+  ctx->md_step = step;
+  return NNPE_OK;
+}

@@ -42,7 +42,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 namespace {
 
 constexpr int kMaxLayers = 6;   // weight layers including the output layer
-constexpr int kMaxWidth = 512;  // widest hidden layer
+constexpr int kMaxWidth = 128;  // widest hidden layer: one 32-column block per wave
 constexpr int kMaxType = 4;
 
 struct MlpDev {
@@ -83,7 +83,7 @@ template <> struct Prec<1> {  // BF16
   typedef __bf16 T;
   typedef bf16x8 F;
   static constexpr int KS = 16;
-  static constexpr int NB = 8;
+  static constexpr int NB = 4;  // B fragments per batch (hi and lo: eight fragment loads in flight)
   __device__ static F ld(const T* p) { return *reinterpret_cast<const bf16x8*>(p); }
   __device__ static f32x16 mma(F a, F b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
   __device__ static T cvt(float v) { return (__bf16)v; }
@@ -211,20 +211,23 @@ __device__ __forceinline__ f32x16 gemm_block_pairs(const __bf16* Ahi, const __bf
 
 constexpr int kMlpWaves = 4;
 
-// tanh: the FP32 form calls tanhf; the 16-bit forms store the activation rounded to 8 or 11
-// significant bits, so they take 1 - 2/(exp(2x) + 1) on the hardware exp2 and reciprocal
-// (absolute error about 1e-7, saturating to +-1 for large |x|)
-template <int KIND>
-__device__ __forceinline__ float act_tanh(float x) {
-  if constexpr (KIND == 0) {
-    return tanhf(x);
-  } else {
-    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // exp(2x) = 2^(2x / ln 2)
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
-  }
+// tanh and its derivative from one exponential: with e = exp(-2|z|) (hardware exp2) and
+// r = 1/(1 + e) (hardware reciprocal), tanh z = sign(z) (1 - e) r and 1 - tanh^2 z = 4 e r^2.
+// Both keep their relative precision (a few ulp) for any |z|: the derivative of a saturated unit
+// is never formed as 1 - a^2 from an activation that has been rounded to 8 or 11 bits.
+__device__ __forceinline__ void tanh_d(float z, float& a, float& t) {
+  const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(z));  // exp(-2|z|) = 2^(-2|z| / ln 2)
+  const float r = __builtin_amdgcn_rcpf(1.0f + e);
+  a = copysignf((1.0f - e) * r, z);
+  t = 4.0f * e * r * r;
 }
 
-template <int KIND>
+// One workgroup of kMlpWaves waves per 32-atom tile of one element, NH hidden layers of width at
+// most 32 * kMlpWaves, so wave w owns the output column block w of every layer.  The forward
+// epilogue of hidden layer l keeps 1 - a_l^2 in FP32 registers (the backward epilogue of the
+// same column block needs exactly those values); the last hidden layer's epilogue forms the
+// energy partial sums and d_L = s_out w_out (1 - a_L^2) directly.
+template <int KIND, int NH>
 // three waves per SIMD: the LDS of three tiles fits a CU, so registers are capped to match
 __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(3))) k_mlp(MlpDev P, const float* __restrict__ G, float* __restrict__ dG,
                                            float* __restrict__ e_atom, const int* __restrict__ type_idx) {
@@ -239,16 +242,17 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
   const int r0 = (tile - P.tile_off[e]) * 32;
   const int nrows = min(32, P.count[e] - r0);
   const int* rows = type_idx + P.type_off[e] + r0;
-  const int D = P.D, Dp = P.Dp, nh = P.nh;
+  const int D = P.D, Dp = P.Dp;
   T* Gs = lds + P.slab_off[0];
   const int ldg = P.ld[0];
 
   // stage the descriptor rows standardised in fp32, zero padding rows/cols; row ids go through
   // LDS so every global load of the loop is independent of the others (they overlap)
   __shared__ int srow[32];
-  __shared__ float s_wout[kMaxWidth];  // output-layer weights of the element
+  __shared__ float s_wout[32 * kMlpWaves];  // output-layer weights of the element
+  __shared__ float s_epart[kMlpWaves][32];  // per-wave energy partial sums of the rows
   if (tid < 32) srow[tid] = tid < nrows ? rows[tid] : -1;
-  for (int c = tid; c < P.H[nh - 1]; c += NT) s_wout[c] = P.wout[e][c];
+  for (int c = tid; c < P.H[NH - 1]; c += NT) s_wout[c] = P.wout[e][c];
   __syncthreads();
   const float* shv = P.shift[e];
   const float* scv = P.scale[e];
@@ -283,8 +287,11 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
   }
   __syncthreads();
 
+  const int row = lane & 31, cb = wave, f0 = cb * 32 + 4 * (lane >> 5);  // this lane: features f0 + 8q + t
+  float tv[NH > 1 ? NH - 1 : 1][16];  // 1 - a_l^2 of this lane's features, hidden layers 0 .. NH-2
   // forward
-  for (int l = 0; l < nh; ++l) {
+#pragma unroll
+  for (int l = 0; l < NH; ++l) {
     const T* src = l == 0 ? Gs : lds + P.slab_off[l];
     const int lds_src = P.ld[l];
     const int K = l == 0 ? Dp : P.H[l - 1];
@@ -293,8 +300,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
     const T* wt = reinterpret_cast<const T*>(P.Wt[e][l]);
     const float* bias = P.b[e][l];
     const float un = 1.0f / P.s_w[l];
-    for (int cb = wave; cb < P.H[l] / 32; cb += kMlpWaves) {
-      const int row = lane & 31, f0 = cb * 32 + 4 * (lane >> 5);  // features f0 + 8q + t
+    const bool have = cb < P.H[l] / 32;
+    float es = 0.0f;  // last layer: this lane's part of its row's energy
+    if (have) {
       float4 bq[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) bq[q] = *reinterpret_cast<const float4*>(bias + f0 + 8 * q);  // before the chain
@@ -303,47 +311,47 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
         acc = gemm_block_pairs(src, nullptr, lds_src, wt, reinterpret_cast<const T*>(P.Wt_lo[e][l]), K, cb, lane);
       else
         acc = gemm_block<KIND>(src, lds_src, wt, K, cb, lane);
+      const float sc = P.out_scale * P.s_d[NH - 1];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float bv[4] = {bq[q].x, bq[q].y, bq[q].z, bq[q].w};
-        T o[4];
+        T o[4], lo[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) o[t] = Prec<KIND>::cvt(act_tanh<KIND>(acc[4 * q + t] * un + bv[t]));
+        for (int t = 0; t < 4; ++t) {
+          float a, d;
+          tanh_d(acc[4 * q + t] * un + bv[t], a, d);
+          if (l + 1 < NH) {
+            o[t] = Prec<KIND>::cvt(a);
+            tv[l < NH - 1 ? l : 0][4 * q + t] = d;
+          } else {
+            // d_L = s_out w_out (1 - a_L^2), scaled by s_d[NH-1]; the energy uses the FP32 a_L
+            const float w = s_wout[f0 + 8 * q + t];
+            es += a * w;
+            const float v = sc * w * d;
+            o[t] = Prec<KIND>::cvt(v);
+            if constexpr (KIND == 1) lo[t] = Prec<KIND>::cvt(v - Prec<KIND>::back(o[t]));
+          }
+        }
         store4(dst + row * ldd + f0 + 8 * q, o);
+        if constexpr (KIND == 1)
+          if (l + 1 == NH) store4(lds + P.lo_off[NH] + row * ldd + f0 + 8 * q, lo);
       }
+    }
+    if (l + 1 == NH) {
+      es += __shfl_xor(es, 32, 64);  // lanes l and l + 32 hold the two halves of row l % 32
+      if (lane < 32) s_epart[wave][lane] = es;
     }
     __syncthreads();
   }
-
-  // energy of each row and d_L = s_out * w_out * (1 - a_L^2) (stored in place of a_L, scaled by
-  // s_d[nh-1]) in one pass: 8 threads per row, each over every 8th column, then a 3-step
-  // butterfly over the 8
-  static_assert(NT == 256, "one row per 8 threads");
-  {
-    const int HL = P.H[nh - 1];
-    T* dL = lds + P.slab_off[nh];
-    const int ldl = P.ld[nh];
-    const float* wo = s_wout;
-    const float sc = P.out_scale * P.s_d[nh - 1];
-    const int r = tid >> 3, sub = tid & 7;
+  // energy of each row: the waves' partial sums in a fixed order
+  if (tid < nrows) {
     float s = 0.0f;
-#pragma unroll 4
-    for (int c = sub; c < HL; c += 8) {
-      const float a = Prec<KIND>::back(dL[r * ldl + c]);
-      const float w = wo[c];
-      s += a * w;
-      const float v = sc * w * (1.0f - a * a);
-      const T hi = Prec<KIND>::cvt(v);
-      dL[r * ldl + c] = hi;
-      if constexpr (KIND == 1) lds[P.lo_off[nh] + r * ldl + c] = Prec<KIND>::cvt(v - Prec<KIND>::back(hi));
-    }
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    s += __shfl_xor(s, 4, 64);
-    if (sub == 0 && r < nrows) e_atom[srow[r]] = P.out_scale * (s + P.bout[e]);
+#pragma unroll
+    for (int w = 0; w < kMlpWaves; ++w) s += s_epart[w][tid];
+    e_atom[srow[tid]] = P.out_scale * (s + P.bout[e]);
   }
-  __syncthreads();
-  for (int l = nh - 1; l >= 1; --l) {
+#pragma unroll
+  for (int l = NH - 1; l >= 1; --l) {
     // d_{l-1} = (d_l W_l^T) * (1 - a_{l-1}^2): A = d_l [32 x H_l], B[k=out][n=in] = W_l [in][out]
     const T* dl = lds + P.slab_off[l + 1];
     const int ldl2 = P.ld[l + 1];
@@ -352,23 +360,20 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
     const T* w = reinterpret_cast<const T*>(P.W[e][l]);
     const float un = 1.0f / (P.s_d[l] * P.s_w[l]);
     const float sd = P.s_d[l - 1];
-    for (int cb = wave; cb < P.H[l - 1] / 32; cb += kMlpWaves) {
+    if (cb < P.H[l - 1] / 32) {
       f32x16 acc;
       if constexpr (KIND == 1)
         acc = gemm_block_pairs(dl, lds + P.lo_off[l + 1], ldl2, w, reinterpret_cast<const T*>(P.W_lo[e][l]), P.H[l], cb,
                                lane);
       else
         acc = gemm_block<KIND>(dl, ldl2, w, P.H[l], cb, lane);
-      const int row = lane & 31, f0 = cb * 32 + 4 * (lane >> 5);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         T* p = prev + row * ldp + f0 + 8 * q;
-        T a4[4], hi[4], lo[4];
-        load4(p, a4);
+        T hi[4], lo[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const float a = Prec<KIND>::back(a4[t]);
-          const float v = acc[4 * q + t] * un * (1.0f - a * a) * sd;
+          const float v = acc[4 * q + t] * un * tv[l - 1][4 * q + t] * sd;
           hi[t] = Prec<KIND>::cvt(v);
           if constexpr (KIND == 1) lo[t] = Prec<KIND>::cvt(v - Prec<KIND>::back(hi[t]));
         }
@@ -383,19 +388,19 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
     const T* d1 = lds + P.slab_off[1];
     const T* w = reinterpret_cast<const T*>(P.W[e][0]);
     const float un = 1.0f / (P.s_d[0] * P.s_w[0]);
-    for (int cb = wave; cb < Dp / 32; cb += kMlpWaves) {
+    for (int cb2 = wave; cb2 < Dp / 32; cb2 += kMlpWaves) {
       f32x16 acc;
       if constexpr (KIND == 1)
-        acc = gemm_block_pairs(d1, lds + P.lo_off[1], P.ld[1], w, reinterpret_cast<const T*>(P.W_lo[e][0]), P.H[0], cb,
+        acc = gemm_block_pairs(d1, lds + P.lo_off[1], P.ld[1], w, reinterpret_cast<const T*>(P.W_lo[e][0]), P.H[0], cb2,
                                lane);
       else
-        acc = gemm_block<KIND>(d1, P.ld[1], w, P.H[0], cb, lane);
-      const int row = lane & 31, f0 = cb * 32 + 4 * (lane >> 5);
+        acc = gemm_block<KIND>(d1, P.ld[1], w, P.H[0], cb2, lane);
+      const int f02 = cb2 * 32 + 4 * (lane >> 5);
       if (row < nrows) {
         float* out = dG + (size_t)srow[row] * D;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int f = f0 + 8 * q;
+          const int f = f02 + 8 * q;
           if ((D & 3) == 0) {
             if (f < D) {  // D a multiple of 4: the 4 features are all in or all out
               const float4 sc = *reinterpret_cast<const float4*>(scv + f);
@@ -504,7 +509,7 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
   if (widths[nlayer] != 1) return ctx_fail(ctx, NNPE_ERR_ARG, "the output layer must have width 1");
   for (int l = 1; l < nlayer; ++l)
     if (widths[l] % 32 || widths[l] <= 0 || widths[l] > kMaxWidth)
-      return ctx_fail(ctx, NNPE_ERR_ARG, "hidden widths must be multiples of 32, at most 512");
+      return ctx_fail(ctx, NNPE_ERR_ARG, "hidden widths must be multiples of 32, at most 128");
   mlp_state_free(ctx);
   MlpState* S = ctx->mlp = new MlpState;
   S->kind = precision; S->nl = nlayer;
@@ -691,12 +696,22 @@ int mlp_enqueue(nnpe_ctx* ctx, bool ev) {
   MlpState* S = ctx->mlp;
   const MlpDev& d = S->dev;
   if (S->ntiles > 0) {
-    if (S->kind == 0)
-      hipLaunchKernelGGL(k_mlp<0>, dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
-    else if (S->kind == 1)
-      hipLaunchKernelGGL(k_mlp<1>, dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
-    else
-      hipLaunchKernelGGL(k_mlp<2>, dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx);
+#define NNPE_MLP(K, NH)                                                                                      \
+  hipLaunchKernelGGL((k_mlp<K, NH>), dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, \
+                     ctx->dG, ctx->e_atom, ctx->type_idx)
+#define NNPE_MLP_K(K)                         \
+  switch (d.nh) {                             \
+    case 1: NNPE_MLP(K, 1); break;            \
+    case 2: NNPE_MLP(K, 2); break;            \
+    case 3: NNPE_MLP(K, 3); break;            \
+    case 4: NNPE_MLP(K, 4); break;            \
+    default: NNPE_MLP(K, 5); break;           \
+  }
+    if (S->kind == 0) { NNPE_MLP_K(0) }
+    else if (S->kind == 1) { NNPE_MLP_K(1) }
+    else { NNPE_MLP_K(2) }
+#undef NNPE_MLP_K
+#undef NNPE_MLP
     NNPE_CHECK_LAUNCH(ctx);
   }
   if (ev) {

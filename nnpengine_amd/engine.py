@@ -295,6 +295,33 @@ class Engine:
     def md_reset_step(self):
         self._check(self.L.nnpe_md_reset_step(self.h))
 
+    def md_set_step(self, step):
+        self._check(self.L.nnpe_md_set_step(self.h, int(step)))
+
+    def md_run_checkpointed(self, nsteps, chunk=1000, retries=3, **kw):
+        """md_run in chunks of `chunk` steps (a multiple of the rebuild interval); a chunk that
+        ends on a list or ghost overflow (the engine grows its capacities) is repeated from the
+        positions, velocities and accelerations saved at its start.  Returns the concatenated
+        log and the number of repeated chunks."""
+        logs, redo, done = [], 0, 0
+        while done < nsteps:
+            n = min(chunk, nsteps - done)
+            step0 = self.md_stats()["steps"]
+            saved = [self.get(w) for w in ("x", "v", "f", "f_prev")]
+            for attempt in range(retries + 1):
+                try:
+                    logs.append(self.md_run(n, **kw))
+                    break
+                except _lib.NnpeError as err:
+                    if err.code not in (4, 5) or attempt == retries:
+                        raise
+                    redo += 1
+                    for w, a in zip(("x", "v", "f", "f_prev"), saved):
+                        self.set(w, a)
+                    self.md_set_step(step0)
+            done += n
+        return (np.concatenate(logs) if logs else np.zeros((0, 2 * _lib.NTERM))), redo
+
     def profile(self, stage, reps=20):
         """Average ms of one launch of a stage (HIP events on the engine's stream)."""
         ms = C.c_double(0)

@@ -36,10 +36,13 @@ def mlp_init(widths, ntype, seed):
     return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
 
 
-def standardisation(G, typ, ntype, floor=0.5):
+def standardisation(G, typ, ntype, floor=0.5, abs_floor=0.1):
     """Per-element input standardisation x = (G - shift) * scale of the network: shift = mean,
-    scale = 1 / max(std, floor * rms).  The floor keeps components whose spread is tiny against
-    their magnitude (e.g. on a lattice snapshot) from amplifying descriptor rounding.
+    scale = 1 / max(std, floor * rms, abs_floor * RMS), rms the component's root mean square and
+    RMS that of all the element's components.  The first floor keeps components whose spread is
+    tiny against their magnitude (e.g. on a lattice snapshot) from amplifying descriptor
+    rounding; the second keeps components that are nearly zero on the snapshot (short-range
+    radial terms of a lattice, ~1e-10 there but O(0.1) in the liquid) from scales of 1e10.
     Returns shift, scale as [ntype, D] float32."""
     D = G.shape[1]
     shift = np.zeros((ntype, D))
@@ -50,8 +53,9 @@ def standardisation(G, typ, ntype, floor=0.5):
             m = g.mean(0)
             sd = g.std(0)
             rms = np.sqrt(m * m + sd * sd)
+            big = np.sqrt((rms * rms).mean())
             shift[t] = m
-            scale[t] = 1.0 / np.maximum(np.maximum(sd, floor * rms), 1e-12)
+            scale[t] = 1.0 / np.maximum(np.maximum(np.maximum(sd, floor * rms), abs_floor * big), 1e-12)
     return shift.astype(np.float32), scale.astype(np.float32)
 
 

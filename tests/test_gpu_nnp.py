@@ -114,6 +114,51 @@ def test_sf_forces_bf16_error(water_c1):
     assert errs["fp16"] < 5e-3
 
 
+def test_precision_forms_100_snapshots():
+    """NNP force error of each precision form vs FP64 on 100 seeded 768-atom snapshots (the
+    water lattice start with independent orientations per seed), 2 x 64 tanh network, one
+    standardisation (seed 0).  Bars on every snapshot: FP32 1e-5, BF16 1e-2, FP16 5e-3."""
+    from nnpengine_amd.engine import Engine
+    widths = [80, 64, 64, 1]
+    sfp = M.sf_params(RC_NNP)
+    par = M.mlp_init(widths, 2, 5)
+    norm = None
+    errs = {"fp32": [], "bf16": [], "fp16": []}
+    for seed in range(100):
+        sysm = S.water_box(256, 19.73, seed=1000 + seed)
+        o = pyoracle.Oracle(sysm, 1)
+        o.neigh_init(RC_PAIR, SKIN, RC_PAIR)
+        o.rebuild(True)
+        o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+        G64 = o.sf_descriptor()
+        if norm is None:
+            norm = M.standardisation(G64, sysm.typ, sysm.ntype)
+        o.mlp_set(widths, par, 0.5, *norm)
+        _, _, dG64 = o.mlp_eval(G64)
+        o.clear_f()
+        o.sf_force(dG64)
+        f64 = o.get("f")
+        e = Engine(sysm)
+        e.neigh_init(RC_PAIR, SKIN, RC_PAIR, RC_NNP)
+        e.build()
+        e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+        for prec in errs:
+            e.mlp_set(widths, par, 0.5, prec, *norm)
+            e.sf_descriptor()
+            e.clear_force()
+            e.mlp_eval(False)
+            e.sf_force(False)
+            errs[prec].append(rel_rms(e.get("f"), f64))
+        e.close()
+    for prec, v in errs.items():
+        v = np.array(v)
+        print(f"[snapshots] {prec}: NNP force rel-rms vs fp64 over {len(v)} snapshots: mean {v.mean():.2e} "
+              f"max {v.max():.2e}", flush=True)
+    assert max(errs["fp32"]) < 1e-5
+    assert max(errs["bf16"]) < 1e-2
+    assert max(errs["fp16"]) < 5e-3
+
+
 def test_tbl_descriptor_and_force(water_c0):
     sysm = water_c0
     from nnpengine_amd.engine import Engine
@@ -191,3 +236,32 @@ def test_md_nve_energy_conservation(water_c0):
     st = e.md_stats()
     assert st["skin_violations"] == 0
     assert abs(et[-1] - et[1]) < 0.02 * abs(log[1, 14])
+
+
+def test_md_checkpoint_rerun_bit_identical(water_c0):
+    """A chunk repeated from a checkpoint (nnpe_state_set of x, v, f, f_prev and nnpe_md_set_step,
+    the path that repeats a chunk after a list overflow) reproduces the first attempt bit for bit,
+    NNP on, FP64 integration."""
+    widths = [80, 64, 64, 1]
+    e, o, par, G64, norm = _setup(water_c0, widths)
+    e.pair_table(RC_PAIR, *S.lj_table(water_c0, RC_PAIR, 900, S.WATER_LJ))
+    e.bonded_set_from_system()
+    e.mlp_set(widths, par, 0.5, "bf16", *norm)
+    kw = dict(dt=0.5, fp64=True, rebuild_every=10, nnp=2, log_every=10)
+    e.md_run(100, **kw)
+    step0 = e.md_stats()["steps"]
+    saved = {w: e.get(w) for w in ("x", "v", "f", "f_prev")}
+    log_a = e.md_run(100, **kw)
+    xa, va = e.get("x"), e.get("v")
+    for w, a in saved.items():
+        e.set(w, a)
+    e.md_set_step(step0)
+    log_b = e.md_run(100, **kw)
+    assert np.array_equal(xa, e.get("x")) and np.array_equal(va, e.get("v"))
+    assert np.array_equal(log_a, log_b)
+    # the checkpointed driver over the same steps gives the same trajectory
+    for w, a in saved.items():
+        e.set(w, a)
+    e.md_set_step(step0)
+    log_c, redo = e.md_run_checkpointed(100, chunk=50, **kw)
+    assert redo == 0 and np.array_equal(log_a, log_c) and np.array_equal(xa, e.get("x"))
