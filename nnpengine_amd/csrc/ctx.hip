@@ -58,7 +58,7 @@ static void free_all(nnpe_ctx* c) {
   dev_free(c->cell_start); dev_free(c->cell_cnt); dev_free(c->cell_fill);
   dev_free(c->gcell_start); dev_free(c->gcell_cnt); dev_free(c->gcell_fill);
   dev_free(c->g_src); dev_free(c->g_code); dev_free(c->g_cell); dev_free(c->gs_src); dev_free(c->gs_code);
-  dev_free(c->stencil_pair); dev_free(c->stencil_gr);
+  dev_free(c->stencil_pair); dev_free(c->stencil_gr); dev_free(c->stencil_gr_outer);
   dev_free(c->nbr_long); dev_free(c->nbr_short); dev_free(c->cnt_long); dev_free(c->cnt_short);
   dev_free(c->scan_tmp); dev_free(c->ev_partial);
   dev_free(c->pt_tab);

@@ -19,6 +19,7 @@
 namespace {
 
 constexpr int kHistLdsMax = 48 * 1024;
+constexpr int kHistLdsFused = 16 * 1024;  // histogram added to the rebuild kernel's LDS
 constexpr int kGrG = 16;  // lanes per atom, inner tally
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
@@ -26,7 +27,7 @@ __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   return __mul24(i, ntype) - (__mul24(i, i - 1) >> 1) + (j - i);
 }
 
-__device__ __forceinline__ int gr_bin(float rsq, float rdr) { return int(__fmul_rn(sqrtf(rsq), rdr)); }
+__device__ __forceinline__ int gr_bin(float rsq, float rdr) { return gr_bin_of(rsq, rdr); }
 
 __global__ void k_gr_zero(unsigned long long* count, int npair, int nbin, int nzero) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,6 +188,60 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
   }
 }
 
+// Shell pass of the full tally folded into a list rebuild: the rebuild counted every pair of its
+// own candidates (the pair stencil); the pairs within r_max whose cells lie outside it are in
+// the outer stencil cells.  Thread per (atom, outer cell): cells whose box lies farther than
+// r_max (plus a margin) from the atom are skipped, the rest are walked (real rows, then ghost
+// images): a real pair adds 2 from its lower row, a ghost image 1 (2 per unordered pair, the
+// reference's rule).
+// Positions and cells are those of the rebuild that just ran, so the cell boxes bound the atoms.
+__global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* __restrict__ outer,
+                                                  const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
+                                                  const int* __restrict__ cell, const float4* __restrict__ xq,
+                                                  const int* __restrict__ typ, const int* __restrict__ cstart,
+                                                  const int* __restrict__ gstart, const int* __restrict__ gs_src,
+                                                  const int* __restrict__ gs_code, int ntype, float rmaxsq,
+                                                  float rcull_sq, float rdr, int nbin,
+                                                  unsigned long long* __restrict__ count) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n * nout) return;
+  const int i = int(t / nout), s = int(t - (long long)i * nout);
+  const BoxPar B = boxes[rbox[i]];
+  const int num_gx = B.num_gx, lc = cell[i] - B.cell_base;
+  const int cz = lc % num_gx, cy = (lc / num_gx) % num_gx, cx = lc / (num_gx * num_gx);
+  const int4 st = outer[s];
+  const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
+  if (ax < 0 || ay < 0 || az < 0 || ax >= num_gx || ay >= num_gx || az >= num_gx) return;
+  const float4 xi = xq[i];
+  const float gdx = 1.0f / B.g_rdx;
+  // distance from the atom to the cell box [k gdx - off, (k+1) gdx - off) along one axis
+  auto gap = [&](int d, int k, float x) -> float {
+    if (d > 0) return fmaxf(0.0f, float(k) * gdx - B.g_offset - x);
+    if (d < 0) return fmaxf(0.0f, x - (float(k + 1) * gdx - B.g_offset));
+    return 0.0f;
+  };
+  const float gx = gap(st.x, ax, xi.x), gy = gap(st.y, ay, xi.y), gz = gap(st.z, az, xi.z);
+  if (gx * gx + gy * gy + gz * gz > rcull_sq) return;
+  const int nc = B.cell_base + (ax * num_gx + ay) * num_gx + az;
+  const int ti = typ[i];
+  const float bl = B.bl, m_bl = B.m_bl;
+  for (int j = max(cstart[nc], i + 1); j < cstart[nc + 1]; ++j) {
+    const float4 xj = xq[j];
+    const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+    if (!(rsq < rmaxsq)) continue;
+    const int b = gr_bin(rsq, rdr);
+    if (b < nbin) atomicAdd(&count[(size_t)pair_index(ti, typ[j], ntype) * nbin + b], 2ull);
+  }
+  for (int g = gstart[nc]; g < gstart[nc + 1]; ++g) {
+    const int src = gs_src[g];
+    const float4 xj = image_pos(xq[src], gs_code[g], bl, m_bl);
+    const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+    if (!(rsq < rmaxsq)) continue;
+    const int b = gr_bin(rsq, rdr);
+    if (b < nbin) atomicAdd(&count[(size_t)pair_index(ti, typ[src], ntype) * nbin + b], 1ull);
+  }
+}
+
 __global__ void k_gr_reduce(int total, const unsigned long long* count, const double* norm, double* h) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < total) h[k] = double(count[k]) * norm[k] - 1.0;
@@ -292,6 +347,35 @@ int gr_enqueue(nnpe_ctx* ctx, bool full) {
   NNPE_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_gr_reduce, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, s, npair * nb, ctx->gr_count, ctx->gr_norm,
                      ctx->gr_h);
+  NNPE_CHECK_LAUNCH(ctx);
+  return NNPE_OK;
+}
+
+// the full tally can ride on a rebuild when its histogram fits the rebuild kernel's LDS
+bool gr_fusable(nnpe_ctx* ctx) {
+  return ctx->gr_count && ctx->nbox == 1 && size_t(ctx->npair) * ctx->gr_nbin * sizeof(unsigned int) <= kHistLdsFused;
+}
+
+int gr_zero_all_enqueue(nnpe_ctx* ctx) {
+  const int npair = ctx->npair, nb = ctx->gr_nbin;
+  hipLaunchKernelGGL(k_gr_zero, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, ctx->stream, ctx->gr_count, npair, nb, nb);
+  NNPE_CHECK_LAUNCH(ctx);
+  return NNPE_OK;
+}
+
+int gr_shell_finish_enqueue(nnpe_ctx* ctx) {
+  const int npair = ctx->npair, nb = ctx->gr_nbin;
+  if (ctx->n_stencil_gr_outer > 0) {
+    const float rcull = std::sqrt(ctx->gr_rmaxsq) + 0.01f;
+    const long long nt = (long long)ctx->natom * ctx->n_stencil_gr_outer;
+    hipLaunchKernelGGL(k_gr_shell, dim3(blocks_for(nt, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->n_stencil_gr_outer,
+                       ctx->stencil_gr_outer, ctx->d_box, ctx->rbox, ctx->cell, ctx->xq, ctx->typ, ctx->cell_start,
+                       ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->ntype, ctx->gr_rmaxsq, rcull * rcull, ctx->gr_rdr, nb,
+                       ctx->gr_count);
+    NNPE_CHECK_LAUNCH(ctx);
+  }
+  hipLaunchKernelGGL(k_gr_reduce, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, ctx->stream, npair * nb, ctx->gr_count,
+                     ctx->gr_norm, ctx->gr_h);
   NNPE_CHECK_LAUNCH(ctx);
   return NNPE_OK;
 }

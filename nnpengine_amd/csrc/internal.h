@@ -101,6 +101,7 @@ struct nnpe_ctx {
   int *gs_src = nullptr, *gs_code = nullptr;                  // sorted by cell
   int4* stencil_pair = nullptr; int n_stencil_pair = 0;
   int4* stencil_gr = nullptr; int n_stencil_gr = 0;
+  int4* stencil_gr_outer = nullptr; int n_stencil_gr_outer = 0;  // g(r) stencil cells outside the pair stencil
   int cap_long = 0, cap_short = 0;
   uint32_t *nbr_long = nullptr, *nbr_short = nullptr;
   int *cnt_long = nullptr, *cnt_short = nullptr;
@@ -165,13 +166,18 @@ struct nnpe_ctx {
 };
 
 // enqueue helpers (no host synchronisation; safe inside stream capture)
-int neigh_enqueue_build(nnpe_ctx* ctx);
+int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally = false);
 int pair_enqueue(nnpe_ctx* ctx, bool ev);
 int bonded_enqueue(nnpe_ctx* ctx, bool ev);
 int verlet_enqueue_x(nnpe_ctx* ctx);
 int verlet_enqueue_v(nnpe_ctx* ctx);
 int ke_enqueue(nnpe_ctx* ctx);
 int gr_enqueue(nnpe_ctx* ctx, bool full);
+// full g(r) tally folded into a list rebuild (neigh.hip, grsq.hip): the rebuild counts the pairs
+// of its candidates, the outer cells follow in a shell pass
+bool gr_fusable(nnpe_ctx* ctx);
+int gr_zero_all_enqueue(nnpe_ctx* ctx);
+int gr_shell_finish_enqueue(nnpe_ctx* ctx);
 int sq_enqueue(nnpe_ctx* ctx);
 int nnp_tbl_desc_enqueue(nnpe_ctx* ctx);
 int nnp_tbl_force_enqueue(nnpe_ctx* ctx, bool ev);
@@ -340,6 +346,10 @@ __device__ __forceinline__ int xcd_block(int nblocks) {
 __device__ __forceinline__ const uint32_t* list_row(const uint32_t* nbr, int cap, int i) {
   return nbr + (size_t)i * cap;
 }
+
+// g(r) bin of a squared distance, int(sqrt(rsq) * 1/dr) in FP32 as the reference
+// (src/grsq.cpp:118-121): correctly rounded sqrt, one rounded product
+__device__ __forceinline__ int gr_bin_of(float rsq, float rdr) { return int(__fmul_rn(sqrtf(rsq), rdr)); }
 
 __device__ __forceinline__ int box_cell_coord(float g_offset, float x, float g_rdx) {
   return int(__fmul_rn(__fadd_rn(g_offset, x), g_rdx));

@@ -77,16 +77,18 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   int rc;
   hipStream_t s = ctx->stream;
   if (!first && (rc = verlet_enqueue_x(ctx))) return rc;
+  // a full g(r) tally on a rebuild step rides on the rebuild (same positions, fresh cells)
+  const bool gr_fused = (key & kRebuild) && (key & kGr) && (key & kGrFull) && gr_fusable(ctx);
   if (key & kRebuild) {
     if (!first && P.rebuild_every > 0) {
       hipLaunchKernelGGL(k_skin_check, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, s, ctx->natom, ctx->dx2, ctx->xq,
                          ctx->xprev, ctx->counters);
       hipLaunchKernelGGL(k_skin_accum, dim3(1), dim3(1), 0, s, ctx->counters);
     }
-    if ((rc = neigh_enqueue_build(ctx))) return rc;
+    if ((rc = neigh_enqueue_build(ctx, gr_fused))) return rc;
   }
   if (key & kGr) {
-    if ((rc = gr_enqueue(ctx, (key & kGrFull) != 0))) return rc;
+    if (!gr_fused && (rc = gr_enqueue(ctx, (key & kGrFull) != 0))) return rc;
     if (P.sq && (rc = sq_enqueue(ctx))) return rc;
     // RMDF compute_Qspace / compute_Rspace, and compute_Potential on logged steps when
     // gamma != 1 (src/run.cpp:294-298, :1206-1210)
@@ -251,6 +253,7 @@ extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* av
       case NNPE_STAGE_SF_ADJOINT: return sf_adjoint_enqueue(ctx);
       case NNPE_STAGE_SF_FORCE: return sf_force_only_enqueue(ctx, false);
       case NNPE_STAGE_REBUILD: return neigh_enqueue_build(ctx);
+      case NNPE_STAGE_REBUILD_GR: return neigh_enqueue_build(ctx, true);
       case NNPE_STAGE_GR_FULL: return ctx->gr_count ? gr_enqueue(ctx, true) : NNPE_ERR_STATE;
       case NNPE_STAGE_GR_LIST: return ctx->gr_count ? gr_enqueue(ctx, false) : NNPE_ERR_STATE;
       case NNPE_STAGE_VERLET: {

@@ -284,7 +284,7 @@ constexpr int kBuildHome = 64;     // home atoms per pass
 constexpr int kBuildMaxRanges = 2 * 343;
 constexpr int kShortMaxType = 4;   // short lists: one per neighbour element
 
-template <bool kShort>
+template <bool kShort, bool kGr>
 __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __restrict__ boxes,
                                                      const int* __restrict__ cell_box, const int4* __restrict__ stencil,
                                                      int nst, float cutsq, float cutsq_s,
@@ -296,7 +296,13 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
                                                      int ntype, int stride, int cap, int cap_s,
                                                      uint32_t* __restrict__ nbr, int* __restrict__ cnt_out,
                                                      uint32_t* __restrict__ nbr_s, int* __restrict__ cnt_s_out,
-                                                     float4* __restrict__ xprev, DevCounters* counters) {
+                                                     float4* __restrict__ xprev, DevCounters* counters,
+                                                     float gr_rmaxsq, float gr_rdr, int gr_nbin, int npair,
+                                                     unsigned long long* __restrict__ gr_count) {
+  // kGr: the full g(r) tally of the rebuild's positions rides along -- a real candidate pair
+  // within r_max adds 2 from its lower row, a ghost image 1 (2 per unordered pair, the
+  // reference's rule), into the workgroup's LDS histogram
+  extern __shared__ unsigned int s_gh[];  // [npair][gr_nbin] when kGr
   __shared__ int r_begin[kBuildMaxRanges], r_pref[kBuildMaxRanges + 1];
   __shared__ unsigned char r_flag[kBuildMaxRanges];  // bit 0: ghost, bit 1: no distance check
   __shared__ float4 c_pos[kBuildBatch];
@@ -312,6 +318,8 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
   if (hb == he) return;  // uniform across the workgroup
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid < 2) s_max[tid] = 0;
+  if (kGr)
+    for (int k = tid; k < npair * gr_nbin; k += blockDim.x) s_gh[k] = 0u;
   const BoxPar B = boxes[cell_box[c]];
   const float bl = B.bl, m_bl = B.m_bl;
   const int num_gx = B.num_gx, lc = c - B.cell_base;
@@ -365,6 +373,7 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
       for (int h = h0 + wave; h < h1; h += blockDim.x / 64) {
         const float4 xi = xq[h];
         const int4 ii = info[h];
+        const int ti_gr = kGr ? typ[h] : 0;
         const int mtn = mt_natom[ii.w];
         const int* sp_row = mt_sp + mt_off[ii.w] + ii.z * mtn;
         int nl = h_nl[h - h0];
@@ -390,6 +399,14 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
               const int sp = (mj.x == ii.y) ? sp_row[mj.y] : 0;
               e = (ce & 0x7FFFFFFFu) | ((uint32_t)sp << 30);
               tj = c_typ[u];
+              // a real pair once, from its lower row, with weight 2; a ghost image from each side
+              if (kGr && (ghost || j > h) && rsq < gr_rmaxsq) {
+                const int b = gr_bin_of(rsq, gr_rdr);
+                if (b < gr_nbin) {
+                  const int a = min(ti_gr, tj), bb = max(ti_gr, tj);
+                  atomicAdd(&s_gh[(a * ntype - (a * (a - 1)) / 2 + (bb - a)) * gr_nbin + b], ghost ? 1u : 2u);
+                }
+              }
             }
           }
           const unsigned long long bal = __ballot(take);
@@ -435,6 +452,13 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
   if (tid == 0) {
     atomicMax(&counters->max_long, s_max[0]);
     if (kShort) atomicMax(&counters->max_short, s_max[1]);
+  }
+  if (kGr) {
+    __syncthreads();  // every wave's histogram adds are done
+    for (int k = tid; k < npair * gr_nbin; k += blockDim.x) {
+      const unsigned int v = s_gh[k];
+      if (v) atomicAdd(&gr_count[k], (unsigned long long)v);
+    }
   }
 }
 
@@ -529,6 +553,17 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
   NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->stencil_pair, sp.data(), sp.size() * sizeof(int4), hipMemcpyHostToDevice));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->stencil_gr, sg.size()));
   NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->stencil_gr, sg.data(), sg.size() * sizeof(int4), hipMemcpyHostToDevice));
+  // g(r) cells outside the pair stencil: the shell pass of a tally folded into a rebuild
+  std::vector<int4> so;
+  for (const int4& g : sg) {
+    bool in = false;
+    for (const int4& q : sp) in = in || (q.x == g.x && q.y == g.y && q.z == g.z);
+    if (!in) so.push_back(g);
+  }
+  ctx->n_stencil_gr_outer = int(so.size());
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->stencil_gr_outer, std::max<size_t>(1, so.size())));
+  if (!so.empty())
+    NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->stencil_gr_outer, so.data(), so.size() * sizeof(int4), hipMemcpyHostToDevice));
   const int G = ctx->num_grid;
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_start, G + 1)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_cnt, G));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_fill, G));
@@ -598,7 +633,7 @@ int box_upload_single(nnpe_ctx* ctx) {
 }
 
 // Enqueue the rebuild kernels (no host synchronisation; graph-capturable).
-int neigh_enqueue_build(nnpe_ctx* ctx) {
+int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
   const int N = ctx->natom, G = ctx->num_grid, bs = 256;
   hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ctx->counters);
@@ -633,22 +668,31 @@ int neigh_enqueue_build(nnpe_ctx* ctx) {
                      ctx->mol_first, ctx->inv, ctx->xmol_code);
   if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
   if (ctx->cap_short && ctx->ntype > kShortMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "short lists: at most 4 elements");
-  if (ctx->cap_short) {
-    hipLaunchKernelGGL(k_build_cells<true>, dim3(G), dim3(256), 0, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair,
-                       ctx->n_stencil_pair, ctx->cutsq_list, ctx->cutsq_short, ctx->xq, ctx->info,
-                       ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
-                       ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, ctx->cap_short, ctx->nbr_long,
-                       ctx->cnt_long, ctx->nbr_short, ctx->cnt_short, ctx->xprev, ctx->counters);
-  } else {
-    hipLaunchKernelGGL(k_build_cells<false>, dim3(G), dim3(256), 0, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair,
-                       ctx->n_stencil_pair, ctx->cutsq_list, 0.0f, ctx->xq, ctx->info,
-                       ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,
-                       ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, 0, ctx->nbr_long,
-                       ctx->cnt_long, nullptr, nullptr, ctx->xprev, ctx->counters);
+  gr_tally = gr_tally && gr_fusable(ctx);
+  if (gr_tally) {
+    const int rc = gr_zero_all_enqueue(ctx);
+    if (rc) return rc;
   }
-
+  const size_t gh = gr_tally ? sizeof(unsigned int) * ctx->npair * ctx->gr_nbin : 0;
+#define NNPE_BUILD(SHORT, GR)                                                                                             \
+  hipLaunchKernelGGL((k_build_cells<SHORT, GR>), dim3(G), dim3(256), gh, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair, \
+                     ctx->n_stencil_pair, ctx->cutsq_list, SHORT ? ctx->cutsq_short : 0.0f, ctx->xq, ctx->info,            \
+                     ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,      \
+                     ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, SHORT ? ctx->cap_short : 0,          \
+                     ctx->nbr_long, ctx->cnt_long, SHORT ? ctx->nbr_short : nullptr, SHORT ? ctx->cnt_short : nullptr,      \
+                     ctx->xprev, ctx->counters, ctx->gr_rmaxsq, ctx->gr_rdr, ctx->gr_nbin, ctx->npair, ctx->gr_count)
+  if (ctx->cap_short) {
+    if (gr_tally) NNPE_BUILD(true, true); else NNPE_BUILD(true, false);
+  } else {
+    if (gr_tally) NNPE_BUILD(false, true); else NNPE_BUILD(false, false);
+  }
+#undef NNPE_BUILD
   NNPE_CHECK_LAUNCH(ctx);
   ctx->nbuild++;
+  if (gr_tally) {
+    const int rc = gr_shell_finish_enqueue(ctx);
+    if (rc) return rc;
+  }
   return type_index_enqueue(ctx);  // rows of each element moved with the sort
 }
 
