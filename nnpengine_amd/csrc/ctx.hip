@@ -65,7 +65,7 @@ static void free_all(nnpe_ctx* c) {
   dev_free(c->bcoef); dev_free(c->acoef); dev_free(c->dcoef);
   dev_free(c->bterm); dev_free(c->aterm); dev_free(c->dterm);
   dev_free(c->bstart); dev_free(c->astart); dev_free(c->dstart);
-  dev_free(c->gr_count); dev_free(c->gr_h); dev_free(c->gr_norm); dev_free(c->sq_out); dev_free(c->sq_ftmx);
+  dev_free(c->gr_count); dev_free(c->gr_shell); dev_free(c->gr_h); dev_free(c->gr_norm); dev_free(c->sq_out); dev_free(c->sq_ftmx);
   md_state_free(c);
   rmdf_state_free(c);
   sqd_state_free(c);

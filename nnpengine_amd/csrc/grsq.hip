@@ -22,6 +22,10 @@ constexpr int kHistLdsMax = 48 * 1024;
 constexpr int kHistLdsFused = 16 * 1024;  // histogram added to the rebuild kernel's LDS
 constexpr int kGrG = 16;  // lanes per atom, inner tally
 
+constexpr int kGrShellCopies = 32;
+
+__device__ __forceinline__ int npair_of(int ntype) { return ntype * (ntype + 1) / 2; }
+
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   const int i = min(a, b), j = max(a, b);  // small non-negative values: 24-bit multiplies
   return __mul24(i, ntype) - (__mul24(i, i - 1) >> 1) + (j - i);
@@ -195,6 +199,8 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
 // images): a real pair adds 2 from its lower row, a ghost image 1 (2 per unordered pair, the
 // reference's rule).
 // Positions and cells are those of the rebuild that just ran, so the cell boxes bound the atoms.
+// The shell's counts go to kGrShellCopies private copies (one per workgroup residue), folded
+// into the tally by k_gr_fold_shell.
 __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* __restrict__ outer,
                                                   const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
                                                   const int* __restrict__ cell, const float4* __restrict__ xq,
@@ -202,7 +208,9 @@ __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* _
                                                   const int* __restrict__ gstart, const int* __restrict__ gs_src,
                                                   const int* __restrict__ gs_code, int ntype, float rmaxsq,
                                                   float rcull_sq, float rdr, int nbin,
-                                                  unsigned long long* __restrict__ count) {
+                                                  unsigned long long* __restrict__ copies) {
+  // the few bins near r_max take every add: workgroups spread them over kGrShellCopies copies
+  unsigned long long* count = copies + (size_t)(blockIdx.x % kGrShellCopies) * (npair_of(ntype) * nbin);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)n * nout) return;
   const int i = int(t / nout), s = int(t - (long long)i * nout);
@@ -240,6 +248,17 @@ __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* _
     const int b = gr_bin(rsq, rdr);
     if (b < nbin) atomicAdd(&count[(size_t)pair_index(ti, typ[src], ntype) * nbin + b], 1ull);
   }
+}
+
+// count += the shell copies (integer sums: order-free), then h(r) as k_gr_reduce
+__global__ void k_gr_fold_shell(int total, unsigned long long* count, const unsigned long long* copies, const double* norm,
+                                double* h) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  unsigned long long c = count[k];
+  for (int r = 0; r < kGrShellCopies; ++r) c += copies[(size_t)r * total + k];
+  count[k] = c;
+  h[k] = double(c) * norm[k] - 1.0;
 }
 
 __global__ void k_gr_reduce(int total, const unsigned long long* count, const double* norm, double* h) {
@@ -294,6 +313,7 @@ extern "C" int nnpe_gr_init(nnpe_ctx* ctx, int nbi, float r_max, float r_max_inn
   const int npair = ctx->npair, nb = ctx->gr_nbin;
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gr_count, size_t(npair) * nb));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->gr_count, 0, sizeof(unsigned long long) * npair * nb));
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gr_shell, size_t(kGrShellCopies) * npair * nb));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gr_h, size_t(npair) * nb));
   // normalisation (src/grsq.cpp:56-70) in FP64
   std::vector<double> norm(size_t(npair) * nb);
@@ -356,12 +376,7 @@ bool gr_fusable(nnpe_ctx* ctx) {
   return ctx->gr_count && ctx->nbox == 1 && size_t(ctx->npair) * ctx->gr_nbin * sizeof(unsigned int) <= kHistLdsFused;
 }
 
-int gr_zero_all_enqueue(nnpe_ctx* ctx) {
-  const int npair = ctx->npair, nb = ctx->gr_nbin;
-  hipLaunchKernelGGL(k_gr_zero, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, ctx->stream, ctx->gr_count, npair, nb, nb);
-  NNPE_CHECK_LAUNCH(ctx);
-  return NNPE_OK;
-}
+int gr_shell_size(nnpe_ctx* ctx) { return kGrShellCopies * ctx->npair * ctx->gr_nbin; }
 
 int gr_shell_finish_enqueue(nnpe_ctx* ctx) {
   const int npair = ctx->npair, nb = ctx->gr_nbin;
@@ -371,11 +386,11 @@ int gr_shell_finish_enqueue(nnpe_ctx* ctx) {
     hipLaunchKernelGGL(k_gr_shell, dim3(blocks_for(nt, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->n_stencil_gr_outer,
                        ctx->stencil_gr_outer, ctx->d_box, ctx->rbox, ctx->cell, ctx->xq, ctx->typ, ctx->cell_start,
                        ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->ntype, ctx->gr_rmaxsq, rcull * rcull, ctx->gr_rdr, nb,
-                       ctx->gr_count);
+                       ctx->gr_shell);
     NNPE_CHECK_LAUNCH(ctx);
   }
-  hipLaunchKernelGGL(k_gr_reduce, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, ctx->stream, npair * nb, ctx->gr_count,
-                     ctx->gr_norm, ctx->gr_h);
+  hipLaunchKernelGGL(k_gr_fold_shell, dim3(blocks_for(npair * nb, 256)), dim3(256), 0, ctx->stream, npair * nb, ctx->gr_count,
+                     ctx->gr_shell, ctx->gr_norm, ctx->gr_h);
   NNPE_CHECK_LAUNCH(ctx);
   return NNPE_OK;
 }

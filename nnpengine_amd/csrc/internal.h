@@ -137,6 +137,7 @@ struct nnpe_ctx {
   int gr_nbin = 0, gr_nbin_inner = 0;
   float gr_dr = 0, gr_rdr = 0, gr_rmaxsq = 0, gr_rmaxsq_inner = 0;
   unsigned long long* gr_count = nullptr;  // [npair][gr_nbin]
+  unsigned long long* gr_shell = nullptr;  // [copies][npair][gr_nbin]: outer-shell adds of a fused tally
   double* gr_h = nullptr;
   double* gr_norm = nullptr;
   int sq_nq = 0;
@@ -176,7 +177,7 @@ int gr_enqueue(nnpe_ctx* ctx, bool full);
 // full g(r) tally folded into a list rebuild (neigh.hip, grsq.hip): the rebuild counts the pairs
 // of its candidates, the outer cells follow in a shell pass
 bool gr_fusable(nnpe_ctx* ctx);
-int gr_zero_all_enqueue(nnpe_ctx* ctx);
+int gr_shell_size(nnpe_ctx* ctx);  // entries of the shell copies (cleared by the rebuild's reset)
 int gr_shell_finish_enqueue(nnpe_ctx* ctx);
 int sq_enqueue(nnpe_ctx* ctx);
 int nnp_tbl_desc_enqueue(nnpe_ctx* ctx);

@@ -19,13 +19,15 @@ namespace {
 
 constexpr int kStencilNoCheck = 1;
 
-__global__ void k_zero_i32(int* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-__global__ void k_reset_counters(DevCounters* c) {
-  if (threadIdx.x == 0) {
+// the rebuild's counters and per-cell counts cleared in one launch, and the g(r) counts of a
+// tally folded into the rebuild (gr_n == 0 otherwise)
+__global__ void k_build_reset(int G, int* cell_cnt, int* cell_fill, int* gcell_cnt, int* gcell_fill, DevCounters* c,
+                              int gr_n, unsigned long long* gr_count, int gr_shell_n, unsigned long long* gr_shell) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < G) { cell_cnt[i] = 0; cell_fill[i] = 0; gcell_cnt[i] = 0; gcell_fill[i] = 0; }
+  if (i < gr_n) gr_count[i] = 0ull;
+  if (i < gr_shell_n) gr_shell[i] = 0ull;
+  if (i == 0) {
     c->nghost = 0; c->overflow_long = 0; c->overflow_short = 0; c->ghost_overflow = 0;
     c->cell_range = 0; c->max_long = 0; c->max_short = 0;
   }
@@ -148,9 +150,39 @@ __global__ void k_copy_back(int n, bool fp64, const float4* xq2, const float4* v
   if (fp64) { xd[i] = xd2[i]; vd[i] = vd2[i]; ad[i] = ad2[i]; }
 }
 
+// LOCAL::update_ghost_and_info x_mol part (src/environ.cpp:461-469): the image of an atom whose
+// every coordinate lies within L/2 of its molecule's intra-0 atom, among the images the atom
+// has ghosts for; the atom itself otherwise.
+__device__ int xmol_code_of(int i, const BoxPar& B, const float4* xq, const int4* info, const int* mol_first,
+                            const int* inv) {
+  const float ghost_lim = B.ghost_lim, bl = B.bl, m_bl = B.m_bl, h_bl = B.h_bl;
+  int4 in = info[i];
+  int code_sel = 13;
+  if (in.z != 0) {
+    float4 p = xq[i];
+    float4 r = xq[inv[mol_first[in.y]]];
+    const float mg = -ghost_lim;
+    int xs = p.x > ghost_lim ? -1 : 0, xe = p.x < mg ? 1 : 0;
+    int ys = p.y > ghost_lim ? -1 : 0, ye = p.y < mg ? 1 : 0;
+    int zs = p.z > ghost_lim ? -1 : 0, ze = p.z < mg ? 1 : 0;
+    for (int a = xs; a <= xe; ++a)
+      for (int b = ys; b <= ye; ++b)
+        for (int c = zs; c <= ze; ++c) {
+          if (a == 0 && b == 0 && c == 0) continue;
+          int code = (a + 1) * 9 + (b + 1) * 3 + (c + 1);
+          float4 g = image_pos(p, code, bl, m_bl);
+          if (fabsf(__fsub_rn(g.x, r.x)) < h_bl && fabsf(__fsub_rn(g.y, r.y)) < h_bl && fabsf(__fsub_rn(g.z, r.z)) < h_bl)
+            code_sel = code;
+        }
+  }
+  return code_sel;
+}
+
 // LOCAL::generate_ghost + update_ghost_pos<true> (src/environ.cpp:401-442, :488-514)
 __global__ void k_ghost_gen(int n, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox, int cap,
-                            const float4* xq, int* g_src, int* g_code, int* g_cell, int* gcell_cnt, DevCounters* cnt) {
+                            const float4* xq, int* g_src, int* g_code, int* g_cell, int* gcell_cnt, DevCounters* cnt,
+                            const int4* __restrict__ info, const int* __restrict__ mol_first, const int* __restrict__ inv,
+                            int* __restrict__ xmol_code) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   BoxPar B{};
@@ -177,6 +209,7 @@ __global__ void k_ghost_gen(int n, const BoxPar* __restrict__ boxes, const int* 
   int base = 0;
   if (lane == 0 && wtot > 0) base = atomicAdd(&cnt->nghost, wtot);
   base = __shfl(base, 0, 64);
+  if (i < n) xmol_code[i] = xmol_code_of(i, B, xq, info, mol_first, inv);
   if (nimg == 0) return;
   int slot = base + inc - nimg;
   const float bl = B.bl, m_bl = B.m_bl, g_offset = B.g_offset, g_rdx = B.g_rdx;
@@ -239,36 +272,6 @@ __global__ void __launch_bounds__(256) k_ghost_cell_sort(int ncell, const int* g
   }
 }
 
-// LOCAL::update_ghost_and_info x_mol part (src/environ.cpp:461-469): the image of an atom whose
-// every coordinate lies within L/2 of its molecule's intra-0 atom, among the images the atom
-// has ghosts for; the atom itself otherwise.
-__global__ void k_xmol(int n, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox, const float4* xq,
-                       const int4* info, const int* mol_first, const int* inv, int* xmol_code) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const BoxPar B = boxes[rbox[i]];
-  const float ghost_lim = B.ghost_lim, bl = B.bl, m_bl = B.m_bl, h_bl = B.h_bl;
-  int4 in = info[i];
-  int code_sel = 13;
-  if (in.z != 0) {
-    float4 p = xq[i];
-    float4 r = xq[inv[mol_first[in.y]]];
-    const float mg = -ghost_lim;
-    int xs = p.x > ghost_lim ? -1 : 0, xe = p.x < mg ? 1 : 0;
-    int ys = p.y > ghost_lim ? -1 : 0, ye = p.y < mg ? 1 : 0;
-    int zs = p.z > ghost_lim ? -1 : 0, ze = p.z < mg ? 1 : 0;
-    for (int a = xs; a <= xe; ++a)
-      for (int b = ys; b <= ye; ++b)
-        for (int c = zs; c <= ze; ++c) {
-          if (a == 0 && b == 0 && c == 0) continue;
-          int code = (a + 1) * 9 + (b + 1) * 3 + (c + 1);
-          float4 g = image_pos(p, code, bl, m_bl);
-          if (fabsf(__fsub_rn(g.x, r.x)) < h_bl && fabsf(__fsub_rn(g.y, r.y)) < h_bl && fabsf(__fsub_rn(g.z, r.z)) < h_bl)
-            code_sel = code;
-        }
-  }
-  xmol_code[i] = code_sel;
-}
 
 // NEIGHLIST::build (src/neighlist.cpp:190-331) as a full list: every real neighbour j != i and
 // every ghost image within (rc+skin), or in a stencil cell flagged "no check", with the
@@ -283,6 +286,105 @@ constexpr int kBuildBatch = 1024;  // candidates staged per pass
 constexpr int kBuildHome = 64;     // home atoms per pass
 constexpr int kBuildMaxRanges = 2 * 343;
 constexpr int kShortMaxType = 4;   // short lists: one per neighbour element
+
+template <bool kGr>
+struct BuildRowArgs {
+  float4 xi;
+  int mol_i, h, tn;
+  float cutsq, cutsq_s;
+  int ntype, stride, cap, cap_s;
+  uint32_t* rowl;
+  uint32_t* nbr_s;
+  int ti_gr;
+  float gr_rmaxsq, gr_rdr;
+  int gr_nbin;
+  unsigned int* s_gh;
+};
+
+// One home atom against the staged candidates: 64 candidates per step, ballot compaction into
+// the full row and the per-element short rows.  kBigMol: molecules of more than 64 atoms read
+// the special class from the molecule type's table; smaller ones take it from the lane masks
+// (bit 0 of class in sp0, bit 1 in sp1, indexed by intra id).
+template <bool kShort, bool kGr, bool kBigMol, int NT>
+__device__ __forceinline__ void build_row(const BuildRowArgs<kGr>& A, const float4* c_pos, const uint32_t* c_ent,
+                                          const uint32_t* c_aux, const int* sp_row, unsigned long long sp0,
+                                          unsigned long long sp1, int lane, int& nl, int (&ns)[kShortMaxType]) {
+  // short rows of the NT elements: wave-uniform base pointers
+  uint32_t* srow[NT];
+#pragma unroll
+  for (int el = 0; el < NT; ++el) srow[el] = A.nbr_s + ((size_t)el * A.stride + A.h) * A.cap_s;
+  for (int u0 = 0; u0 < A.tn; u0 += 64) {
+    // every lane reads and tests its slot (u < kBuildBatch: u0 <= tn - 1 rounded down to 64);
+    // slots past tn hold stale candidates and are masked, so the step has no branch around
+    // the LDS reads and the compiler issues them together
+    const int u = u0 + lane;
+    const uint32_t ce = c_ent[u];
+    const float4 xj = c_pos[u];
+    const uint32_t ax = c_aux[u];
+    const int j = NNPE_ENTRY_J(ce);
+    const bool ghost = NNPE_ENTRY_CODE(ce) != 13;
+    const bool valid = (u < A.tn) & (ghost | (j != A.h));
+    const float rsq = dist2_rn(__fsub_rn(A.xi.x, xj.x), __fsub_rn(A.xi.y, xj.y), __fsub_rn(A.xi.z, xj.z));
+    const bool take = valid & (((ce & 0x80000000u) != 0) | (rsq < A.cutsq));
+    const bool take_s = kShort & take & (rsq < A.cutsq_s);
+    const int tj = (int)(ax >> 24);
+    const int jm = (int)(ax & 0xFFFFFFu);
+    const bool same_mol = valid & (__float_as_int(xj.w) == A.mol_i);
+    int sp = 0;
+    if (kBigMol) {
+      if (same_mol) sp = sp_row[jm];
+    } else {
+      // intra ids of the home molecule are < 64; other lanes' shifts are discarded
+      sp = same_mol ? (int)(((sp0 >> (jm & 63)) & 1ull) | (((sp1 >> (jm & 63)) & 1ull) << 1)) : 0;
+    }
+    const uint32_t e = (ce & 0x7FFFFFFFu) | ((uint32_t)sp << 30);
+    // a real pair once, from its lower row, with weight 2; a ghost image from each side
+    if (kGr && valid && (ghost || j > A.h) && rsq < A.gr_rmaxsq) {
+      const int b = gr_bin_of(rsq, A.gr_rdr);
+      if (b < A.gr_nbin) {
+        const int a = min(A.ti_gr, tj), bb = max(A.ti_gr, tj);
+        atomicAdd(&A.s_gh[(a * A.ntype - (a * (a - 1)) / 2 + (bb - a)) * A.gr_nbin + b], ghost ? 1u : 2u);
+      }
+    }
+    const unsigned long long bal = __ballot(take);
+    const unsigned rank = (unsigned)nl + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+    if (take && rank < (unsigned)A.cap) A.rowl[rank] = e;
+    nl += __popcll(bal);
+    if (kShort) {
+      // one ballot for the short test, split by the chunk's element masks: each lane ranks
+      // itself within its own element, and one store covers every element's row
+      const unsigned long long bs = __ballot(take_s);
+      unsigned long long em[NT];
+#pragma unroll
+      for (int el = 0; el < NT; ++el) em[el] = __ballot(tj == el);
+      unsigned long long mine = em[0];
+      int base = ns[0];
+      uint32_t* row = srow[0];
+#pragma unroll
+      for (int el = 1; el < NT; ++el)
+        if (tj == el) { mine = em[el]; base = ns[el]; row = srow[el]; }
+      const unsigned long long bm = bs & mine;
+      const unsigned rs = (unsigned)base + __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0));
+      if (take_s && rs < (unsigned)A.cap_s) row[rs] = e;
+#pragma unroll
+      for (int el = 0; el < NT; ++el) ns[el] += __popcll(bs & em[el]);
+    }
+  }
+}
+
+// the short rows' element count as a template argument (1 when there are no short rows)
+template <bool kShort, bool kGr, bool kBigMol>
+__device__ __forceinline__ void build_row_nt(const BuildRowArgs<kGr>& A, const float4* c_pos, const uint32_t* c_ent,
+                                             const uint32_t* c_aux, const int* sp_row, unsigned long long sp0,
+                                             unsigned long long sp1, int lane, int& nl, int (&ns)[kShortMaxType]) {
+  if (!kShort) { build_row<false, kGr, kBigMol, 1>(A, c_pos, c_ent, c_aux, sp_row, sp0, sp1, lane, nl, ns); return; }
+  switch (A.ntype) {
+    case 1: build_row<kShort, kGr, kBigMol, 1>(A, c_pos, c_ent, c_aux, sp_row, sp0, sp1, lane, nl, ns); break;
+    case 2: build_row<kShort, kGr, kBigMol, 2>(A, c_pos, c_ent, c_aux, sp_row, sp0, sp1, lane, nl, ns); break;
+    case 3: build_row<kShort, kGr, kBigMol, 3>(A, c_pos, c_ent, c_aux, sp_row, sp0, sp1, lane, nl, ns); break;
+    default: build_row<kShort, kGr, kBigMol, 4>(A, c_pos, c_ent, c_aux, sp_row, sp0, sp1, lane, nl, ns); break;
+  }
+}
 
 template <bool kShort, bool kGr>
 __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __restrict__ boxes,
@@ -305,10 +407,9 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
   extern __shared__ unsigned int s_gh[];  // [npair][gr_nbin] when kGr
   __shared__ int r_begin[kBuildMaxRanges], r_pref[kBuildMaxRanges + 1];
   __shared__ unsigned char r_flag[kBuildMaxRanges];  // bit 0: ghost, bit 1: no distance check
-  __shared__ float4 c_pos[kBuildBatch];
+  __shared__ float4 c_pos[kBuildBatch];              // x, y, z, molecule id (int bits)
   __shared__ uint32_t c_ent[kBuildBatch];            // row | code << 25 | nocheck << 31
-  __shared__ int2 c_mol[kBuildBatch];                // molecule id, intra id
-  __shared__ unsigned char c_typ[kBuildBatch];
+  __shared__ uint32_t c_aux[kBuildBatch];            // intra id | element << 24
   __shared__ int h_nl[kBuildHome], h_ns[kBuildHome][kShortMaxType];
   __shared__ int s_wsum[256 / 64 + 1];
   __shared__ int s_max[2];  // longest full and short row of this workgroup
@@ -316,7 +417,10 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
   if (c >= ncell) return;
   const int hb = cstart[c], he = cstart[c + 1];
   if (hb == he) return;  // uniform across the workgroup
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // wave index as a wave-uniform value: the home atom, its row pointers and counters then live
+  // in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (tid < 2) s_max[tid] = 0;
   if (kGr)
     for (int k = tid; k < npair * gr_nbin; k += blockDim.x) s_gh[k] = 0u;
@@ -364,13 +468,12 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
         if (fl & 1) { row = gs_src[pos]; code = gs_code[pos]; p = image_pos(xq[row], code, bl, m_bl); }
         else { row = pos; code = 13; p = xq[pos]; }
         const int4 in = info[row];
-        c_pos[u] = p;
+        c_pos[u] = make_float4(p.x, p.y, p.z, __int_as_float(in.y));
         c_ent[u] = NNPE_MAKE_ENTRY(row, code, 0) | ((fl & 2) ? 0x80000000u : 0u);
-        c_mol[u] = make_int2(in.y, in.z);
-        c_typ[u] = (unsigned char)typ[row];
+        c_aux[u] = (uint32_t)in.z | ((uint32_t)typ[row] << 24);
       }
       __syncthreads();
-      for (int h = h0 + wave; h < h1; h += blockDim.x / 64) {
+      for (int h = h0 + wave; h < h1; h += 256 / 64) {
         const float4 xi = xq[h];
         const int4 ii = info[h];
         const int ti_gr = kGr ? typ[h] : 0;
@@ -381,47 +484,16 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
 #pragma unroll
         for (int e = 0; e < kShortMaxType; ++e) ns[e] = h_ns[h - h0][e];
         uint32_t* rowl = nbr + (size_t)h * cap;
-        for (int u0 = 0; u0 < tn; u0 += 64) {
-          const int u = u0 + lane;
-          bool take = false, take_s = false;
-          uint32_t e = 0;
-          int tj = 0;
-          if (u < tn) {
-            const uint32_t ce = c_ent[u];
-            const int j = NNPE_ENTRY_J(ce);
-            const bool ghost = NNPE_ENTRY_CODE(ce) != 13;
-            if (ghost || j != h) {
-              const float4 xj = c_pos[u];
-              const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
-              take = (ce & 0x80000000u) || rsq < cutsq;
-              take_s = kShort && take && rsq < cutsq_s;
-              const int2 mj = c_mol[u];
-              const int sp = (mj.x == ii.y) ? sp_row[mj.y] : 0;
-              e = (ce & 0x7FFFFFFFu) | ((uint32_t)sp << 30);
-              tj = c_typ[u];
-              // a real pair once, from its lower row, with weight 2; a ghost image from each side
-              if (kGr && (ghost || j > h) && rsq < gr_rmaxsq) {
-                const int b = gr_bin_of(rsq, gr_rdr);
-                if (b < gr_nbin) {
-                  const int a = min(ti_gr, tj), bb = max(ti_gr, tj);
-                  atomicAdd(&s_gh[(a * ntype - (a * (a - 1)) / 2 + (bb - a)) * gr_nbin + b], ghost ? 1u : 2u);
-                }
-              }
-            }
-          }
-          const unsigned long long bal = __ballot(take);
-          const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-          if (take && nl + rank < cap) rowl[nl + rank] = e;
-          nl += __popcll(bal);
-          if (kShort) {
-            for (int el = 0; el < ntype; ++el) {
-              const bool t = take_s && tj == el;
-              const unsigned long long bs = __ballot(t);
-              const int rs = __builtin_amdgcn_mbcnt_hi((unsigned)(bs >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bs, 0));
-              if (t && ns[el] + rs < cap_s) nbr_s[((size_t)el * stride + h) * cap_s + ns[el] + rs] = e;
-              ns[el] += __popcll(bs);
-            }
-          }
+        BuildRowArgs<kGr> A{xi, ii.y, h, tn, cutsq, cutsq_s, ntype, stride, cap, cap_s, rowl, nbr_s, ti_gr,
+                            gr_rmaxsq, gr_rdr, gr_nbin, s_gh};
+        if (mtn <= 64) {
+          // the home atom's special classes against its molecule, as two lane masks: no memory
+          // load in the candidate loop, so no wait on the row stores already in flight
+          const int v = lane < mtn ? sp_row[lane] : 0;
+          const unsigned long long b0 = __ballot(v & 1), b1 = __ballot(v & 2);
+          build_row_nt<kShort, kGr, false>(A, c_pos, c_ent, c_aux, nullptr, b0, b1, lane, nl, ns);
+        } else {
+          build_row_nt<kShort, kGr, true>(A, c_pos, c_ent, c_aux, sp_row, 0ull, 0ull, lane, nl, ns);
         }
         if (lane == 0) {
           h_nl[h - h0] = nl;
@@ -636,11 +708,15 @@ int box_upload_single(nnpe_ctx* ctx) {
 int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
   const int N = ctx->natom, G = ctx->num_grid, bs = 256;
   hipStream_t s = ctx->stream;
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ctx->counters);
-  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->cell_cnt, G);
-  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->cell_fill, G);
-  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->gcell_cnt, G);
-  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(G, bs)), dim3(bs), 0, s, ctx->gcell_fill, G);
+  if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
+  if (ctx->cap_short && ctx->ntype > kShortMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "short lists: at most 4 elements");
+  if (ctx->ntype > 255) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "list build: at most 255 elements");
+  gr_tally = gr_tally && gr_fusable(ctx);
+  const int gr_n = gr_tally ? ctx->npair * ctx->gr_nbin : 0;
+  const int gr_shell_n = gr_tally ? gr_shell_size(ctx) : 0;
+  hipLaunchKernelGGL(k_build_reset, dim3(blocks_for(std::max(G, gr_shell_n), bs)), dim3(bs), 0, s, G, ctx->cell_cnt,
+                     ctx->cell_fill, ctx->gcell_cnt, ctx->gcell_fill, ctx->counters, gr_n, ctx->gr_count, gr_shell_n,
+                     ctx->gr_shell);
   hipLaunchKernelGGL(k_refold, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->d_box, ctx->rbox, ctx->xq, ctx->xd,
                      ctx->t_cell, ctx->cell_cnt, ctx->counters);
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->cell_cnt, ctx->cell_start, G);
@@ -658,21 +734,13 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
                      ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->rbox);
   NNPE_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_ghost_gen, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->d_box, ctx->rbox, ctx->ghost_cap, ctx->xq,
-                     ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_cnt, ctx->counters);
+                     ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_cnt, ctx->counters, ctx->info, ctx->mol_first,
+                     ctx->inv, ctx->xmol_code);
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->gcell_cnt, ctx->gcell_start, G);
   hipLaunchKernelGGL(k_ghost_scatter, dim3(blocks_for(ctx->ghost_cap, bs)), dim3(bs), 0, s, ctx->ghost_cap, ctx->counters,
                      ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_start, ctx->gcell_fill, ctx->gs_src, ctx->gs_code);
   hipLaunchKernelGGL(k_ghost_cell_sort, dim3(blocks_for((long long)G * 64, 256)), dim3(256), 0, s, G, ctx->gcell_start,
                      ctx->gs_src, ctx->gs_code, ctx->info);
-  hipLaunchKernelGGL(k_xmol, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->d_box, ctx->rbox, ctx->xq, ctx->info,
-                     ctx->mol_first, ctx->inv, ctx->xmol_code);
-  if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
-  if (ctx->cap_short && ctx->ntype > kShortMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "short lists: at most 4 elements");
-  gr_tally = gr_tally && gr_fusable(ctx);
-  if (gr_tally) {
-    const int rc = gr_zero_all_enqueue(ctx);
-    if (rc) return rc;
-  }
   const size_t gh = gr_tally ? sizeof(unsigned int) * ctx->npair * ctx->gr_nbin : 0;
 #define NNPE_BUILD(SHORT, GR)                                                                                             \
   hipLaunchKernelGGL((k_build_cells<SHORT, GR>), dim3(G), dim3(256), gh, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair, \
