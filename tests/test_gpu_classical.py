@@ -50,6 +50,68 @@ def test_list_bit_exact(case, water_c0, water_c1):
     assert np.array_equal(e.get("x"), o.get("x"))
 
 
+def _chain_system(base, lens):
+    """The base water box with its first sum(lens) atoms regrouped into linear chain molecules of
+    the given lengths (bonds, angles and dihedrals along the chain): molecules larger and
+    smaller than 64 atoms in one system, for the two ways the list build looks up special
+    classes."""
+    n_ch = int(sum(lens))
+    assert n_ch % 3 == 0
+    moltypes = [base.moltypes[0]]
+    kind = {}
+    for L in sorted(set(lens)):
+        kind[L] = len(moltypes)
+        moltypes.append(S.MolType(L, bonds=[(0, k, k + 1) for k in range(L - 1)],
+                                  angles=[(0, k, k + 1, k + 2) for k in range(L - 2)],
+                                  dihedrals=[(0, k, k + 1, k + 2, k + 3) for k in range(L - 3)]))
+    mol_id, intra, mtyp = [], [], []
+    for m, L in enumerate(lens):
+        mol_id += [m] * L
+        intra += list(range(L))
+        mtyp.append(kind[L])
+    nw = (base.natom - n_ch) // 3
+    mol_id += list(np.repeat(np.arange(len(lens), len(lens) + nw), 3))
+    intra += [0, 1, 2] * nw
+    mtyp += [0] * nw
+    s = base.copy()
+    s.mol_id = np.asarray(mol_id, np.int32)
+    s.intra_id = np.asarray(intra, np.int32)
+    s.mol_type = np.asarray(mtyp, np.int32)
+    s.moltypes = moltypes
+    return s
+
+
+def test_list_bit_exact_large_molecules(water_c0):
+    sysm = _chain_system(water_c0, [81, 63, 81, 63])
+    print(f"[list] chains 81/63 natom={sysm.natom}", flush=True)
+    e = _engine(sysm, RC, SKIN, RC, 6.0)
+    e.build()
+    o = _oracle(sysm, 0)
+    o.rebuild(False)
+    full = e.list_entries()
+    kg = pyoracle.canonical(full)
+    ko = pyoracle.canonical(o.list_entries())
+    assert np.array_equal(kg, ko)
+    # special classes 1..3 of chain atoms on both sides of the 64-atom boundary
+    for lo, hi in ((0, 81), (81, 144)):
+        m = (full[:, 0] >= lo) & (full[:, 0] < hi)
+        assert set(np.unique(full[m, 3])) >= {0, 1, 2, 3}
+    # short rows: exactly the full-list entries within (6 + skin), by the same FP32 operations
+    # (image shift, differences, squared distance summed x, y, z)
+    x = e.get("x").astype(np.float32)
+    bl = np.float32(sysm.bl)
+    i, j, code = full[:, 0], full[:, 1], full[:, 2]
+    shift = np.array([-bl, np.float32(0), bl], np.float32)
+    img = x[j] + np.stack([shift[code // 9], shift[(code % 9) // 3], shift[code % 3]], 1)
+    img[code == 13] = x[j[code == 13]]
+    d = x[i] - img
+    rsq = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    sc = np.float32(6.0) + np.float32(SKIN)
+    want = pyoracle.canonical(full[rsq < sc * sc])
+    got = pyoracle.canonical(e.list_entries(1))
+    assert len(got) > 0 and np.array_equal(got, want)
+
+
 def test_pair_bonded_forces(water_c0):
     sysm = water_c0
     ft, ut = S.lj_table(sysm, RC, 900, S.WATER_LJ)
