@@ -262,20 +262,30 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
       // list order, into the active list and into the group's LDS buffer as (u, |u|^2).
       // Phase 2: all lanes accumulate the buffered pairs, so the exp-heavy moment arithmetic
       // runs on full lanes instead of under the scan's partial mask.
+      static_assert(kSfBuf >= kSfU * G, "the scan must fit one step into the pair buffer");
+      // entries of the next step are loaded while this step's positions are gathered and tested
+      uint32_t en[kSfU];
+#pragma unroll
+      for (int u = 0; u < kSfU; ++u) {
+        const int q = u * G + lane;
+        en[u] = q < cn ? row[q] : 0u;
+      }
       for (int q0 = 0; q0 < cn;) {
         int nb = 0;
         for (; q0 < cn && nb + kSfU * G <= kSfBuf; q0 += kSfU * G) {
           uint32_t ec[kSfU];
           float4 xr[kSfU];
 #pragma unroll
-          for (int u = 0; u < kSfU; ++u) {
-            const int q = q0 + u * G + lane;
-            ec[u] = q < cn ? row[q] : 0u;
-          }
+          for (int u = 0; u < kSfU; ++u) ec[u] = en[u];
 #pragma unroll
           for (int u = 0; u < kSfU; ++u) {
             const int q = q0 + u * G + lane;
             xr[u] = xq[q < cn ? (int)NNPE_ENTRY_J(ec[u]) : i];
+          }
+#pragma unroll
+          for (int u = 0; u < kSfU; ++u) {
+            const int q = q0 + (kSfU + u) * G + lane;
+            en[u] = q < cn ? row[q] : 0u;
           }
 #pragma unroll
           for (int u = 0; u < kSfU; ++u) {
