@@ -194,13 +194,14 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
 
 // Shell pass of the full tally folded into a list rebuild: the rebuild counted every pair of its
 // own candidates (the pair stencil); the pairs within r_max whose cells lie outside it are in
-// the outer stencil cells.  Thread per (atom, outer cell): cells whose box lies farther than
-// r_max (plus a margin) from the atom are skipped, the rest are walked (real rows, then ghost
-// images): a real pair adds 2 from its lower row, a ghost image 1 (2 per unordered pair, the
-// reference's rule).
+// the outer stencil cells (on the device only those a point of the home cell can reach).
+// Lane per (atom, outer cell): cells whose box lies farther than r_max (plus a margin) from the
+// atom, or that hold no candidate, are dropped; the wave then takes its remaining items one at a
+// time, its 64 lanes walking the item cell's real rows above the atom (weight 2) and its ghost
+// images (weight 1) -- 2 per unordered pair, the reference's rule.  Counts go to
+// kGrShellCopies private copies (the few bins near r_max take every add), folded into the tally
+// by k_gr_fold_shell.
 // Positions and cells are those of the rebuild that just ran, so the cell boxes bound the atoms.
-// The shell's counts go to kGrShellCopies private copies (one per workgroup residue), folded
-// into the tally by k_gr_fold_shell.
 __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* __restrict__ outer,
                                                   const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
                                                   const int* __restrict__ cell, const float4* __restrict__ xq,
@@ -209,44 +210,59 @@ __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* _
                                                   const int* __restrict__ gs_code, int ntype, float rmaxsq,
                                                   float rcull_sq, float rdr, int nbin,
                                                   unsigned long long* __restrict__ copies) {
-  // the few bins near r_max take every add: workgroups spread them over kGrShellCopies copies
   unsigned long long* count = copies + (size_t)(blockIdx.x % kGrShellCopies) * (npair_of(ntype) * nbin);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)n * nout) return;
-  const int i = int(t / nout), s = int(t - (long long)i * nout);
-  const BoxPar B = boxes[rbox[i]];
-  const int num_gx = B.num_gx, lc = cell[i] - B.cell_base;
-  const int cz = lc % num_gx, cy = (lc / num_gx) % num_gx, cx = lc / (num_gx * num_gx);
-  const int4 st = outer[s];
-  const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
-  if (ax < 0 || ay < 0 || az < 0 || ax >= num_gx || ay >= num_gx || az >= num_gx) return;
-  const float4 xi = xq[i];
-  const float gdx = 1.0f / B.g_rdx;
-  // distance from the atom to the cell box [k gdx - off, (k+1) gdx - off) along one axis
-  auto gap = [&](int d, int k, float x) -> float {
-    if (d > 0) return fmaxf(0.0f, float(k) * gdx - B.g_offset - x);
-    if (d < 0) return fmaxf(0.0f, x - (float(k + 1) * gdx - B.g_offset));
-    return 0.0f;
-  };
-  const float gx = gap(st.x, ax, xi.x), gy = gap(st.y, ay, xi.y), gz = gap(st.z, az, xi.z);
-  if (gx * gx + gy * gy + gz * gz > rcull_sq) return;
-  const int nc = B.cell_base + (ax * num_gx + ay) * num_gx + az;
-  const int ti = typ[i];
-  const float bl = B.bl, m_bl = B.m_bl;
-  for (int j = max(cstart[nc], i + 1); j < cstart[nc + 1]; ++j) {
-    const float4 xj = xq[j];
-    const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
-    if (!(rsq < rmaxsq)) continue;
-    const int b = gr_bin(rsq, rdr);
-    if (b < nbin) atomicAdd(&count[(size_t)pair_index(ti, typ[j], ntype) * nbin + b], 2ull);
+  const int lane = threadIdx.x & 63;
+  bool keep = false;
+  int i = 0, nc = 0;
+  if (t < (long long)n * nout) {
+    i = int(t / nout);
+    const int s = int(t - (long long)i * nout);
+    const BoxPar B = boxes[rbox[i]];
+    const int num_gx = B.num_gx, lc = cell[i] - B.cell_base;
+    const int cz = lc % num_gx, cy = (lc / num_gx) % num_gx, cx = lc / (num_gx * num_gx);
+    const int4 st = outer[s];
+    const int ax = cx + st.x, ay = cy + st.y, az = cz + st.z;
+    if (ax >= 0 && ay >= 0 && az >= 0 && ax < num_gx && ay < num_gx && az < num_gx) {
+      const float4 xi = xq[i];
+      const float gdx = 1.0f / B.g_rdx;
+      // distance from the atom to the cell box [k gdx - off, (k+1) gdx - off) along one axis
+      auto gap = [&](int d, int k, float x) -> float {
+        if (d > 0) return fmaxf(0.0f, float(k) * gdx - B.g_offset - x);
+        if (d < 0) return fmaxf(0.0f, x - (float(k + 1) * gdx - B.g_offset));
+        return 0.0f;
+      };
+      const float gx = gap(st.x, ax, xi.x), gy = gap(st.y, ay, xi.y), gz = gap(st.z, az, xi.z);
+      nc = B.cell_base + (ax * num_gx + ay) * num_gx + az;
+      keep = gx * gx + gy * gy + gz * gz <= rcull_sq &&
+             (cstart[nc + 1] > max(cstart[nc], i + 1) || gstart[nc + 1] > gstart[nc]);
+    }
   }
-  for (int g = gstart[nc]; g < gstart[nc + 1]; ++g) {
-    const int src = gs_src[g];
-    const float4 xj = image_pos(xq[src], gs_code[g], bl, m_bl);
-    const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
-    if (!(rsq < rmaxsq)) continue;
-    const int b = gr_bin(rsq, rdr);
-    if (b < nbin) atomicAdd(&count[(size_t)pair_index(ti, typ[src], ntype) * nbin + b], 1ull);
+  unsigned long long bal = __ballot(keep);
+  while (bal) {  // wave-uniform: every lane holds the same mask
+    const int src = __ffsll((long long)bal) - 1;
+    bal &= bal - 1ull;
+    const int ii = __shfl(i, src, 64), cc = __shfl(nc, src, 64);
+    const float4 xi = xq[ii];
+    const int ti = typ[ii];
+    const BoxPar& B = boxes[rbox[ii]];
+    const int j0 = max(cstart[cc], ii + 1), nr = max(0, cstart[cc + 1] - j0);
+    const int g0 = gstart[cc], ng = gstart[cc + 1] - g0;
+    for (int k = lane; k < nr + ng; k += 64) {
+      float4 xj;
+      int tj;
+      unsigned long long wgt;
+      if (k < nr) {
+        xj = xq[j0 + k]; tj = typ[j0 + k]; wgt = 2ull;
+      } else {
+        const int g = g0 + (k - nr), sr = gs_src[g];
+        xj = image_pos(xq[sr], gs_code[g], B.bl, B.m_bl); tj = typ[sr]; wgt = 1ull;
+      }
+      const float rsq = dist2_rn(__fsub_rn(xi.x, xj.x), __fsub_rn(xi.y, xj.y), __fsub_rn(xi.z, xj.z));
+      if (!(rsq < rmaxsq)) continue;
+      const int b = gr_bin(rsq, rdr);
+      if (b < nbin) atomicAdd(&count[(size_t)pair_index(ti, tj, ntype) * nbin + b], wgt);
+    }
   }
 }
 
@@ -329,7 +345,7 @@ extern "C" int nnpe_gr_init(nnpe_ctx* ctx, int nbi, float r_max, float r_max_inn
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->gr_norm, norm.size()));
   NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->gr_norm, norm.data(), norm.size() * sizeof(double), hipMemcpyHostToDevice));
   if (nbin_total) *nbin_total = nb;
-  return NNPE_OK;
+  return gr_outer_refresh(ctx);
 }
 
 int gr_enqueue(nnpe_ctx* ctx, bool full) {
@@ -377,6 +393,26 @@ bool gr_fusable(nnpe_ctx* ctx) {
 }
 
 int gr_shell_size(nnpe_ctx* ctx) { return kGrShellCopies * ctx->npair * ctx->gr_nbin; }
+
+// The outer stencil cells that some point of the home cell can reach within the tally's range
+// (r_max of nnpe_gr_init, else the g(r) cutoff + skin, plus a margin).  A cell at offset d along
+// an axis is at least (|d| - 1) cell edges away.
+int gr_outer_refresh(nnpe_ctx* ctx) {
+  if (!ctx->have_grid) return NNPE_OK;
+  const double edge = 1.0 / double(ctx->g_rdx);
+  const double reach = (ctx->gr_count ? std::sqrt(double(ctx->gr_rmaxsq)) + 0.01 : double(ctx->gr_cutoff + ctx->skin)) + 0.05;
+  std::vector<int4> keep;
+  for (const int4& c : ctx->h_stencil_gr_outer) {
+    auto g = [&](int d) { return d == 0 ? 0.0 : (std::abs(d) - 1) * edge; };
+    const double gx = g(c.x), gy = g(c.y), gz = g(c.z);
+    if (gx * gx + gy * gy + gz * gz < reach * reach) keep.push_back(c);
+  }
+  ctx->n_stencil_gr_outer = int(keep.size());
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->stencil_gr_outer, std::max<size_t>(1, keep.size())));
+  if (!keep.empty())
+    NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->stencil_gr_outer, keep.data(), keep.size() * sizeof(int4), hipMemcpyHostToDevice));
+  return NNPE_OK;
+}
 
 int gr_shell_finish_enqueue(nnpe_ctx* ctx) {
   const int npair = ctx->npair, nb = ctx->gr_nbin;

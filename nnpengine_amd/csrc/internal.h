@@ -101,7 +101,10 @@ struct nnpe_ctx {
   int *gs_src = nullptr, *gs_code = nullptr;                  // sorted by cell
   int4* stencil_pair = nullptr; int n_stencil_pair = 0;
   int4* stencil_gr = nullptr; int n_stencil_gr = 0;
-  int4* stencil_gr_outer = nullptr; int n_stencil_gr_outer = 0;  // g(r) stencil cells outside the pair stencil
+  // g(r) stencil cells outside the pair stencil (all of them on the host; on the device those an
+  // atom of the home cell can reach within the tally's range)
+  std::vector<int4> h_stencil_gr_outer;
+  int4* stencil_gr_outer = nullptr; int n_stencil_gr_outer = 0;
   int cap_long = 0, cap_short = 0;
   uint32_t *nbr_long = nullptr, *nbr_short = nullptr;
   int *cnt_long = nullptr, *cnt_short = nullptr;
@@ -178,6 +181,7 @@ int gr_enqueue(nnpe_ctx* ctx, bool full);
 // of its candidates, the outer cells follow in a shell pass
 bool gr_fusable(nnpe_ctx* ctx);
 int gr_shell_size(nnpe_ctx* ctx);  // entries of the shell copies (cleared by the rebuild's reset)
+int gr_outer_refresh(nnpe_ctx* ctx);  // device outer stencil for the current grid and g(r) range
 int gr_shell_finish_enqueue(nnpe_ctx* ctx);
 int sq_enqueue(nnpe_ctx* ctx);
 int nnp_tbl_desc_enqueue(nnpe_ctx* ctx);

@@ -632,10 +632,7 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
     for (const int4& q : sp) in = in || (q.x == g.x && q.y == g.y && q.z == g.z);
     if (!in) so.push_back(g);
   }
-  ctx->n_stencil_gr_outer = int(so.size());
-  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->stencil_gr_outer, std::max<size_t>(1, so.size())));
-  if (!so.empty())
-    NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->stencil_gr_outer, so.data(), so.size() * sizeof(int4), hipMemcpyHostToDevice));
+  ctx->h_stencil_gr_outer = so;
   const int G = ctx->num_grid;
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_start, G + 1)); NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_cnt, G));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_fill, G));
@@ -677,6 +674,8 @@ extern "C" int nnpe_neigh_init(nnpe_ctx* ctx, float pair_cutoff, float skin, flo
   if (rcb) return rcb;
   ctx->have_grid = true;
   ctx->have_list = false;
+  const int rco = gr_outer_refresh(ctx);
+  if (rco) return rco;
   return nnp_act_reserve(ctx);
 }
 
