@@ -36,6 +36,9 @@ extern "C" int nnpe_create(nnpe_ctx** out, int device) {
   ctx->device = device;
   NNPE_CHECK_HIP(ctx, hipSetDevice(device));
   NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+  NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
   NNPE_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->counters), sizeof(DevCounters)));
   NNPE_CHECK_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->h_counters), sizeof(DevCounters), hipHostMallocDefault));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->counters, 0, sizeof(DevCounters)));
@@ -84,6 +87,10 @@ extern "C" int nnpe_destroy(nnpe_ctx* ctx) {
   dev_free(ctx->ev_sum);
   if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
   if (ctx->h_ev) (void)hipHostFree(ctx->h_ev);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return NNPE_OK;

@@ -58,6 +58,10 @@ struct SqdState;  // direct k-space S(q) (sqd.hip)
 struct nnpe_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // MD steps: the classical terms run on stream2 beside the NNP descriptor and network on
+  // stream (fork/join by the two events; default priority, like stream)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
 
   // ---- system ---------------------------------------------------------------------------

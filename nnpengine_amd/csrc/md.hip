@@ -95,16 +95,42 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
     if (P.rmdf && (rc = rmdf_enqueue(ctx, (key & kLog) && rmdf_gamma(ctx) != 1.0f))) return rc;
   }
   const bool ev = (key & kLog) != 0;
-  if (P.pair) {
-    if ((rc = pair_enqueue(ctx, ev))) return rc;
-  } else {
-    hipLaunchKernelGGL(k_clear_force, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, s, ctx->natom, ctx->f);
+  // The classical terms (they write f first) and the NNP descriptors, network and adjoints (they
+  // touch no force) are independent: with an NNP term the classical ones go to stream2 and the
+  // NNP force, which adds into f, waits for them.  Captured, the step graph gets two branches.
+  const bool fork = P.nnp != 0;
+  if (fork) {
+    NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+    NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    ctx->stream = ctx->stream2;
   }
-  if (P.bonded && (rc = bonded_enqueue(ctx, ev))) return rc;
+  if (P.pair) {
+    rc = pair_enqueue(ctx, ev);
+  } else {
+    hipLaunchKernelGGL(k_clear_force, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->f);
+    rc = NNPE_OK;
+  }
+  if (!rc && P.bonded) rc = bonded_enqueue(ctx, ev);
+  if (fork) {
+    ctx->stream = s;  // the join is recorded even after an error, so a capture always closes
+    NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->stream2));
+  }
+  int rn = NNPE_OK;
   if (P.nnp == 1) {
-    if ((rc = nnp_tbl_desc_enqueue(ctx)) || (rc = mlp_enqueue(ctx, ev)) || (rc = nnp_tbl_force_enqueue(ctx, ev))) return rc;
+    rn = nnp_tbl_desc_enqueue(ctx);
+    if (!rn) rn = mlp_enqueue(ctx, ev);
   } else if (P.nnp == 2) {
-    if ((rc = sf_desc_enqueue(ctx)) || (rc = mlp_enqueue(ctx, ev)) || (rc = sf_force_enqueue(ctx, ev))) return rc;
+    rn = sf_desc_enqueue(ctx);
+    if (!rn) rn = mlp_enqueue(ctx, ev);
+    if (!rn) rn = sf_adjoint_enqueue(ctx);
+  }
+  if (fork) NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
+  if (rc) return rc;
+  if (rn) return rn;
+  if (P.nnp == 1) {
+    if ((rc = nnp_tbl_force_enqueue(ctx, ev))) return rc;
+  } else if (P.nnp == 2) {
+    if ((rc = sf_force_only_enqueue(ctx, ev))) return rc;
   }
   if ((rc = verlet_enqueue_v(ctx))) return rc;
   if (ev) {
