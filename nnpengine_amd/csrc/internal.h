@@ -193,7 +193,7 @@ int nnp_tbl_force_enqueue(nnpe_ctx* ctx, bool ev);
 int sf_desc_enqueue(nnpe_ctx* ctx);
 int sf_force_enqueue(nnpe_ctx* ctx, bool ev);
 int sf_adjoint_enqueue(nnpe_ctx* ctx);
-int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev);
+int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick);  // kick: fused FP64 Verlet half-kick
 int mlp_enqueue(nnpe_ctx* ctx, bool ev);
 void md_state_free(nnpe_ctx* ctx);
 int neigh_check_and_grow(nnpe_ctx* ctx, bool* rebuilt);
@@ -385,6 +385,33 @@ __device__ __forceinline__ void block_sum2_store(double a, double b, double* par
 
 // host-side entry points shared between translation units
 hipError_t ev_reserve(nnpe_ctx* ctx, int nblocks);
+// Velocity Verlet's second half-kick in FP64 for atom i (VERLET::updateV, src/verlet.cpp:29-97,
+// FP64 form): a = f/m * accelF, v += (a_prev + a) * dt/2, a_prev <- a; returns a as the FP32
+// force row (the force array holds the acceleration afterwards, as in the reference).  Shared by
+// the Verlet kernel and the force kernel that finishes an atom's force, so both round alike.
+__device__ __forceinline__ float4 vv_kick64(int i, double dt, const float* rm, double4* vd, double4* ad, float4 ff) {
+#pragma clang fp contract(off)
+  const double r = double(rm[i]) * 4.184e-4;
+  const double4 ap = ad[i];
+  const double4 an = make_double4(double(ff.x) * r, double(ff.y) * r, double(ff.z) * r, 0);
+  double4 vv = vd[i];
+  const double hdt = 0.5 * dt;
+  vv.x += (ap.x + an.x) * hdt;
+  vv.y += (ap.y + an.y) * hdt;
+  vv.z += (ap.z + an.z) * hdt;
+  vd[i] = vv;
+  ad[i] = an;
+  return make_float4(float(an.x), float(an.y), float(an.z), 0);
+}
+
+// arguments of a fused FP64 half-kick
+struct VvArgs {
+  double dt;
+  const float* rm;
+  double4* vd;
+  double4* ad;
+};
+
 int ev_finish(nnpe_ctx* ctx, int term, int nblocks);  // add partials of a term into ev_sum
 double* ev_part(nnpe_ctx* ctx, int term, int which);  // partial array for term (0 energy, 1 virial)
 int counters_read(nnpe_ctx* ctx);

@@ -127,12 +127,14 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   if (fork) NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
   if (rc) return rc;
   if (rn) return rn;
+  // the SF force pass is the last writer of each atom's force: the FP64 half-kick rides on it
+  const bool kick = P.nnp == 2 && ctx->fp64;
   if (P.nnp == 1) {
     if ((rc = nnp_tbl_force_enqueue(ctx, ev))) return rc;
   } else if (P.nnp == 2) {
-    if ((rc = sf_force_only_enqueue(ctx, ev))) return rc;
+    if ((rc = sf_force_only_enqueue(ctx, ev, kick))) return rc;
   }
-  if ((rc = verlet_enqueue_v(ctx))) return rc;
+  if (!kick && (rc = verlet_enqueue_v(ctx))) return rc;
   if (ev) {
     if ((rc = ke_enqueue(ctx))) return rc;
     hipLaunchKernelGGL(k_log_append, dim3(1), dim3(64), 0, s, ctx->ev_sum, ctx->md_log, ctx->md_log_cap, 2 * NNPE_NTERM,
@@ -277,7 +279,7 @@ extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* av
       case NNPE_STAGE_SF_DESC: return sf_desc_enqueue(ctx);
       case NNPE_STAGE_MLP: return ctx->mlp ? mlp_enqueue(ctx, false) : NNPE_ERR_STATE;
       case NNPE_STAGE_SF_ADJOINT: return sf_adjoint_enqueue(ctx);
-      case NNPE_STAGE_SF_FORCE: return sf_force_only_enqueue(ctx, false);
+      case NNPE_STAGE_SF_FORCE: return sf_force_only_enqueue(ctx, false, false);
       case NNPE_STAGE_REBUILD: return neigh_enqueue_build(ctx);
       case NNPE_STAGE_REBUILD_GR: return neigh_enqueue_build(ctx, true);
       case NNPE_STAGE_GR_FULL: return ctx->gr_count ? gr_enqueue(ctx, true) : NNPE_ERR_STATE;

@@ -450,13 +450,16 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
 // list per neighbour element, compacted by the descriptor pass); the atom's own adjoint row is
 // staged in LDS (one row per group), each lane gathers the neighbour's element slice of its
 // pairs from global memory.
-template <int NT, bool EV, int G, bool RR>
+// VV: this pass finishes each atom's force (the classical terms were added before), so it also
+// applies the FP64 velocity half-kick (vv_kick64) and stores the acceleration, in place of a
+// separate Verlet pass.
+template <int NT, bool EV, int G, bool RR, bool VV>
 __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                   const int* __restrict__ typ, const BoxPar* __restrict__ boxes,
                                                   const int* __restrict__ rbox, SfParams P,
                                                   const float* __restrict__ adj, float4* __restrict__ f, double* pe,
-                                                  double* pw) {
+                                                  double* pw, VvArgs V) {
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kSlice;
   __shared__ __align__(16) float s_adj[256 / G][NMT];
@@ -572,6 +575,7 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
     if (lane == 0) {
       float4 fo = f[i];
       fo.x += fx; fo.y += fy; fo.z += fz;
+      if constexpr (VV) fo = vv_kick64(i, V.dt, V.rm, V.vd, V.ad, fo);
       f[i] = fo;
     }
     wacc = vir;
@@ -827,14 +831,17 @@ int sf_adjoint_enqueue(nnpe_ctx* ctx) {
 }
 
 // the pair-force kernel alone (uses the adjoints of the last sf_adjoint_enqueue)
-int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
+int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick) {
   NnpState& S = nnp_state(ctx);
   const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
   double* pe = ev_part(ctx, NNPE_TERM_SF, 0);
   double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
-#define NNPE_SF_FORCE1(NT, EV, RR)                                                                                \
-  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG, RR>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
-                     S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, S.P, S.adj, ctx->f, pe, pw)
+  if (kick && !ctx->fp64) return ctx_fail(ctx, NNPE_ERR_STATE, "fused half-kick needs the FP64 integrator");
+  const VvArgs V{ctx->dt, ctx->rm, ctx->vd, ctx->ad};
+#define NNPE_SF_FORCE2(NT, EV, RR, VV)                                                                                \
+  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG, RR, VV>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
+                     S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, S.P, S.adj, ctx->f, pe, pw, V)
+#define NNPE_SF_FORCE1(NT, EV, RR) if (kick) NNPE_SF_FORCE2(NT, EV, RR, true); else NNPE_SF_FORCE2(NT, EV, RR, false)
 #define NNPE_SF_FORCE(NT, EV) if (S.rad_recur) NNPE_SF_FORCE1(NT, EV, true); else NNPE_SF_FORCE1(NT, EV, false)
   switch (ctx->ntype) {
     case 1: if (ev) NNPE_SF_FORCE(1, true); else NNPE_SF_FORCE(1, false); break;
@@ -844,6 +851,7 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
   }
 #undef NNPE_SF_FORCE
 #undef NNPE_SF_FORCE1
+#undef NNPE_SF_FORCE2
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_SF, nblk);
   return NNPE_OK;
@@ -852,7 +860,7 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev) {
 int sf_force_enqueue(nnpe_ctx* ctx, bool ev) {
   int rc = sf_adjoint_enqueue(ctx);
   if (rc) return rc;
-  return sf_force_only_enqueue(ctx, ev);
+  return sf_force_only_enqueue(ctx, ev, false);
 }
 
 extern "C" int nnpe_sf_descriptor(nnpe_ctx* ctx) {

@@ -63,18 +63,7 @@ __global__ void k_update_x64(int n, double dt, double4* xd, const double4* vd, d
 __global__ void k_update_v64(int n, double dt, const float* rm, double4* vd, double4* ad, float4* f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double r = double(rm[i]) * 4.184e-4;
-  const float4 ff = f[i];
-  const double4 ap = ad[i];
-  const double4 an = make_double4(double(ff.x) * r, double(ff.y) * r, double(ff.z) * r, 0);
-  double4 vv = vd[i];
-  const double hdt = 0.5 * dt;
-  vv.x += (ap.x + an.x) * hdt;
-  vv.y += (ap.y + an.y) * hdt;
-  vv.z += (ap.z + an.z) * hdt;
-  vd[i] = vv;
-  ad[i] = an;
-  f[i] = make_float4(float(an.x), float(an.y), float(an.z), 0);
+  f[i] = vv_kick64(i, dt, rm, vd, ad, f[i]);
 }
 
 __global__ void k_ke(int n, bool fp64, const float* rm, const float4* v, const double4* vd, double* part, double* part2) {
