@@ -363,8 +363,10 @@ def main():
     N = sysm.natom
     flop_atom = 4 * sum(WIDTHS[i] * WIDTHS[i + 1] for i in range(len(WIDTHS) - 1))
     # (the full g(r) tally runs on the rebuild steps, folded into the rebuild: stage rebuild_gr)
+    # (the verlet stage times both half-steps; in the NNP step the second rides on the SF force
+    # pass, so about half of it is a launch of its own)
     per_step = {"pair": 1.0, "bonded": 1.0, "sf_desc": 1.0, "mlp": 1.0, "sf_adjoint": 1.0, "sf_force": 1.0,
-                "verlet": 1.0, "rebuild_gr": 0.1, "sq": 0.1}
+                "verlet": 0.5, "rebuild_gr": 0.1, "sq": 0.1}
     share = {k: stages[k] * f for k, f in per_step.items()}
     # the dominant KERNEL: rebuild_gr and sq are several launches each (the rebuild's largest,
     # k_build_cells, is well under half of its stage), so the pick is among one-kernel stages
