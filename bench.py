@@ -244,6 +244,29 @@ def hbm_traffic(stage):
     return None
 
 
+# the kernel each one-kernel stage launches in this workload (the rows of the VALU summary)
+STAGE_KERNEL = {"pair": "k_pair<false, 32>", "bonded": "k_bonded<false>", "sf_desc": "k_sf_desc<2, 16, true>",
+                "mlp": "k_mlp<1, 3>", "sf_adjoint": "k_sf_adjoint<2>", "sf_force": "k_sf_force<2, false, 16, true, false>",
+                "verlet": "k_update_x64"}
+
+
+def valu_busy(stage):
+    """VALU busy share of the stage's kernel from the committed VALU counter summary (one
+    rocprofv3 pass of this workload; the file states the command and the formula), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_valu_water24k_nnp_step.md")
+    name = STAGE_KERNEL.get(stage)
+    if name is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 9 and cells[0] == f"`{name}`":
+            try:
+                return float(cells[4].rstrip(" %")) / 100.0
+            except ValueError:
+                return None
+    return None
+
+
 def force_errors(sysm, par, norm):
     """NNP and total force relative RMS vs the FP64 oracle on one snapshot."""
     from nnpengine_amd import model as M
@@ -395,6 +418,9 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": alg_bytes[dom]}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = hbm_traffic(dom)
+    # the SF and pair kernels are bound by their vector-ALU issue and gather latency rather than
+    # by HBM: the committed VALU counter pass gives the kernel's VALU busy share beside frac
+    roof["valu_busy"] = valu_busy(dom)
     roof["step_share"] = {k: round(v / sum(share.values()), 4) for k, v in share.items()}
     mlp_roof = {"achieved_tflops_bf16": flop_atom * N / (stages["mlp"] * 1e-3) / 1e12,
                 "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
