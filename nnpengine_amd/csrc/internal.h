@@ -160,6 +160,7 @@ struct nnpe_ctx {
   SqdState* sqd = nullptr;
   float* G = nullptr;
   float* dG = nullptr;
+  int dG_kind = 0;  // precision form of the network that wrote dG (0: FP32 or given by the caller)
   int D_cur = 0, D_cap = 0;
   float* e_atom = nullptr;              // [natom] by row
   int* type_idx = nullptr;              // rows grouped by element: [ntype][typecount]

@@ -695,6 +695,7 @@ int mlp_retile(nnpe_ctx* ctx) {
 int mlp_enqueue(nnpe_ctx* ctx, bool ev) {
   MlpState* S = ctx->mlp;
   const MlpDev& d = S->dev;
+  ctx->dG_kind = S->kind;
   if (S->ntiles > 0) {
 #define NNPE_MLP(K, NH)                                                                                      \
   hipLaunchKernelGGL((k_mlp<K, NH>), dim3(S->ntiles), dim3(64 * kMlpWaves), S->lds_bytes, ctx->stream, d, ctx->G, \
@@ -775,5 +776,6 @@ extern "C" int nnpe_nnp_set_dedg(nnpe_ctx* ctx, const float* dGin, int D) {
   NNPE_CHECK_HIP(ctx, hipFreeAsync(tmp, ctx->stream));
   NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->D_cur = D;
+  ctx->dG_kind = 0;
   return NNPE_OK;
 }

@@ -378,9 +378,10 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
 // forms the 11 adjoints of moment group (a, m) -- summing the partner elements b in order, each
 // pair's terms in the order of the pair loop (x <= y) of the descriptor definition, the
 // self pair twice -- and copies four of element a's sixteen radial dE/dG into the slice.
-template <int NT>
+template <int NT, bool HALF>
 __global__ void __launch_bounds__(256) k_sf_adjoint(int n, int stride, const float* __restrict__ M,
-                                                    const float* __restrict__ dG, float* __restrict__ adj) {
+                                                    const float* __restrict__ dG, float* __restrict__ adj,
+                                                    uint4* __restrict__ adjh) {
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int D = NT * kNRad + NT * (NT + 1) / 2 * kNEta * 4;
@@ -423,6 +424,27 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, int stride, const flo
   if (m == 0)
     reinterpret_cast<float4*>(adj)[(size_t)(a * (kSlice / 4) + (kSliceRad + kNRad) / 4) * stride + i] =
         make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if constexpr (HALF) {
+    // slice value f of element a of row i at half f % 8 of adjh[(a * 8 + f / 8) * stride + i]:
+    // the four lanes of (i, a) -- one wave -- assemble the slice in LDS, then each stores two
+    // whole 16-byte chunks
+    static_assert(kSlice / 8 == 2 * kNEta, "two 8-half chunks per exponent lane");
+    __shared__ __align__(16) _Float16 s_h[256 / kNEta][kSlice];
+    _Float16* sh = s_h[threadIdx.x / kNEta];
+#pragma unroll
+    for (int c = 0; c < kNMom; ++c) sh[m * kNMom + c] = (_Float16)dA[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh[kSliceRad + 4 * m + k] = (_Float16)gr[k];
+    if (m == 0)
+#pragma unroll
+      for (int k = kSliceRad + kNRad; k < kSlice; ++k) sh[k] = (_Float16)0.0f;
+    __builtin_amdgcn_wave_barrier();  // the group lives in one wave: its LDS writes are ordered
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = 2 * m + h;
+      adjh[(size_t)(a * (kSlice / 8) + q) * stride + i] = *reinterpret_cast<const uint4*>(sh + 8 * q);
+    }
+  }
 }
 
 // dE_owner/du for one neighbour at unit vector h (u = r h) of the given element slice of an
@@ -453,13 +475,16 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
 // VV: this pass finishes each atom's force (the classical terms were added before), so it also
 // applies the FP64 velocity half-kick (vv_kick64) and stores the acceleration, in place of a
 // separate Verlet pass.
-template <int NT, bool EV, int G, bool RR, bool VV>
-__global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
+// HALF (16-bit network forms): the neighbour's slice is gathered from its FP16 copy, 8 values
+// per 16-byte load (8 loads per pair instead of 15); the atom's own slice stays FP32.
+template <int NT, bool EV, int G, bool RR, bool VV, bool HALF>
+// three waves per SIMD (at most 168 VGPRs): measured faster than the compiler's own 178-198
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_sf_force(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
                                                   const int* __restrict__ cnt, const float4* __restrict__ xq,
                                                   const int* __restrict__ typ, const BoxPar* __restrict__ boxes,
                                                   const int* __restrict__ rbox, SfParams P,
-                                                  const float* __restrict__ adj, float4* __restrict__ f, double* pe,
-                                                  double* pw, VvArgs V) {
+                                                  const float* __restrict__ adj, const uint4* __restrict__ adjh,
+                                                  float4* __restrict__ f, double* pe, double* pw, VvArgs V) {
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kSlice;
   __shared__ __align__(16) float s_adj[256 / G][NMT];
@@ -514,10 +539,22 @@ __global__ void __launch_bounds__(256) k_sf_force(int n, int stride, int cap, co
       // the neighbour's slice for this atom's element, 16-byte chunk q at adj4[(ti*16 + q) * stride
       // + j]: the lanes' neighbours are rows of a few cells, so a wave's chunk loads share lines
       float aj[kSliceRad + kNRad];
+      if constexpr (HALF) {
 #pragma unroll
-      for (int q = 0; q < (kSliceRad + kNRad) / 4; ++q) {
-        const float4 v = adj4[(size_t)(ti * (kSlice / 4) + q) * stride + j];
-        aj[4 * q] = v.x; aj[4 * q + 1] = v.y; aj[4 * q + 2] = v.z; aj[4 * q + 3] = v.w;
+        for (int q = 0; q < kSlice / 8; ++q) {
+          const uint4 v = adjh[(size_t)(ti * (kSlice / 8) + q) * stride + j];
+          _Float16 hv[8];
+          __builtin_memcpy(hv, &v, 16);
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            if (8 * q + t < kSliceRad + kNRad) aj[8 * q + t] = (float)hv[t];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < (kSliceRad + kNRad) / 4; ++q) {
+          const float4 v = adj4[(size_t)(ti * (kSlice / 4) + q) * stride + j];
+          aj[4 * q] = v.x; aj[4 * q + 1] = v.y; aj[4 * q + 2] = v.z; aj[4 * q + 3] = v.w;
+        }
       }
       const float* as = ai + tj * kSlice;                      // own slice for the neighbour's element
       // radial: sum_n g_n'(r) (dR_i[tj][n] + dR_j[ti][n])
@@ -600,6 +637,9 @@ struct NnpState {
   SfParams P{};
   float* M = nullptr;    // moments [N][NMT]
   float* adj = nullptr;  // [N][ntype][kSlice]
+  // FP16 copy of the slices for the neighbour gathers of the 16-bit network forms: chunk q (8
+  // halves, slice values 8q .. 8q+7) of element e of row j at adjh[(e * 8 + q) * stride + j]
+  uint4* adjh = nullptr;
   uint32_t* act = nullptr;  // active lists: pairs within the cutoff, [ntype][stride][act_cap]
   int* act_cnt = nullptr;   // [ntype][stride]
   int act_cap = 0;
@@ -615,7 +655,7 @@ void nnp_state_free(nnpe_ctx* ctx) {
   NnpState* S = ctx->nnp;
   if (!S) return;
   dev_free(S->tdesc); dev_free(S->tfdesc); dev_free(S->tpair); dev_free(S->tfpair);
-  dev_free(S->M); dev_free(S->adj); dev_free(S->act); dev_free(S->act_cnt);
+  dev_free(S->M); dev_free(S->adj); dev_free(S->adjh); dev_free(S->act); dev_free(S->act_cnt);
   delete S;
   ctx->nnp = nullptr;
 }
@@ -771,6 +811,7 @@ extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, f
   S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
   NNPE_CHECK_HIP(ctx, dev_alloc(S.M, size_t(ctx->natom) * S.nmt));
   NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->stride) * ctx->ntype * kSlice));
+  NNPE_CHECK_HIP(ctx, dev_alloc(S.adjh, size_t(ctx->stride) * ctx->ntype * (kSlice / 8)));
   S.sf = true;
   S.act_cap = 0;
   int rc0 = nnp_act_reserve(ctx);
@@ -813,12 +854,18 @@ int sf_desc_enqueue(nnpe_ctx* ctx) {
   return NNPE_OK;
 }
 
+// dE/dG from a 16-bit network form: the neighbours' adjoints are gathered in FP16 (dE/dG of the
+// FP32 form or given by the caller keeps FP32 slices)
+static bool sf_half_gather(nnpe_ctx* ctx) { return ctx->dG_kind != 0; }
+
 int sf_adjoint_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
   const int nblk = blocks_for((long long)ctx->natom * ctx->ntype * kNEta, 256);
-#define NNPE_SF_ADJ(NT) \
-  hipLaunchKernelGGL(k_sf_adjoint<NT>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, S.M, nnp_dG(ctx), \
-                     S.adj)
+  const bool half = sf_half_gather(ctx);
+#define NNPE_SF_ADJ1(NT, H)                                                                                   \
+  hipLaunchKernelGGL((k_sf_adjoint<NT, H>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, S.M, \
+                     nnp_dG(ctx), S.adj, S.adjh)
+#define NNPE_SF_ADJ(NT) if (half) NNPE_SF_ADJ1(NT, true); else NNPE_SF_ADJ1(NT, false)
   switch (ctx->ntype) {
     case 1: NNPE_SF_ADJ(1); break;
     case 2: NNPE_SF_ADJ(2); break;
@@ -826,6 +873,7 @@ int sf_adjoint_enqueue(nnpe_ctx* ctx) {
     default: NNPE_SF_ADJ(4); break;
   }
 #undef NNPE_SF_ADJ
+#undef NNPE_SF_ADJ1
   NNPE_CHECK_LAUNCH(ctx);
   return NNPE_OK;
 }
@@ -838,9 +886,11 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick) {
   double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
   if (kick && !ctx->fp64) return ctx_fail(ctx, NNPE_ERR_STATE, "fused half-kick needs the FP64 integrator");
   const VvArgs V{ctx->dt, ctx->rm, ctx->vd, ctx->ad};
-#define NNPE_SF_FORCE2(NT, EV, RR, VV)                                                                                \
-  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG, RR, VV>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
-                     S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, S.P, S.adj, ctx->f, pe, pw, V)
+  const bool half = sf_half_gather(ctx);
+#define NNPE_SF_FORCE3(NT, EV, RR, VV, H)                                                                                \
+  hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG, RR, VV, H>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \
+                     S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, S.P, S.adj, S.adjh, ctx->f, pe, pw, V)
+#define NNPE_SF_FORCE2(NT, EV, RR, VV) if (half) NNPE_SF_FORCE3(NT, EV, RR, VV, true); else NNPE_SF_FORCE3(NT, EV, RR, VV, false)
 #define NNPE_SF_FORCE1(NT, EV, RR) if (kick) NNPE_SF_FORCE2(NT, EV, RR, true); else NNPE_SF_FORCE2(NT, EV, RR, false)
 #define NNPE_SF_FORCE(NT, EV) if (S.rad_recur) NNPE_SF_FORCE1(NT, EV, true); else NNPE_SF_FORCE1(NT, EV, false)
   switch (ctx->ntype) {
@@ -852,6 +902,7 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick) {
 #undef NNPE_SF_FORCE
 #undef NNPE_SF_FORCE1
 #undef NNPE_SF_FORCE2
+#undef NNPE_SF_FORCE3
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_SF, nblk);
   return NNPE_OK;
