@@ -250,6 +250,21 @@ STAGE_KERNEL = {"pair": "k_pair<false, 32>", "bonded": "k_bonded<false>", "sf_de
                 "verlet": "k_update_x64"}
 
 
+def active_pairs(e, bl, rc):
+    """Entries of the short lists within the symmetry-function cutoff (the active lists the SF
+    force pass reads): the FP32 distance test on the exported entries and positions."""
+    ent = e.list_entries(1)
+    x = e.get("x").astype(np.float32)
+    i, j, code = ent[:, 0], ent[:, 1], ent[:, 2]
+    L = np.float32(bl)
+    shift = np.array([-L, np.float32(0), L], np.float32)
+    img = x[j] + np.stack([shift[code // 9], shift[(code % 9) // 3], shift[code % 3]], 1)
+    img[code == 13] = x[j[code == 13]]
+    d = img - x[i]
+    rsq = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    return int(np.count_nonzero(rsq < np.float32(rc) * np.float32(rc)))
+
+
 def valu_busy(stage):
     """VALU busy share of the stage's kernel from the committed VALU counter summary (one
     rocprofv3 pass of this workload; the file states the command and the formula), or None."""
@@ -383,6 +398,7 @@ def main():
     # step; rebuild and the full g(r) tally run every 10th step)
     n_long, n_ghost = eb.list_size(0)
     n_short, _ = eb.list_size(1)
+    n_act = active_pairs(eb, sysm.bl, RC_NNP)
     N = sysm.natom
     flop_atom = 4 * sum(WIDTHS[i] * WIDTHS[i + 1] for i in range(len(WIDTHS) - 1))
     # (the full g(r) tally runs on the rebuild steps, folded into the rebuild: stage rebuild_gr)
@@ -403,7 +419,8 @@ def main():
     alg_bytes = {"pair": 4 * n_long + N * (16 + 4 + 16),
                  "sf_desc": 4 * n_short + N * (16 + 4 + 4 * nmt + 4 * D),
                  "sf_adjoint": N * 4 * (nmt + D + nadj),
-                 "sf_force": 4 * n_short + N * (16 + 4 + 4 * nadj + 16 + 16),
+                 # (the active lists; own slices FP32 + the FP16 copy the 16-bit forms gather from)
+                 "sf_force": 4 * n_act + N * (16 + 4 + 4 * nadj + 2 * nadj + 16 + 16),
                  "bonded": N * (16 + 16 + 16),
                  "verlet": N * 164,
                  "rebuild": 4 * (n_long + n_short) + N * 2 * (16 + 16 + 16 + 4 + 4 + 16 + 3 * 32),
