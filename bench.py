@@ -246,7 +246,8 @@ def hbm_traffic(stage):
 
 # the kernel each one-kernel stage launches in this workload (the rows of the VALU summary)
 STAGE_KERNEL = {"pair": "k_pair<false, 32>", "bonded": "k_bonded<false>", "sf_desc": "k_sf_desc<2, 16, true>",
-                "mlp": "k_mlp<1, 3>", "sf_adjoint": "k_sf_adjoint<2>", "sf_force": "k_sf_force<2, false, 16, true, false>",
+                "mlp": "k_mlp<1, 3>", "sf_adjoint": "k_sf_adjoint<2, true>",
+                "sf_force": "k_sf_force<2, false, 16, true, false, true>",
                 "verlet": "k_update_x64"}
 
 
