@@ -196,6 +196,56 @@ __device__ __forceinline__ void sf_gauss(float r, const SfParams& P, float* g) {
   }
 }
 
+constexpr int kHEta = kNEta / 2, kHRad = kNRad / 2;  // per half of a descriptor lane group
+
+// The 8 radial Gaussians of half hg of the centres, in the order the half accumulates them.
+// RR: from the middle centre s = 8 outwards, the same products as sf_gauss -- half 1 holds
+// g_8, g_9, ..., g_15 (g_{8+t+1} = g_{8+t} R rho_t), half 0 holds g_7, g_6, ..., g_0
+// (g_{8-t-1} = g_{8-t} R^-1 rho_t); without RR, centres 8 hg + t directly.
+template <bool RR>
+__device__ __forceinline__ void sf_gauss_half(float r, const SfParams& P, int hg, float* g) {
+#pragma clang fp contract(fast)
+  static_assert(kRadS == kHRad, "the recurrence starts at the first centre of the upper half");
+  if constexpr (RR) {
+    const float e = r - P.mu[kRadS];
+    const float g8 = __builtin_amdgcn_exp2f(P.eta_r_l2 * e * e);
+    const float l = P.rad_step_l2 * e;
+    const float Rx = __builtin_amdgcn_exp2f(hg ? l : -l);
+    float cur = hg ? g8 : g8 * Rx * P.rad_rho[0];
+    g[0] = cur;
+#pragma unroll
+    for (int t = 1; t < kHRad; ++t) {
+      cur = cur * Rx * (hg ? P.rad_rho[t - 1] : P.rad_rho[t]);
+      g[t] = cur;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < kHRad; ++t) {
+      const float d = r - (hg ? P.mu[kHRad + t] : P.mu[t]);
+      g[t] = __builtin_amdgcn_exp2f(P.eta_r_l2 * d * d);
+    }
+  }
+}
+
+// radial centre of entry t of half hg (the order of sf_gauss_half)
+template <bool RR>
+__device__ __forceinline__ int sf_half_centre(int hg, int t) {
+  if constexpr (RR) return hg ? kRadS + t : kRadS - 1 - t;
+  else return kHRad * hg + t;
+}
+
+// sums over the 8 lanes of each half of a 16-lane group, N values at once (DPP inside a row:
+// half-row mirror, then pair swaps at distance 2 and 1); every lane of the half gets the sum
+template <int N>
+__device__ __forceinline__ void half_group_sum_n(float* v) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] += dpp_f32<0x141>(v[k]);
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] += dpp_f32<0x4E>(v[k]);
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] += dpp_f32<0xB1>(v[k]);
+}
+
 // fc = (cos(pi r/rc) + 1)/2 and dfc/dr for 0 <= r <= rc.  With y = pi (r/rc - 1/2), |y| <= pi/2:
 // cos(pi r/rc) = -sin(y), sin(pi r/rc) = cos(y), both from their Taylor series about 0 (sin to
 // y^11, cos to y^12; truncation below 6e-8 on the range), evaluated by Horner's rule.
@@ -230,8 +280,10 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   constexpr int D = NT * kNRad + NP * kNEta * 4;
   __shared__ float s_mom[256 / G][NMT];
   __shared__ float4 s_pair[256 / G][kSfBuf];
+  static_assert(G == 16, "the moment sums split the 16 lanes of an atom into two halves");
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
+  const int hg = lane >> 3, hl = lane & 7;  // half of the group and lane within it
   const bool live = i < n;
   float* sm = s_mom[grp];
   float* mo = Mout + (size_t)(live ? i : 0) * NMT;
@@ -242,14 +294,16 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
     for (int a = 0; a < NT; ++a) {
       const int cn = cnt[(size_t)a * stride + i];
       const uint32_t* row = nbr + ((size_t)a * stride + i) * cap;
-      float A[kNEta][kNMom];
-      float R[kNRad];
+      // the sums: lanes 0-7 of the group hold exponents 0-1 and the lower 8 radial centres,
+      // lanes 8-15 exponents 2-3 and the upper 8 (half the accumulators live per lane)
+      float A[kHEta][kNMom];
+      float R[kHRad];
 #pragma unroll
-      for (int m = 0; m < kNEta; ++m)
+      for (int m = 0; m < kHEta; ++m)
 #pragma unroll
         for (int c = 0; c < kNMom; ++c) A[m][c] = 0.0f;
 #pragma unroll
-      for (int k = 0; k < kNRad; ++k) R[k] = 0.0f;
+      for (int k = 0; k < kHRad; ++k) R[k] = 0.0f;
       // in-range entries are also compacted, in list order, into this step's active list
       // (a group-masked ballot gives each lane its slot), so the force pass has no idle lanes
       uint32_t* arow = act + ((size_t)a * stride + i) * cap;
@@ -311,20 +365,21 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
           }
         }
         __builtin_amdgcn_wave_barrier();  // a group lives in one wave: its LDS writes are ordered
-        for (int k = lane; k < nb; k += G) {
+        for (int k = hl; k < nb; k += G / 2) {
           const float4 pu = buf[k];
           const float rsq = pu.w;
           const float ir = __builtin_amdgcn_rsqf(rsq), r = rsq * ir;
           const float hx = pu.x * ir, hy = pu.y * ir, hz = pu.z * ir;
           float fc, dfc;
           sf_cut(r, P.rc_inv, fc, dfc);
-          float gk[kNRad];
-          sf_gauss<RR>(r, P, gk);
+          float gk[kHRad];
+          sf_gauss_half<RR>(r, P, hg, gk);
 #pragma unroll
-          for (int k2 = 0; k2 < kNRad; ++k2) R[k2] += gk[k2] * fc;
+          for (int k2 = 0; k2 < kHRad; ++k2) R[k2] += gk[k2] * fc;
 #pragma unroll
-          for (int m = 0; m < kNEta; ++m) {
-            const float w = __builtin_amdgcn_exp2f(P.eta_l2[m] * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
+          for (int m = 0; m < kHEta; ++m) {
+            const float el2 = hg ? P.eta_l2[kHEta + m] : P.eta_l2[m];
+            const float w = __builtin_amdgcn_exp2f(el2 * rsq) * fc, wx = w * hx, wy = w * hy, wz = w * hz;
             A[m][0] += w;
             A[m][1] += wx; A[m][2] += wy; A[m][3] += wz;
             A[m][4] += wx * hx; A[m][5] += wy * hy; A[m][6] += wz * hz;
@@ -335,20 +390,23 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
         __builtin_amdgcn_wave_barrier();  // buffer consumed before the next scan overwrites it
       }
       if (lane == 0) act_cnt[(size_t)a * stride + i] = na;
-      group_sum_n<G, kNEta * kNMom>(&A[0][0]);
-      group_sum_n<G, kNRad>(R);
+      half_group_sum_n<kHEta * kNMom>(&A[0][0]);
+      half_group_sum_n<kHRad>(R);
+      // value v of this half (angular m*11 + c, then radial) is stored by its lane v mod 8
 #pragma unroll
-      for (int m = 0; m < kNEta; ++m)
+      for (int m = 0; m < kHEta; ++m)
 #pragma unroll
-        for (int c = 0; c < kNMom; ++c) {
-          const int idx = (a * kNEta + m) * kNMom + c;
-          if (idx % G == lane) { mo[idx] = A[m][c]; sm[idx] = A[m][c]; }
+        for (int c = 0; c < kNMom; ++c)
+          if ((m * kNMom + c) % (G / 2) == hl) {
+            const int idx = (a * kNEta + kHEta * hg + m) * kNMom + c;
+            mo[idx] = A[m][c]; sm[idx] = A[m][c];
+          }
+#pragma unroll
+      for (int k = 0; k < kHRad; ++k)
+        if ((kHEta * kNMom + k) % (G / 2) == hl) {
+          const int idx = RAD0 + a * kNRad + sf_half_centre<RR>(hg, k);
+          mo[idx] = R[k]; sm[idx] = R[k];
         }
-#pragma unroll
-      for (int k = 0; k < kNRad; ++k) {
-        const int idx = RAD0 + a * kNRad + k;
-        if (idx % G == lane) { mo[idx] = R[k]; sm[idx] = R[k]; }
-      }
     }
   }
   __syncthreads();
