@@ -443,6 +443,16 @@ def main():
     mlp_roof = {"achieved_tflops_bf16": flop_atom * N / (stages["mlp"] * 1e-3) / 1e12,
                 "achieved_tflops_fp32": flop_atom * N / (ms32_mlp * 1e-3) / 1e12,
                 "flop_per_atom": flop_atom}
+    mlp_roof["frac_mfma_bf16"] = mlp_roof["achieved_tflops_bf16"] / MFMA_PEAK["bf16"]
+    mlp_roof["frac_mfma_fp32"] = mlp_roof["achieved_tflops_fp32"] / MFMA_PEAK["fp32"]
+    # every stage against the HBM roofline (algorithmic bytes / live stage time), with the HBM
+    # traffic and VALU busy share of its kernel from the committed counter summaries where the
+    # stage is one kernel (the rebuild is a dozen launches: its time and bytes cover them all)
+    stage_roof = {}
+    for st in ("rebuild", "pair", "bonded", "sf_desc", "sf_adjoint", "sf_force", "verlet"):
+        ach = alg_bytes[st] / (stages[st] * 1e-3) / 1e9
+        stage_roof[st] = {"algorithmic_bytes": alg_bytes[st], "achieved_gbs": ach, "frac_hbm": ach / HBM_PEAK_GBS,
+                          "traffic": hbm_traffic(st), "valu_busy": valu_busy(st)}
 
     sqd = None if args.no_sqd else sqd_timing(eb)
 
@@ -498,6 +508,7 @@ def main():
         "stage_ms": stages,
         "mlp": mlp_roof,
         "roofline": roof,
+        "stage_roofline": stage_roof,
         "sqd": sqd,
         "cpu_baseline": cpu,
     }
