@@ -98,7 +98,13 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   // The classical terms (they write f first) and the NNP descriptors, network and adjoints (they
   // touch no force) are independent: with an NNP term the classical ones go to stream2 and the
   // NNP force, which adds into f, waits for them.  Captured, the step graph gets two branches.
+  // The descriptors run first, alone: the classical kernels (VALU-bound) then overlap the network
+  // (MFMA, latency-bound) and the adjoints instead of the equally VALU-bound descriptor pass.
   const bool fork = P.nnp != 0;
+  int rn = NNPE_OK;
+  if (P.nnp == 1) rn = nnp_tbl_desc_enqueue(ctx);
+  else if (P.nnp == 2) rn = sf_desc_enqueue(ctx);
+  if (rn) return rn;
   if (fork) {
     NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
     NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
@@ -115,13 +121,10 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
     ctx->stream = s;  // the join is recorded even after an error, so a capture always closes
     NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->stream2));
   }
-  int rn = NNPE_OK;
   if (P.nnp == 1) {
-    rn = nnp_tbl_desc_enqueue(ctx);
-    if (!rn) rn = mlp_enqueue(ctx, ev);
+    rn = mlp_enqueue(ctx, ev);
   } else if (P.nnp == 2) {
-    rn = sf_desc_enqueue(ctx);
-    if (!rn) rn = mlp_enqueue(ctx, ev);
+    rn = mlp_enqueue(ctx, ev);
     if (!rn) rn = sf_adjoint_enqueue(ctx);
   }
   if (fork) NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
