@@ -282,6 +282,7 @@ __global__ void __launch_bounds__(256) k_ghost_cell_sort(int ncell, const int* g
 // a time, its 64 lanes test 64 candidates per step, and a ballot + mbcnt gives every accepted
 // lane its slot, so a wave appends up to 64 entries to the atom's row with one coalesced store.
 // Entry order in a row: stencil cell, real before ghost, then position in the cell.
+constexpr int kBuildThreads = 512;  // workgroup size of the list build
 constexpr int kBuildBatch = 1024;  // candidates staged per pass
 constexpr int kBuildHome = 64;     // home atoms per pass
 constexpr int kBuildMaxRanges = 2 * 343;
@@ -387,7 +388,7 @@ __device__ __forceinline__ void build_row_nt(const BuildRowArgs<kGr>& A, const f
 }
 
 template <bool kShort, bool kGr>
-__global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __restrict__ boxes,
+__global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const BoxPar* __restrict__ boxes,
                                                      const int* __restrict__ cell_box, const int4* __restrict__ stencil,
                                                      int nst, float cutsq, float cutsq_s,
                                                      const float4* __restrict__ xq, const int4* __restrict__ info,
@@ -411,7 +412,7 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
   __shared__ uint32_t c_ent[kBuildBatch];            // row | code << 25 | nocheck << 31
   __shared__ uint32_t c_aux[kBuildBatch];            // intra id | element << 24
   __shared__ int h_nl[kBuildHome], h_ns[kBuildHome][kShortMaxType];
-  __shared__ int s_wsum[256 / 64 + 1];
+  __shared__ int s_wsum[kBuildThreads / 64 + 1];
   __shared__ int s_max[2];  // longest full and short row of this workgroup
   const int c = xcd_block(gridDim.x);
   if (c >= ncell) return;
@@ -444,7 +445,7 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
   }
   __syncthreads();
   const int nr2 = 2 * nst;
-  const int total = block_scan_excl<256>(r_pref, nr2, s_wsum);
+  const int total = block_scan_excl<kBuildThreads>(r_pref, nr2, s_wsum);
   for (int h0 = hb; h0 < he; h0 += kBuildHome) {
     const int h1 = min(he, h0 + kBuildHome);
     for (int k = tid; k < h1 - h0; k += blockDim.x) {
@@ -473,7 +474,7 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell, const BoxPar* __
         c_aux[u] = (uint32_t)in.z | ((uint32_t)typ[row] << 24);
       }
       __syncthreads();
-      for (int h = h0 + wave; h < h1; h += 256 / 64) {
+      for (int h = h0 + wave; h < h1; h += kBuildThreads / 64) {
         const float4 xi = xq[h];
         const int4 ii = info[h];
         const int ti_gr = kGr ? typ[h] : 0;
@@ -742,7 +743,7 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
                      ctx->gs_src, ctx->gs_code, ctx->info);
   const size_t gh = gr_tally ? sizeof(unsigned int) * ctx->npair * ctx->gr_nbin : 0;
 #define NNPE_BUILD(SHORT, GR)                                                                                             \
-  hipLaunchKernelGGL((k_build_cells<SHORT, GR>), dim3(G), dim3(256), gh, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair, \
+  hipLaunchKernelGGL((k_build_cells<SHORT, GR>), dim3(G), dim3(kBuildThreads), gh, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair, \
                      ctx->n_stencil_pair, ctx->cutsq_list, SHORT ? ctx->cutsq_short : 0.0f, ctx->xq, ctx->info,            \
                      ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,      \
                      ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, SHORT ? ctx->cap_short : 0,          \
