@@ -36,7 +36,9 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
                                               double* pw) {
 #pragma clang fp contract(fast)
   extern __shared__ float s_tab[];  // [npair][nb1] F/r
+  __shared__ float4 s_img[27];
   for (int k = threadIdx.x; k < npair * nb1; k += blockDim.x) s_tab[k] = tab[k];
+  image_table_fill(s_img);
   __syncthreads();
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1);
@@ -45,7 +47,7 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
     const float4 xi = xq[i];
     const int ti = typ[i];
     const BoxPar& B = boxes[rbox[i]];
-    const float bl = B.bl, m_bl = B.m_bl;
+    const float bl = B.bl;
     float fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     const int m = cnt[i];
     const uint32_t* row = list_row(nbr, cap, i);
@@ -79,7 +81,7 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
         if (k0 + u * G >= m) break;
         const uint32_t ent = ec[u];
         const int code = NNPE_ENTRY_CODE(ent), sp = NNPE_ENTRY_SP(ent);
-        const float4 xj = image_pos(xr[u], code, bl, m_bl);
+        const float4 xj = image_pos_t(xr[u], code, bl, s_img);
         float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float rsq = dist2_rn(dx, dy, dz);  // the cut as the reference rounds it
         if (!(rsq < cutsq)) continue;

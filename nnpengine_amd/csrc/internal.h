@@ -261,6 +261,20 @@ __device__ __forceinline__ float4 image_pos(float4 xs, int code, float bl, float
 }
 // (a-b)^2 summed left to right with IEEE single rounding and no contraction, as the
 // reference's sqr(dx)+sqr(dy)+sqr(dz) (src/neighlist.cpp:260)
+// The same images through a 27-entry LDS table of shift multipliers (code 9a + 3b + c ->
+// (a-1, b-1, c-1)): x + m L as one fused multiply-add, exact as the sum above because m L is
+// -L, 0 or L.  (Code 13 gives x + 0, which differs from x only in the sign of a zero
+// coordinate; no distance or force depends on that.)  The pair-loop kernels fill the table once
+// per workgroup and save the decode of the code on every list entry.
+__device__ __forceinline__ void image_table_fill(float4* t) {
+  for (int k = threadIdx.x; k < 27; k += blockDim.x)
+    t[k] = make_float4(float(k / 9 - 1), float((k / 3) % 3 - 1), float(k % 3 - 1), 0.0f);
+}
+__device__ __forceinline__ float4 image_pos_t(float4 xs, int code, float bl, const float4* t) {
+  const float4 m = t[code];
+  return make_float4(__fmaf_rn(m.x, bl, xs.x), __fmaf_rn(m.y, bl, xs.y), __fmaf_rn(m.z, bl, xs.z), xs.w);
+}
+
 __device__ __forceinline__ float dist2_rn(float dx, float dy, float dz) {
   return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
 }

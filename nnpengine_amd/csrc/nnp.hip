@@ -280,6 +280,9 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   constexpr int D = NT * kNRad + NP * kNEta * 4;
   __shared__ float s_mom[256 / G][NMT];
   __shared__ float4 s_pair[256 / G][kSfBuf];
+  __shared__ float4 s_img[27];
+  image_table_fill(s_img);
+  __syncthreads();
   static_assert(G == 16, "the moment sums split the 16 lanes of an atom into two halves");
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
@@ -289,7 +292,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   float* mo = Mout + (size_t)(live ? i : 0) * NMT;
   if (live) {
     const float4 xi = xq[i];
-    const float bl = boxes[rbox[i]].bl, m_bl = boxes[rbox[i]].m_bl;
+    const float bl = boxes[rbox[i]].bl;
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
       const int cn = cnt[(size_t)a * stride + i];
@@ -348,7 +351,7 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
             float ux = 0, uy = 0, uz = 0, rsq = 0;
             bool in = false;
             if (q < cn) {
-              const float4 xj = image_pos(xr[u], NNPE_ENTRY_CODE(ec[u]), bl, m_bl);
+              const float4 xj = image_pos_t(xr[u], NNPE_ENTRY_CODE(ec[u]), bl, s_img);
               ux = xj.x - xi.x; uy = xj.y - xi.y; uz = xj.z - xi.z;
               rsq = dist2_rn(ux, uy, uz);  // rounded as written: the force pass makes the same cut
               in = rsq < P.rcsq;
@@ -546,6 +549,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kSlice;
   __shared__ __align__(16) float s_adj[256 / G][NMT];
+  __shared__ float4 s_img[27];
+  image_table_fill(s_img);
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
   const float4* __restrict__ adj4 = reinterpret_cast<const float4*>(adj);
@@ -556,7 +561,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   if (i < n) {
     const float4 xi = xq[i];
     const int ti = typ[i];
-    const float bl = boxes[rbox[i]].bl, m_bl = boxes[rbox[i]].m_bl;
+    const float bl = boxes[rbox[i]].bl;
     const float* ai = s_adj[grp];
     float fx = 0, fy = 0, fz = 0, vir = 0;
 #pragma unroll
@@ -579,7 +584,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       if (q0 + u * G >= cn) break;
       const uint32_t ent = ec[u];
       const int j = NNPE_ENTRY_J(ent);
-      const float4 xj = image_pos(xr[u], NNPE_ENTRY_CODE(ent), bl, m_bl);
+      const float4 xj = image_pos_t(xr[u], NNPE_ENTRY_CODE(ent), bl, s_img);
       const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
       const float rsq = dist2_rn(ux, uy, uz);
       if (!(rsq < P.rcsq)) continue;
