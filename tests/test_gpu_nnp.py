@@ -238,6 +238,39 @@ def test_md_nve_energy_conservation(water_c0):
     assert abs(et[-1] - et[1]) < 0.02 * abs(log[1, 14])
 
 
+def test_md_step_equals_call_sequence(water_c0):
+    """nnpe_md_run's NNP step -- graph-replayed, classical terms on a second stream, the FP64
+    half-kick fused into the SF force pass -- against the same step spelt out through the C ABI
+    (update_x, build every 10th step, pair, bonded, SF descriptors, network, SF force, update_v):
+    bit-identical positions and velocities after 25 steps with three rebuilds, BF16 network."""
+    widths = [80, 64, 64, 1]
+    runs = []
+    for mode in ("md", "calls"):
+        e, o, par, G64, norm = _setup(water_c0, widths)
+        e.pair_table(RC_PAIR, *S.lj_table(water_c0, RC_PAIR, 900, S.WATER_LJ))
+        e.bonded_set_from_system()
+        e.mlp_set(widths, par, 0.5, "bf16", *norm)
+        if mode == "md":
+            e.md_run(25, dt=0.5, fp64=True, rebuild_every=10, pair=True, bonded=True, nnp=2, use_graph=True)
+        else:
+            e.verlet_set(0.5, fp64=True)
+            for step in range(25):
+                if step > 0:
+                    e.update_x()
+                if step % 10 == 0:
+                    e.build()
+                e.pair_compute(False)
+                e.bonded_compute(False)
+                e.sf_descriptor()
+                e.mlp_eval(False)
+                e.sf_force(False)
+                e.update_v()
+        runs.append((e.get("x"), e.get("v")))
+        print(f"[md vs calls] {mode}: |v| rms {np.sqrt((runs[-1][1] ** 2).mean()):.6f}", flush=True)
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
 def test_md_checkpoint_rerun_bit_identical(water_c0):
     """A chunk repeated from a checkpoint (nnpe_state_set of x, v, f, f_prev and nnpe_md_set_step,
     the path that repeats a chunk after a list overflow) reproduces the first attempt bit for bit,
