@@ -492,10 +492,12 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, int stride, const flo
     static_assert(kSlice / 8 == 2 * kNEta, "two 8-half chunks per exponent lane");
     __shared__ __align__(16) _Float16 s_h[256 / kNEta][kSlice];
     _Float16* sh = s_h[threadIdx.x / kNEta];
+    // saturate instead of rounding to infinity (|adjoint| beyond 65504 would be a broken network)
+    auto h16 = [](float x) { return (_Float16)fminf(fmaxf(x, -65504.0f), 65504.0f); };
 #pragma unroll
-    for (int c = 0; c < kNMom; ++c) sh[m * kNMom + c] = (_Float16)dA[c];
+    for (int c = 0; c < kNMom; ++c) sh[m * kNMom + c] = h16(dA[c]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) sh[kSliceRad + 4 * m + k] = (_Float16)gr[k];
+    for (int k = 0; k < 4; ++k) sh[kSliceRad + 4 * m + k] = h16(gr[k]);
     if (m == 0)
 #pragma unroll
       for (int k = kSliceRad + kNRad; k < kSlice; ++k) sh[k] = (_Float16)0.0f;
