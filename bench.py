@@ -348,8 +348,8 @@ def main():
     ap.add_argument("--drift-ps", type=float, default=20.0, help="NVE run per precision form for the drift (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is "
-              "described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a sharded run is not "
+              "part of this program", file=sys.stderr)
         sys.exit(2)
     from nnpengine_amd import model as M
     from nnpengine_amd import system as S

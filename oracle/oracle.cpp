@@ -1,7 +1,7 @@
 // oracle.cpp -- CPU restatement of NNPEngine's per-step MD hot path.
 //
 // TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
-// leg load this library, and only as the checker (never as the thing measured or shipped).
+// leg load this library, and only as the test reference (never as the thing measured or shipped).
 //
 // Every algorithm is restated from the reference sources (paths relative to the reference repo),
 // templated on the arithmetic type:

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (liboracle.so, built from oracle/oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, as the checker.  prec=0 reproduces the reference's float32 arithmetic,
+cpu_baseline leg, as the test reference.  prec=0 reproduces the reference's float32 arithmetic,
 prec=1 is the FP64 oracle.
 """
 from __future__ import annotations

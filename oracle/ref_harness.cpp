@@ -1,7 +1,7 @@
 // ref_harness.cpp -- drives the reference's own classes, compiled from the reference sources
 // (see oracle/Makefile), to produce fixtures and the CPU baseline.
 //
-// TEST INFRASTRUCTURE ONLY (the checker and the CPU-baseline leg; never the product path).
+// TEST INFRASTRUCTURE ONLY (test reference and the CPU-baseline leg; never the product path).
 //
 // It follows the classical() MD loop of src/run.cpp:204-394 with the reference's 27 OpenMP
 // threads: refold_and_tag -> ENVIRON::sort -> generate_ghost -> tidyup_ghost ->

@@ -17,7 +17,8 @@ def available():
 
 
 def build():
-    """Compile the harness when the reference tree is present (never on the GPU box)."""
+    """Compile the harness when the reference tree is present (machines without it use the
+    prebuilt files)."""
     if os.path.isdir("/root/reference/src"):
         subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
 

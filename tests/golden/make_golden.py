@@ -3,7 +3,7 @@
 The reference C++ (TF-free sources: ENVIRON, NEIGHLIST, LJTableCoulDSF, BOND, ANGLE, DIHEDRAL,
 VERLET, ...) is compiled from the reference tree by oracle/Makefile into oracle/_ref/ref_harness,
 which runs the reference's classes with its 27 OpenMP threads on seeded inputs.  This script
-needs the reference tree (run it in the build container, not on the GPU box):
+needs the reference tree (run it where the reference sources are checked out):
 
     python tests/golden/make_golden.py
 
