@@ -27,7 +27,7 @@ enum nnpe_status {
   NNPE_ERR_GHOST_OVERFLOW = 5,  /* ghost capacity exceeded ("ghost overflow", src/environ.cpp:440) */
   NNPE_ERR_CELL_RANGE = 6,      /* atom outside the cell grid (the CHECK_BINNING case, src/environ.cpp:245-259) */
   NNPE_ERR_NO_DEVICE = 7,       /* no HIP device */
-  NNPE_ERR_UNSUPPORTED = 8      /* configuration this build does not support */
+  NNPE_ERR_UNSUPPORTED = 8      /* configuration the library does not support */
 };
 
 /* per-atom state selectors for nnpe_state_get / nnpe_state_set */

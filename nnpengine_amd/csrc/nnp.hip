@@ -857,7 +857,7 @@ extern "C" int nnpe_nnp_tbl_force(nnpe_ctx* ctx, int ev) {
 extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, float eta_r, int neta, const float* eta) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
   if (!ctx || nrad != kNRad || neta != kNEta || !mu || !eta || !(rc > 0))
-    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_sf_set: this build has 16 radial centres and 4 angular exponents");
+    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_sf_set: the library supports 16 radial centres and 4 angular exponents");
   if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
   if (ctx->ntype < 1 || ctx->ntype > 4) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "symmetry functions support 1-4 elements");
   NnpState& S = nnp_state(ctx);
