@@ -112,6 +112,75 @@ def test_list_bit_exact_large_molecules(water_c0):
     assert len(got) > 0 and np.array_equal(got, want)
 
 
+def _capacity0(natom, bl, rc):
+    """The list capacity a context starts with: 1.5x the ideal-gas count within rc at the mean
+    density, +64, rounded up to 64 (nnpe_neigh_init)."""
+    rho = natom / float(bl) ** 3
+    k = int(1.5 * (4.0 / 3.0 * np.pi * rc ** 3 * rho)) + 64
+    return (k + 63) // 64 * 64
+
+
+def test_list_capacity_growth_dense_slab():
+    """A liquid squeezed into a slab of 1/16 of the box height (16x the mean density): rows far
+    beyond the starting capacities of both lists, and cells of more than 64 atoms (the sort's
+    long-cell path).  The reference stops with "nei_list overflow" (src/neighlist.cpp:327)
+    beyond its fixed 2048 columns; this library grows the capacity and builds again, and the
+    result must still be the oracle's list, bit for bit."""
+    sysm = S.water_box(1000, 31.0, seed=5)
+    x = sysm.x.astype(np.float32).copy()
+    x[:, 2] *= np.float32(1.0 / 16.0)
+    sysm.x = x
+    sc = 4.0
+    print(f"[list] dense slab natom={sysm.natom}", flush=True)
+    e = _engine(sysm, RC, SKIN, RC, sc)
+    e.build()
+    o = _oracle(sysm, 0)
+    ng_o = o.rebuild(False)
+    full = e.list_entries()
+    assert np.array_equal(pyoracle.canonical(full), pyoracle.canonical(o.list_entries()))
+    assert e.list_size()[1] == ng_o
+    # the growth path ran: the longest row exceeds the starting capacity
+    rows = np.bincount(full[:, 0], minlength=sysm.natom)
+    assert rows.max() > _capacity0(sysm.natom, sysm.bl, RC + SKIN)
+    # and cells of more than 64 atoms were sorted (cell edge (rc + skin) / 2 from the grid origin)
+    xr = e.get("x").astype(np.float64)
+    g_dx = (RC + SKIN) / 2.0
+    cell = np.floor((xr + (RC + SKIN) + sysm.bl / 2.0) / g_dx).astype(np.int64)
+    _, occ = np.unique(cell[:, 0] * 1_000_000 + cell[:, 1] * 1000 + cell[:, 2], return_counts=True)
+    assert occ.max() > 64
+    # short rows: the full-list entries within sc + skin (same FP32 operations), past their
+    # own starting capacity
+    xf = xr.astype(np.float32)
+    bl = np.float32(sysm.bl)
+    i, j, code = full[:, 0], full[:, 1], full[:, 2]
+    shift = np.array([-bl, np.float32(0), bl], np.float32)
+    img = xf[j] + np.stack([shift[code // 9], shift[(code % 9) // 3], shift[code % 3]], 1)
+    img[code == 13] = xf[j[code == 13]]
+    d = xf[i] - img
+    rsq = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    c2 = np.float32(sc) + np.float32(SKIN)
+    short = e.list_entries(1)
+    assert np.array_equal(pyoracle.canonical(short), pyoracle.canonical(full[rsq < c2 * c2]))
+    # the short lists are per neighbour element: a row of one element is what must fit
+    per = np.bincount(short[:, 0] * sysm.ntype + sysm.typ[short[:, 1]], minlength=sysm.natom * sysm.ntype)
+    assert per.max() > _capacity0(sysm.natom, sysm.bl, sc + SKIN)
+
+
+def test_call_order_errors(water_c0):
+    """Calls out of order fail with NNPE_ERR_STATE and a message, as the reference's
+    END_PROGRAM would stop the program (src/util.cpp:117-124); the context stays usable."""
+    from nnpengine_amd.engine import Engine, NnpeError
+    e = Engine(water_c0)
+    with pytest.raises(NnpeError) as ei:
+        e.build()
+    assert ei.value.code == 3 and "nnpe_neigh_init" in str(ei.value)
+    e.neigh_init(RC, SKIN, RC, 0.0)
+    e.build()
+    o = _oracle(water_c0, 0)
+    o.rebuild(False)
+    assert np.array_equal(pyoracle.canonical(e.list_entries()), pyoracle.canonical(o.list_entries()))
+
+
 def test_pair_bonded_forces(water_c0):
     sysm = water_c0
     ft, ut = S.lj_table(sysm, RC, 900, S.WATER_LJ)
