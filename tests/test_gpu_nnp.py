@@ -57,6 +57,28 @@ def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
     assert abs(en["nn"][0] - et) < max(tol_e, 1e-6) * (np.abs(ea64).sum() + 1e-9) * 10
 
 
+@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3)])
+def test_mlp_ragged_tiles_mixed_widths(prec, tol_e, tol_g):
+    """300 O and 600 H atoms: the last 32-row tile of each element is partial (12 and 24 rows),
+    and hidden widths 128, 96, 32 leave one, then three, of the four column blocks idle in the
+    forward and backward passes."""
+    sysm = S.water_box(300, 21.0, seed=3)
+    widths = [80, 128, 96, 32, 1]
+    e, o, par, G64, norm = _setup(sysm, widths, seed=5)
+    e.sf_descriptor()
+    G = e.nnp_get("G").astype(np.float64)
+    e.mlp_set(widths, par, 0.5, prec, *norm)
+    e.energy_clear()
+    e.mlp_eval(True)
+    ea = e.nnp_get("e_atom").astype(np.float64)
+    dG = e.nnp_get("dG").astype(np.float64)
+    _, ea64, dG64 = o.mlp_eval(G)
+    print(f"[mlp] ragged prec={prec} E rel-rms {rel_rms(ea, ea64):.2e} dG rel-rms {rel_rms(dG, dG64):.2e}", flush=True)
+    assert np.all(np.isfinite(ea)) and np.all(np.isfinite(dG))
+    assert rel_rms(ea, ea64) < tol_e
+    assert rel_rms(dG, dG64) < tol_g
+
+
 def test_sf_forces_fp32(water_c0):
     widths = [80, 64, 64, 1]
     e, o, par, G64, norm = _setup(water_c0, widths)
