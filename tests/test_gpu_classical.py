@@ -181,8 +181,11 @@ def test_call_order_errors(water_c0):
     assert np.array_equal(pyoracle.canonical(e.list_entries()), pyoracle.canonical(o.list_entries()))
 
 
-def test_pair_bonded_forces(water_c0):
-    sysm = water_c0
+@pytest.mark.parametrize("case", ["c0", "ragged"])
+def test_pair_bonded_forces(case, water_c0):
+    """Tabulated pair + bonded forces, energies and virials; "ragged" is 900 atoms, so the last
+    workgroup of the pair pass is partial."""
+    sysm = water_c0 if case == "c0" else S.water_box(300, 21.0, seed=3)
     ft, ut = S.lj_table(sysm, RC, 900, S.WATER_LJ)
     e = _engine(sysm)
     e.build()
@@ -204,13 +207,20 @@ def test_pair_bonded_forces(water_c0):
         res[prec] = (o.get("f"), ep, eb)
     f32, ep32, eb32 = res[0]
     f64, ep64, eb64 = res[1]
-    print("force rel rms vs fp32 oracle", rel_rms(fg, f32), "vs fp64", rel_rms(fg, f64))
+    print(f"[pair] {case} force rel rms vs fp32 oracle", rel_rms(fg, f32), "vs fp64", rel_rms(fg, f64))
     assert rel_rms(fg, f32) < 2e-6
     assert rel_rms(fg, f64) < 1e-5
     e_gpu = en["pair"][0] + en["bond"][0] + en["angle"][0]
     w_gpu = en["pair"][1] + en["bond"][1]
-    assert abs(e_gpu - (ep64[0] + eb64[0])) < 1e-5 * abs(ep64[0] + eb64[0])
-    assert abs(w_gpu - (ep64[1] + eb64[1])) < 1e-5 * abs(ep64[1] + eb64[1])
+    # The totals cancel (Coulomb pairs of both signs) and each pair term is FP32, as in the
+    # reference (src/pair.cpp:289-419), so their rounding shows in the sum: the bar is 1e-5 of
+    # the total plus twice the FP32 oracle's own distance from FP64 (2.4e-5 of the total in the
+    # 900-atom box).
+    for k in (0, 1):
+        r64, r32 = ep64[k] + eb64[k], ep32[k] + eb32[k]
+        got = e_gpu if k == 0 else w_gpu
+        print(f"[pair] {case} {'energy' if k == 0 else 'virial'} {got:.6f} fp64 {r64:.6f} fp32 oracle {r32:.6f}")
+        assert abs(got - r64) < 1e-5 * abs(r64) + 2.0 * abs(r32 - r64)
 
 
 def test_md_steps_match_fp32_oracle(water_c0):

@@ -79,9 +79,13 @@ def test_mlp_ragged_tiles_mixed_widths(prec, tol_e, tol_g):
     assert rel_rms(dG, dG64) < tol_g
 
 
-def test_sf_forces_fp32(water_c0):
+@pytest.mark.parametrize("case", ["c0", "ragged"])
+def test_sf_forces_fp32(case, water_c0):
+    """FP32 NNP forces, energy and virial vs FP64; "ragged" is 900 atoms, so the last 16-lane
+    group of the descriptor and force passes is partial."""
+    sysm = water_c0 if case == "c0" else S.water_box(300, 21.0, seed=3)
     widths = [80, 64, 64, 1]
-    e, o, par, G64, norm = _setup(water_c0, widths)
+    e, o, par, G64, norm = _setup(sysm, widths)
     e.mlp_set(widths, par, 0.5, "fp32", *norm)
     e.sf_descriptor()
     e.clear_force()
@@ -94,7 +98,7 @@ def test_sf_forces_fp32(water_c0):
     o.clear_f()
     w64 = o.sf_force(dG64)
     f64 = o.get("f")
-    print(f"[sf] force rel-rms {rel_rms(fg, f64):.2e} virial {en['sf'][1]:.6f} vs {w64:.6f}")
+    print(f"[sf] {case} force rel-rms {rel_rms(fg, f64):.2e} virial {en['sf'][1]:.6f} vs {w64:.6f}")
     assert rel_rms(fg, f64) < 1e-5
     assert abs(en["sf"][1] - w64) < 1e-4 * abs(w64) + 1e-6
     # the total is a sum of per-atom energies of both signs: bound it by their magnitude
