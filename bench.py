@@ -129,7 +129,7 @@ def serving(par, norm, nreq, steps, seed=2027):
     with ProcessPoolExecutor(max_workers=8) as ex:
         boxes = list(ex.map(_serve_box, [(int(n), seed + k) for k, n in enumerate(nmols)]))
     ref = boxes[0]
-    srv = Serve(ref, WIDTHS, par, norm[0], norm[1], precision="bf16", max_atoms=400000, max_boxes=256,
+    srv = Serve(ref, WIDTHS, par, norm[0], norm[1], precision=HEAD, max_atoms=400000, max_boxes=256,
                 pair_cutoff=RC_PAIR, skin=SKIN, nnp_cutoff=RC_NNP, rebuild_every=10, dt=DT, fp64=True)
     natom = np.array([b.natom for b in boxes])
     work = natom * steps
@@ -245,8 +245,12 @@ def hbm_traffic(stage):
 
 
 # the kernel each one-kernel stage launches in this workload (the rows of the VALU summary)
+# the timed BF16 form: one BF16 term per weight, activation and gradient ("bf16x1"); the split
+# hi + lo form ("bf16") is timed beside it
+HEAD = "bf16x1"
+
 STAGE_KERNEL = {"pair": "k_pair<false, 32>", "bonded": "k_bonded<false>", "sf_desc": "k_sf_desc<2, 16, true>",
-                "mlp": "k_mlp<1, 3>", "sf_adjoint": "k_sf_adjoint<2, true>",
+                "mlp": "k_mlp<3, 3>", "sf_adjoint": "k_sf_adjoint<2, true>",
                 "sf_force": "k_sf_force<2, false, 16, true, false, true>",
                 "verlet": "k_update_x64"}
 
@@ -314,7 +318,7 @@ def force_errors(sysm, par, norm):
         return float(np.sqrt(((a - b) ** 2).mean() / (b ** 2).mean()))
 
     out = {}
-    for name in ("fp32", "bf16", "fp16"):
+    for name in ("fp32", "bf16", "fp16", "bf16x1"):
         e = make_engine(sysm, par, norm, name)
         e.build()
         e.sf_descriptor()
@@ -370,7 +374,7 @@ def main():
 
     res = {}
     engines = {}
-    for prec in ("bf16", "fp32"):
+    for prec in ("bf16", "fp32", "bf16x1"):
         e = make_engine(sysm, par, norm, prec)
         t = time_steps(e, args.steps, args.warmup)
         res[prec] = t
@@ -379,7 +383,7 @@ def main():
         log(f"[bench] {prec}: {args.steps} steps in {t:.3f} s = {t / args.steps * 1e3:.4f} ms/step; stats {st}")
 
     # per-step latency distribution of the BF16 form (events around single graph replays)
-    eb = engines["bf16"]
+    eb = engines[HEAD]
     lat = []
     for _ in range(args.lat_samples):
         t0 = time.perf_counter()
@@ -393,6 +397,7 @@ def main():
                "copy"):
         stages[st] = eb.profile(st, reps=args.stage_reps)
     ms32_mlp = engines["fp32"].profile("mlp", reps=args.stage_reps)
+    ms_split_mlp = engines["bf16"].profile("mlp", reps=args.stage_reps)
     log("[bench] stage ms:", {k: round(v, 4) for k, v in stages.items()}, "fp32 mlp", round(ms32_mlp, 4))
 
     # roofline of the dominant kernel: the largest share of the step (stage time x launches per
@@ -464,22 +469,22 @@ def main():
         log("[bench] serving:", {k: v for k, v in serve.items() if k != "latency_s"}, serve["latency_s"])
     drift, x_end = {}, None
     if args.drift_ps > 0:
-        for prec in ("bf16", "fp16", "fp32"):
+        for prec in ("bf16", "fp16", "fp32", "bf16x1"):
             drift[prec], xe = nve_drift(sysm, par, norm, prec, args.drift_ps)
-            if prec == "bf16":
+            if prec == HEAD:
                 x_end = xe
             log(f"[bench] NVE {prec}: {drift[prec]}")
     errs = None
     if not args.no_err:
         errs = {"start": force_errors(sysm, par, norm)}
         if x_end is not None:
-            # the liquid after the BF16 NVE run (positions as the engine holds them)
+            # the liquid after the NVE run of the timed form (positions as the engine holds them)
             snap = sysm.copy()
             snap.x = np.ascontiguousarray(x_end, np.float32)
-            errs[f"after_{drift['bf16']['ps']:g}ps_bf16_nve"] = force_errors(snap, par, norm)
+            errs[f"after_{drift[HEAD]['ps']:g}ps_{HEAD}_nve"] = force_errors(snap, par, norm)
     cpu = None if args.no_cpu else cpu_baseline(sysm, par, norm)
 
-    t_b, t_f = res["bf16"], res["fp32"]
+    t_b, t_f = res[HEAD], res["fp32"]
     value = N * args.steps / t_b
     out = {
         "metric": "NNP-MD atom-steps/s @24k atoms, BF16-MFMA vs FP32 at stated err; p99 latency",
@@ -501,6 +506,10 @@ def main():
         "ns_per_day": args.steps / t_b * DT * 0.0864 / 1.0,
         "fp32_form": {"value": N * args.steps / t_f, "ms_per_step": t_f / args.steps * 1e3},
         "bf16_speedup_vs_fp32": t_f / t_b,
+        # the BF16 network with weights (and backward gradients) as hi + lo BF16 pairs: ten times
+        # smaller NNP force error, slower network
+        "bf16_split_form": {"value": N * args.steps / res["bf16"], "ms_per_step": res["bf16"] / args.steps * 1e3,
+                            "speedup_vs_fp32": t_f / res["bf16"], "mlp_ms": ms_split_mlp},
         "force_error_vs_fp64": errs,
         "nve_drift": drift or None,
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},

@@ -182,7 +182,8 @@ int nnpe_nnp_set_dedg(nnpe_ctx* ctx, const float* dG, int D);
  * (input D, hidden widths multiples of 32, output 1); params per element, per layer:
  * W [in][out] row-major then b [out].  E_i = out_scale * network((G_i - in_shift) * in_scale)
  * with the input standardisation [ntype][D] applied in FP32 before any 16-bit rounding
- * (NULL = identity).  precision 0 = FP32, 1 = BF16, 2 = FP16 (MFMA, FP32 accumulation,
+ * (NULL = identity).  precision 0 = FP32, 1 = BF16 with weights (and backward gradients) as
+ * hi + lo BF16 pairs, 2 = FP16, 3 = BF16 with one term per value (MFMA, FP32 accumulation,
  * per-layer power-of-two scaling in the FP16 form). */
 int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const float* params, float out_scale,
                  int precision, const float* in_shift, const float* in_scale);

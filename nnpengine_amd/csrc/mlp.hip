@@ -89,6 +89,16 @@ template <> struct Prec<1> {  // BF16
   __device__ static T cvt(float v) { return (__bf16)v; }
   __device__ static float back(T v) { return (float)v; }
 };
+template <> struct Prec<3> {  // BF16, one term: the weights rounded once, no remainder parts
+  typedef __bf16 T;
+  typedef bf16x8 F;
+  static constexpr int KS = 16;
+  static constexpr int NB = 8;
+  __device__ static F ld(const T* p) { return *reinterpret_cast<const bf16x8*>(p); }
+  __device__ static f32x16 mma(F a, F b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+  __device__ static T cvt(float v) { return (__bf16)v; }
+  __device__ static float back(T v) { return (float)v; }
+};
 template <> struct Prec<2> {  // FP16
   typedef _Float16 T;
   typedef f16x8 F;
@@ -502,8 +512,8 @@ int type_index_enqueue(nnpe_ctx* ctx) {
 extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const float* params, float out_scale,
                             int precision, const float* in_shift, const float* in_scale) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
-  if (!ctx || nlayer < 2 || nlayer > kMaxLayers || !widths || !params || precision < 0 || precision > 2)
-    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_mlp_set: bad argument (2..6 weight layers, precision 0/1/2)");
+  if (!ctx || nlayer < 2 || nlayer > kMaxLayers || !widths || !params || precision < 0 || precision > 3)
+    return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_mlp_set: bad argument (2..6 weight layers, precision 0..3)");
   if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
   if (ctx->ntype > kMaxType) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "at most 4 elements");
   if (widths[nlayer] != 1) return ctx_fail(ctx, NNPE_ERR_ARG, "the output layer must have width 1");
@@ -616,7 +626,7 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
         } else {
           std::vector<uint16_t> hb(h.size());
           for (size_t k = 0; k < h.size(); ++k) {
-            if (precision == 1) {
+            if (precision == 1 || precision == 3) {
               const uint16_t hi = bf16_rne(h[k]);
               hb[k] = lo ? bf16_rne(h[k] - bf16_value(hi)) : hi;
             } else {
@@ -710,6 +720,7 @@ int mlp_enqueue(nnpe_ctx* ctx, bool ev) {
   }
     if (S->kind == 0) { NNPE_MLP_K(0) }
     else if (S->kind == 1) { NNPE_MLP_K(1) }
+    else if (S->kind == 3) { NNPE_MLP_K(3) }
     else { NNPE_MLP_K(2) }
 #undef NNPE_MLP_K
 #undef NNPE_MLP

@@ -264,7 +264,7 @@ class Engine:
 
     def mlp_set(self, widths, params, out_scale=1.0, precision="fp32", shift=None, scale=None):
         """shift, scale: [ntype][D] input standardisation x = (G - shift) * scale (None: identity)."""
-        prec = {"fp32": 0, "bf16": 1, "fp16": 2}[precision]
+        prec = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x1": 3}[precision]
         sh = None if shift is None else _f32(shift)
         sc = None if scale is None else _f32(scale)
         self._mlp = (_i32(widths), _f32(params), sh, sc)
@@ -353,7 +353,7 @@ class Serve:
                     flat=_i32(flat), mu=_f32(sfp["mu"]), eta=_f32(sfp["eta_a"]), widths=_i32(widths), params=_f32(params),
                     shift=_f32(shift), scale=_f32(scale), c0=coefs[0], c1=coefs[1], c2=coefs[2])
         self._keep = keep
-        prec = {"fp32": 0, "bf16": 1, "fp16": 2}[precision]
+        prec = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x1": 3}[precision]
         cfg = _lib.ServeConfig(device, max_atoms, max_boxes, sysm_ref.ntype, pair_cutoff, skin, nnp_cutoff, rebuild_every,
                                dt, int(fp64), pair_nbin, _p(keep["ft"], C.c_float), _p(keep["ut"], C.c_float),
                                len(coefs[0]), len(coefs[1]), len(coefs[2]), _p(keep["c0"], C.c_float),

@@ -395,8 +395,8 @@ NNPOTENTIAL_TBL::NNPOTENTIAL_TBL(numtype cf, int nbr, const char* paramfile, con
   for (float& v : par)
     if (std::fscanf(fp, "%f", &v) != 1) NNPE_END_PROGRAM("read error");
   std::fclose(fp);
-  const int p = !std::strcmp(prec, "fp32") ? 0 : !std::strcmp(prec, "bf16") ? 1 : !std::strcmp(prec, "fp16") ? 2 : -1;
-  if (p < 0) NNPE_END_PROGRAM("precision must be fp32, bf16 or fp16");
+  const int p = !std::strcmp(prec, "fp32") ? 0 : !std::strcmp(prec, "bf16") ? 1 : !std::strcmp(prec, "fp16") ? 2 : !std::strcmp(prec, "bf16x1") ? 3 : -1;
+  if (p < 0) NNPE_END_PROGRAM("precision must be fp32, bf16, fp16 or bf16x1");
   // weight file: bonded coefficients then the descriptor/pair coefficients (src/pair_nnp.cpp:505-517)
   fp = std::fopen(wtfile, "rb");
   if (!fp) NNPE_END_PROGRAM("read error");

@@ -1,6 +1,7 @@
 """GPU parity of the NNP path: symmetry-function descriptors and forces, the MFMA MLP in FP32 /
 BF16 / FP16, the reference's tabulated descriptor form, and the graph-captured MD loop.
-Tolerances (relative RMS vs the FP64 oracle): FP32 forms 1e-5; BF16 NNP forces 1e-2; FP16 5e-3."""
+Tolerances (relative RMS vs the FP64 oracle): FP32 forms 1e-5; BF16 NNP forces 1e-2 (both the
+split hi + lo form "bf16" and the one-term form "bf16x1"); FP16 5e-3."""
 import numpy as np
 import pytest
 
@@ -36,7 +37,7 @@ def _setup(sysm, widths, seed=11, out_scale=0.5):
     return e, o, par, G64, norm
 
 
-@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3)])
+@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3), ("bf16x1", 2e-2, 3e-2)])
 def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
     widths = [80, 64, 64, 1]
     e, o, par, G64, norm = _setup(water_c0, widths)
@@ -57,7 +58,7 @@ def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
     assert abs(en["nn"][0] - et) < max(tol_e, 1e-6) * (np.abs(ea64).sum() + 1e-9) * 10
 
 
-@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3)])
+@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3), ("bf16x1", 2e-2, 3e-2)])
 def test_mlp_ragged_tiles_mixed_widths(prec, tol_e, tol_g):
     """300 O and 600 H atoms: the last 32-row tile of each element is partial (12 and 24 rows),
     and hidden widths 128, 96, 32 leave one, then three, of the four column blocks idle in the
@@ -125,7 +126,7 @@ def test_sf_forces_bf16_error(water_c1):
     e.sf_force(False)
     print(f"[sf 24k] force kernel on fp64 dE/dG: rel-rms {rel_rms(e.get('f'), f64):.2e}", flush=True)
     errs = {}
-    for prec in ("fp32", "bf16", "fp16"):
+    for prec in ("fp32", "bf16", "fp16", "bf16x1"):
         e.mlp_set(widths, par, 0.5, prec, *norm)
         e.sf_descriptor()
         e.clear_force()
@@ -138,18 +139,20 @@ def test_sf_forces_bf16_error(water_c1):
     assert errs["fp32"] < 1e-5
     assert errs["bf16"] < 1e-2
     assert errs["fp16"] < 5e-3
+    assert errs["bf16x1"] < 1e-2
 
 
 def test_precision_forms_100_snapshots():
     """NNP force error of each precision form vs FP64 on 100 seeded 768-atom snapshots (the
     water lattice start with independent orientations per seed), 2 x 64 tanh network, one
-    standardisation (seed 0).  Bars on every snapshot: FP32 1e-5, BF16 1e-2, FP16 5e-3."""
+    standardisation (seed 0).  Bars on every snapshot: FP32 1e-5, BF16 (both forms) 1e-2,
+    FP16 5e-3."""
     from nnpengine_amd.engine import Engine
     widths = [80, 64, 64, 1]
     sfp = M.sf_params(RC_NNP)
     par = M.mlp_init(widths, 2, 5)
     norm = None
-    errs = {"fp32": [], "bf16": [], "fp16": []}
+    errs = {"fp32": [], "bf16": [], "fp16": [], "bf16x1": []}
     for seed in range(100):
         sysm = S.water_box(256, 19.73, seed=1000 + seed)
         o = pyoracle.Oracle(sysm, 1)
@@ -183,6 +186,7 @@ def test_precision_forms_100_snapshots():
     assert max(errs["fp32"]) < 1e-5
     assert max(errs["bf16"]) < 1e-2
     assert max(errs["fp16"]) < 5e-3
+    assert max(errs["bf16x1"]) < 1e-2
 
 
 def test_tbl_descriptor_and_force(water_c0):
