@@ -87,12 +87,15 @@ int nnpe_pair_table(nnpe_ctx* ctx, float cutoff, int nbin, const float* force, c
 /* LJTableCoulDSF::compute<ev,false> (src/pair.cpp:289-419).  Overwrites the force array. */
 int nnpe_pair_compute(nnpe_ctx* ctx, int ev);
 
-/* BOND / ANGLE / DIHEDRAL setParam + autoGenerate (src/bond.cpp:48-103, src/angle.cpp:25-82,
- * src/dihedral.cpp:46-104).  kind 0 bond (terms [n][3]), 1 angle ([n][4]), 2 dihedral ([n][5]);
- * terms are (type, atom ids...); coef [ntyp][4] in the classes' stored form. */
+/* BOND / ANGLE / DIHEDRAL setParam + autoGenerate (src/bond.cpp:27-82, src/angle.cpp:26-82,
+ * src/dihedral.cpp:27-85).  kind 0 bond (terms [n][3]), 1 angle ([n][4]), 2 dihedral ([n][5]);
+ * terms are (type, atom ids...); coef [ntyp][4] in the classes' stored form.  kind 3 (no
+ * counterpart in the reference, whose dihedral is cubic in cos phi): dihedral terms [n][5] with
+ * coef [ntyp][6] = C0..C5 of U = sum_n C_n cos^n(phi) (Ryckaert-Bellemans; phi = 0 cis, as the
+ * reference measures it); it replaces the kind-2 terms and their energy goes to the same slot. */
 int nnpe_bonded_set(nnpe_ctx* ctx, int kind, int ntyp, const float* coef, int nterm, const int* terms);
-/* BOND/ANGLE/DIHEDRAL::compute<ev> (src/bond.cpp:108-137, src/angle.cpp:87-136,
- * src/dihedral.cpp:109-166) on unwrapped molecule positions.  Adds to the force array. */
+/* BOND/ANGLE/DIHEDRAL::compute<ev> (src/bond.cpp:87-116, src/angle.cpp:87-136,
+ * src/dihedral.cpp:90-147) on unwrapped molecule positions.  Adds to the force array. */
 int nnpe_bonded_compute(nnpe_ctx* ctx, int ev);
 
 /* Energy and virial sums (FP64) of the ev compute calls since the last clear:

@@ -65,9 +65,11 @@ static void free_all(nnpe_ctx* c) {
   dev_free(c->nbr_long); dev_free(c->nbr_short); dev_free(c->cnt_long); dev_free(c->cnt_short);
   dev_free(c->scan_tmp); dev_free(c->ev_partial);
   dev_free(c->pt_tab);
-  dev_free(c->bcoef); dev_free(c->acoef); dev_free(c->dcoef);
-  dev_free(c->bterm); dev_free(c->aterm); dev_free(c->dterm);
-  dev_free(c->bstart); dev_free(c->astart); dev_free(c->dstart);
+  for (int k = 0; k < 3; ++k) {
+    dev_free(c->bd_coef[k]); dev_free(c->bd_type[k]); dev_free(c->bd_atoms[k]);
+    dev_free(c->bd_start[k]); dev_free(c->bd_ref[k]);
+    c->bd_start_len[k] = 0;
+  }
   dev_free(c->gr_count); dev_free(c->gr_shell); dev_free(c->gr_h); dev_free(c->gr_norm); dev_free(c->sq_out); dev_free(c->sq_ftmx);
   md_state_free(c);
   rmdf_state_free(c);
@@ -115,7 +117,7 @@ extern "C" int nnpe_system_set(nnpe_ctx* ctx, int natom, int ntype, float bl, co
   ctx->have_grid = ctx->have_list = ctx->have_pair = false;
   ctx->fp64 = false;
   ctx->md_step = 0;
-  ctx->nbond = ctx->nangle = ctx->ndih = 0;
+  for (int k = 0; k < 3; ++k) ctx->bd_nterm[k] = ctx->bd_ntyp[k] = 0;
   ctx->natom = natom; ctx->ntype = ntype; ctx->npair = ntype * (ntype + 1) / 2;
   ctx->nmol = nmol; ctx->nmoltype = nmoltype;
   ctx->stride = (natom + 63) / 64 * 64;

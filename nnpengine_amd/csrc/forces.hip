@@ -2,12 +2,17 @@
 //
 // Pair (LJTableCoulDSF::compute, src/pair.cpp:289-419): a group of kPairG lanes per atom walks
 // its full list row (G consecutive entries per step), and the group's butterfly sum is written
-// once per atom: no atomics, deterministic.  The F/r table sits in LDS; each directed entry adds half of the
-// pair energy and virial.  The arithmetic per pair is the reference's, in the same order:
+// once per atom: no atomics, deterministic.  The F/r table sits in LDS.  The force arithmetic per
+// pair is the reference's, in the same order:
 //   fpair = lerp(F/r)*s_lj + qi*qj*K*(1/(r*rsq) + fc/r)*s_coul
-// Bonded (src/bond.cpp:108-137, src/angle.cpp:87-136, src/dihedral.cpp:109-166): one thread
-// per molecule evaluates all of the molecule's terms on unwrapped positions (x + image shift of
-// the row's x_mol image), so every atom's bonded force has exactly one writer.
+// On energy steps every directed entry adds half of the pair energy and virial, evaluated in
+// FP64 from the FP32 positions (the forces stay FP32 and identical to the other steps).
+//
+// Bonded (BOND / ANGLE / DIHEDRAL::compute, src/bond.cpp:87-116, src/angle.cpp:87-136,
+// src/dihedral.cpp:90-147): one lane per atom.  Each term's energy is a polynomial of one
+// scalar s of its geometry (bond length, cosine of the angle, cosine of the dihedral); the lane
+// evaluates every term its atom belongs to and adds -U'(s) ds/dx for its own atom only, so each
+// atom's bonded force has one writer and a fixed summation order, and all rows work at once.
 #include <algorithm>
 #include <cmath>
 
@@ -18,8 +23,14 @@ namespace {
 // special-pair factors (src/molecule.h:11): class 0 full, classes 1 and 2 excluded, class 3 scaled
 constexpr float kSpecialLj3 = 0.5f, kSpecialCoul3 = float(5 / 6.);
 constexpr float kElectricK = 332.06371f;                                      // src/environ.h:77
+constexpr double kElectricK64 = 332.06371;
 constexpr int kPairG = 32;  // lanes per atom
 constexpr int kPairU = 4;   // list entries per lane in flight at once
+
+// cutoff constants of the energy pass in FP64 (LJTableCoulDSF ctor, src/pair.cpp:171-195)
+struct PairEv64 {
+  double cut, rdr, fc, vc;
+};
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   const int i = min(a, b), j = max(a, b);  // small non-negative values: 24-bit multiplies
@@ -31,7 +42,7 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
                                               const int* __restrict__ cnt, const float4* __restrict__ xq,
                                               const int* __restrict__ typ, const BoxPar* __restrict__ boxes,
                                               const int* __restrict__ rbox, int ntype, int npair,
-                                              float cutsq, float cut, float rdr, float fc, float vc, int nb1,
+                                              float cutsq, float rdr, float fc, PairEv64 P64, int nb1,
                                               const float* __restrict__ tab, float4* __restrict__ f, double* pe,
                                               double* pw) {
 #pragma clang fp contract(fast)
@@ -48,7 +59,8 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
     const int ti = typ[i];
     const BoxPar& B = boxes[rbox[i]];
     const float bl = B.bl;
-    float fx = 0, fy = 0, fz = 0, e = 0, w = 0;
+    float fx = 0, fy = 0, fz = 0;
+    double e = 0, w = 0;
     const int m = cnt[i];
     const uint32_t* row = list_row(nbr, cap, i);
     // kPairU entries per lane and their gathers are in flight together; the entries of the
@@ -100,11 +112,26 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
         fpair += qq * (ir * ir * ir + fc * ir) * scl;
         fx += (dx *= fpair); fy += (dy *= fpair); fz += (dz *= fpair);
         if (EV) {
+          // Energies and virials sum many terms of both signs that cancel; they are evaluated
+          // in FP64 (image shift, distance, table lerp, Coulomb) from the same FP32 inputs.
+          const float4 mi = s_img[code];
+          const double bd = double(bl);
+          const double ex = double(xi.x) - fma(double(mi.x), bd, double(xr[u].x));
+          const double ey = double(xi.y) - fma(double(mi.y), bd, double(xr[u].y));
+          const double ez = double(xi.z) - fma(double(mi.z), bd, double(xr[u].z));
+          const double rsq2 = ex * ex + ey * ey + ez * ez, r2 = sqrt(rsq2);
+          double h1 = r2 * P64.rdr - 0.5;
+          const int s1 = int(h1);
+          const double h2 = h1 - double(s1);
+          h1 = 1.0 - h2;
           const float* ut = tab + __mul24(npair + p, nb1);
-          // energies sum terms of both signs that cancel: correctly rounded r and 1/r here
-          const float re = sqrtf(rsq);
-          e += 0.5f * ((ut[t1] * ipw1 + ut[t2] * ipw2) * slj + qq * (1.0f / re + vc + fc * (cut - re)) * scl);
-          w += 0.5f * fpair * rsq;
+          const double q2 = double(xi.w) * double(xj.w) * kElectricK64;
+          const double u2 = (double(ut[s1]) * h1 + double(ut[s1 + 1]) * h2) * slj +
+                            q2 * (1.0 / r2 + P64.vc + P64.fc * (P64.cut - r2)) * scl;
+          const double fp2 = (double(ft[s1]) * h1 + double(ft[s1 + 1]) * h2) * slj +
+                             q2 * (1.0 / (r2 * rsq2) + P64.fc / r2) * scl;
+          e += 0.5 * u2;
+          w += 0.5 * fp2 * rsq2;
         }
       }
     }
@@ -115,110 +142,170 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
   if (EV) block_sum2_store<256>(e_acc, w_acc, pe, pw);
 }
 
-__device__ __forceinline__ float4 xmol_pos(int aid, const int* inv, const float4* xq, const int* xcode,
-                                           const BoxPar* boxes, const int* rbox) {
-  const int r = inv[aid];
-  const BoxPar& B = boxes[rbox[r]];
-  return image_pos(xq[r], xcode[r], B.bl, B.m_bl);
+// ---- bonded terms ------------------------------------------------------------------------------
+// The energy functions and the stored coefficients are the reference's (src/bond.cpp:27-34,
+// src/angle.cpp:26-33, src/dihedral.cpp:27-35), the evaluation is this file's own:
+//   bond      s = r,          stored (r0, 2a, b, 2c):   U = (x^2 / 2) (2a (b - x)^2 + 2c),  x = s - r0
+//   angle     s = cos(t),     stored (cos t0, 2a, b, 2c): the same quartic in x = s - cos t0
+//   dihedral  s = cos(phi),   U = e0 + e1 s + ... + e5 s^5 (the reference's cubic, or the
+//             Ryckaert-Bellemans quintic when the coefficients are given that way), phi the angle
+//             between the planes (i, j, k) and (j, k, l), 0 for cis.
+
+// dU/dx of the quartic well: x (2c + 2a (b - x)(b - 2x))
+__device__ __forceinline__ float well_slope(float x, float4 c) {
+  const float bx = c.z - x;
+  return x * (c.w + c.y * bx * (bx - x));
 }
+__device__ __forceinline__ double well_energy64(double x, float4 c) {
+  const double bx = double(c.z) - x;
+  return 0.5 * x * x * (double(c.y) * bx * bx + double(c.w));
+}
+__device__ __forceinline__ double well_slope64(double x, float4 c) {
+  const double bx = double(c.z) - x;
+  return x * (double(c.w) + double(c.y) * bx * (bx - x));
+}
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 to_d3(V3 a) { return {double(a.x), double(a.y), double(a.z)}; }
+__device__ __forceinline__ D3 sub64(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double dot64(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross64(D3 a, D3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// cos(phi) of the dihedral i-j-k-l from the plane normals m = (j - i) x (k - j) and
+// n = (k - j) x (l - k), and the gradient of cos(phi) with respect to atom `slot` (0..3):
+//   P = d cos / d m = n / (|m||n|) - cos m / |m|^2,   Q = d cos / d n = m / (|m||n|) - cos n / |n|^2,
+//   d cos / d x_i = -(b x P),  d cos / d x_j = b x P - P x a - c x Q,
+//   d cos / d x_k = P x a + c x Q - Q x b,  d cos / d x_l = Q x b
+// with a = x_j - x_i, b = x_k - x_j, c = x_l - x_k.
+__device__ __forceinline__ float dihedral_cos_grad(V3 xi, V3 xj, V3 xk, V3 xl, int slot, V3* grad) {
+  const V3 a = xj - xi, b = xk - xj, c = xl - xk;
+  const V3 m = cross3(a, b), n = cross3(b, c);
+  const float mm = dot3(m, m), nn = dot3(n, n);
+  const float rmn = 1.0f / sqrtf(mm * nn);
+  const float cs = dot3(m, n) * rmn;
+  const V3 P = rmn * n - (cs / mm) * m;
+  const V3 Q = rmn * m - (cs / nn) * n;
+  const V3 bP = cross3(b, P), Pa = cross3(P, a), cQ = cross3(c, Q), Qb = cross3(Q, b);
+  // slot weights of the four products: i (-1,0,0,0), j (1,-1,-1,0), k (0,1,1,-1), l (0,0,0,1)
+  const float wb = slot == 0 ? -1.0f : (slot == 1 ? 1.0f : 0.0f);
+  const float wa = slot == 1 ? -1.0f : (slot == 2 ? 1.0f : 0.0f);
+  const float wq = slot == 3 ? 1.0f : (slot == 2 ? -1.0f : 0.0f);
+  *grad = wb * bP + wa * Pa + wa * cQ + wq * Qb;
+  return cs;
+}
+
+// Argument block of the bonded pass: per kind, the membership table and the terms.
+struct BondedArgs {
+  const int* start[3];
+  const int* ref[3];
+  const int* type[3];
+  const int* atoms[3];
+  const float* coef[3];
+};
 
 template <bool EV>
-__global__ void __launch_bounds__(256) k_bonded(int nmol, const int* __restrict__ inv, const float4* __restrict__ xq,
-                                                const int* __restrict__ xcode, const BoxPar* __restrict__ boxes,
-                                                const int* __restrict__ rbox,
-                                                const int* __restrict__ bstart, const int* __restrict__ bterm,
-                                                const float* __restrict__ bcoef, const int* __restrict__ astart,
-                                                const int* __restrict__ aterm, const float* __restrict__ acoef,
-                                                const int* __restrict__ dstart, const int* __restrict__ dterm,
-                                                const float* __restrict__ dcoef, float4* __restrict__ f, double* pe_b,
-                                                double* pw_b, double* pe_a, double* pw_a, double* pe_d, double* pw_d) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64) k_bonded(int n, BondedArgs A, const int4* __restrict__ info,
+                                               const int* __restrict__ inv, const float4* __restrict__ xq,
+                                               const int* __restrict__ xcode, const BoxPar* __restrict__ boxes,
+                                               const int* __restrict__ rbox, float4* __restrict__ f, double* pe_b,
+                                               double* pw_b, double* pe_a, double* pw_a, double* pe_d, double* pw_d) {
+  const int i = xcd_block(gridDim.x) * 64 + (int)threadIdx.x;
   double eb = 0, wb = 0, ea = 0, ed = 0;
-  if (m < nmol) {
-    // BOND::compute (src/bond.cpp:108-137)
-    for (int t = bstart[m]; t < bstart[m + 1]; ++t) {
-      const int* p = bterm + 3 * t;
-      const float* icf = bcoef + 4 * p[0];
-      const float4 xi = xmol_pos(p[1], inv, xq, xcode, boxes, rbox), xj = xmol_pos(p[2], inv, xq, xcode, boxes, rbox);
-      float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-      const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
-      const float dr = rr - icf[0], bdr = icf[2] - dr;
-      const float ff = dr * (icf[3] + icf[1] * bdr * (bdr - dr)) / rr;
-      dx *= ff; dy *= ff; dz *= ff;
-      float4* fi = f + inv[p[1]];
-      float4* fj = f + inv[p[2]];
-      fi->x -= dx; fi->y -= dy; fi->z -= dz;
-      fj->x += dx; fj->y += dy; fj->z += dz;
-      if (EV) { eb += 0.5f * dr * dr * (icf[1] * bdr * bdr + icf[3]); wb -= ff * rr * rr; }
+  if (i < n) {
+    const int aid = info[i].x;
+    const BoxPar& B = boxes[rbox[i]];
+    // unwrapped molecule coordinates (x_mol): every atom of a term is in this atom's molecule
+    auto mol_pos = [&](int a) -> V3 {
+      const int r = inv[a];
+      const float4 p = image_pos(xq[r], xcode[r], B.bl, B.m_bl);
+      return {p.x, p.y, p.z};
+    };
+    V3 g = {0.0f, 0.0f, 0.0f};
+    if (A.start[0]) {
+      for (int k = A.start[0][aid], ke = A.start[0][aid + 1]; k < ke; ++k) {
+        const int ref = A.ref[0][k], t = ref >> 2, slot = ref & 3;
+        const float4 c = reinterpret_cast<const float4*>(A.coef[0])[A.type[0][t]];
+        const V3 p0 = mol_pos(A.atoms[0][2 * t]), p1 = mol_pos(A.atoms[0][2 * t + 1]);
+        const V3 d = p0 - p1;  // ds/dx_0 = d / r, ds/dx_1 = -d / r
+        const float r = sqrtf(dot3(d, d));
+        const float k1 = well_slope(r - c.x, c) / r;
+        g = g + (slot == 0 ? -k1 : k1) * d;
+        if (EV && slot == 0) {
+          const D3 dd = sub64(to_d3(p0), to_d3(p1));
+          const double r2 = sqrt(dot64(dd, dd));
+          eb += well_energy64(r2 - double(c.x), c);
+          wb -= well_slope64(r2 - double(c.x), c) * r2;
+        }
+      }
     }
-    // ANGLE::compute (src/angle.cpp:87-136)
-    for (int t = astart[m]; t < astart[m + 1]; ++t) {
-      const int* p = aterm + 4 * t;
-      const float* icf = acoef + 4 * p[0];
-      const float4 xi = xmol_pos(p[1], inv, xq, xcode, boxes, rbox), xj = xmol_pos(p[2], inv, xq, xcode, boxes, rbox),
-                   xk = xmol_pos(p[3], inv, xq, xcode, boxes, rbox);
-      float dx1 = xi.x - xj.x, dy1 = xi.y - xj.y, dz1 = xi.z - xj.z;
-      float dx2 = xk.x - xj.x, dy2 = xk.y - xj.y, dz2 = xk.z - xj.z;
-      const float r1 = 1.0f / sqrtf(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
-      const float r2 = 1.0f / sqrtf(dx2 * dx2 + dy2 * dy2 + dz2 * dz2);
-      dx1 *= r1; dy1 *= r1; dz1 *= r1; dx2 *= r2; dy2 *= r2; dz2 *= r2;
-      const float cq = dx1 * dx2 + dy1 * dy2 + dz1 * dz2;
-      const float dcq = cq - icf[0], b_dcq = icf[2] - dcq;
-      const float fe = dcq * (icf[3] + icf[1] * b_dcq * (b_dcq - dcq));
-      float ff = fe * r1;
-      float fx = ff * (cq * dx1 - dx2), fy = ff * (cq * dy1 - dy2), fz = ff * (cq * dz1 - dz2);
-      float4* fi = f + inv[p[1]];
-      float4* fj = f + inv[p[2]];
-      float4* fk = f + inv[p[3]];
-      fi->x += fx; fi->y += fy; fi->z += fz; fj->x -= fx; fj->y -= fy; fj->z -= fz;
-      ff = fe * r2;
-      fx = ff * (dx1 - cq * dx2); fy = ff * (dy1 - cq * dy2); fz = ff * (dz1 - cq * dz2);
-      fk->x -= fx; fk->y -= fy; fk->z -= fz; fj->x += fx; fj->y += fy; fj->z += fz;
-      if (EV) ea += 0.5f * dcq * dcq * (icf[1] * b_dcq * b_dcq + icf[3]);
+    if (A.start[1]) {
+      for (int k = A.start[1][aid], ke = A.start[1][aid + 1]; k < ke; ++k) {
+        const int ref = A.ref[1][k], t = ref >> 2, slot = ref & 3;
+        const float4 c = reinterpret_cast<const float4*>(A.coef[1])[A.type[1][t]];
+        const int* at = A.atoms[1] + 3 * t;  // end, vertex, end
+        const V3 pv = mol_pos(at[1]);
+        const V3 u = mol_pos(at[0]) - pv, v = mol_pos(at[2]) - pv;
+        const float iu = 1.0f / sqrtf(dot3(u, u)), iv = 1.0f / sqrtf(dot3(v, v));
+        const V3 uh = iu * u, vh = iv * v;
+        const float cs = dot3(uh, vh);
+        const V3 g0 = iu * (vh - cs * uh), g2 = iv * (uh - cs * vh);  // d cos / d x_end
+        const V3 gs = slot == 0 ? g0 : (slot == 2 ? g2 : -1.0f * (g0 + g2));
+        g = g + (-well_slope(cs - c.x, c)) * gs;
+        if (EV && slot == 0) {
+          const D3 pv2 = to_d3(pv);
+          const D3 u2 = sub64(to_d3(mol_pos(at[0])), pv2), v2 = sub64(to_d3(mol_pos(at[2])), pv2);
+          const double c2 = dot64(u2, v2) / sqrt(dot64(u2, u2) * dot64(v2, v2));
+          ea += well_energy64(c2 - double(c.x), c);
+        }
+      }
     }
-    // DIHEDRAL::compute (src/dihedral.cpp:109-166)
-    for (int t = dstart[m]; t < dstart[m + 1]; ++t) {
-      const int* p = dterm + 5 * t;
-      const float* icf = dcoef + 4 * p[0];
-      const float4 xi = xmol_pos(p[1], inv, xq, xcode, boxes, rbox), xj = xmol_pos(p[2], inv, xq, xcode, boxes, rbox),
-                   xk = xmol_pos(p[3], inv, xq, xcode, boxes, rbox), xl = xmol_pos(p[4], inv, xq, xcode, boxes, rbox);
-      float dx1 = xi.x - xj.x, dy1 = xi.y - xj.y, dz1 = xi.z - xj.z;
-      const float dx2 = xk.x - xj.x, dy2 = xk.y - xj.y, dz2 = xk.z - xj.z;
-      float dx3 = xl.x - xk.x, dy3 = xl.y - xk.y, dz3 = xl.z - xk.z;
-      const float r_rsq2 = 1.0f / (dx2 * dx2 + dy2 * dy2 + dz2 * dz2);
-      const float dot12 = (dx1 * dx2 + dy1 * dy2 + dz1 * dz2) * r_rsq2;
-      const float dot32 = (dx3 * dx2 + dy3 * dy2 + dz3 * dz2) * r_rsq2;
-      dx1 -= dx2 * dot12; dy1 -= dy2 * dot12; dz1 -= dz2 * dot12;
-      dx3 -= dx2 * dot32; dy3 -= dy2 * dot32; dz3 -= dz2 * dot32;
-      float r1 = 1.0f / sqrtf(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
-      float r3 = 1.0f / sqrtf(dx3 * dx3 + dy3 * dy3 + dz3 * dz3);
-      dx1 *= r1; dy1 *= r1; dz1 *= r1; dx3 *= r3; dy3 *= r3; dz3 *= r3;
-      const float cq = dx1 * dx3 + dy1 * dy3 + dz1 * dz3;
-      const float ff = (icf[3] * cq + icf[2]) * cq + icf[1];
-      r1 *= ff; r3 *= ff;
-      const float f1x = r1 * (cq * dx1 - dx3), f1y = r1 * (cq * dy1 - dy3), f1z = r1 * (cq * dz1 - dz3);
-      const float f3x = r3 * (cq * dx3 - dx1), f3y = r3 * (cq * dy3 - dy1), f3z = r3 * (cq * dz3 - dz1);
-      const float f2x = f1x * dot12 + f3x * dot32, f2y = f1y * dot12 + f3y * dot32, f2z = f1z * dot12 + f3z * dot32;
-      float4* fi = f + inv[p[1]];
-      float4* fj = f + inv[p[2]];
-      float4* fk = f + inv[p[3]];
-      float4* fl = f + inv[p[4]];
-      fi->x += f1x; fi->y += f1y; fi->z += f1z;
-      fl->x += f3x; fl->y += f3y; fl->z += f3z;
-      fj->x += f2x - f1x; fj->y += f2y - f1y; fj->z += f2z - f1z;
-      fk->x -= f2x + f3x; fk->y -= f2y + f3y; fk->z -= f2z + f3z;
-      if (EV) ed += ((icf[3] / 3.0f * cq + icf[2] / 2.0f) * cq + icf[1]) * cq + icf[0];
+    if (A.start[2]) {
+      for (int k = A.start[2][aid], ke = A.start[2][aid + 1]; k < ke; ++k) {
+        const int ref = A.ref[2][k], t = ref >> 2, slot = ref & 3;
+        const float4* c4 = reinterpret_cast<const float4*>(A.coef[2]) + 3 * A.type[2][t];
+        const float4 c0 = c4[0], c1 = c4[1], c2 = c4[2];  // (d0..d3), (d4, e0, e1, e2), (e3, e4, e5, -)
+        const int* at = A.atoms[2] + 4 * t;
+        const V3 xi = mol_pos(at[0]), xj = mol_pos(at[1]), xk = mol_pos(at[2]), xl = mol_pos(at[3]);
+        V3 gs;
+        const float cs = dihedral_cos_grad(xi, xj, xk, xl, slot, &gs);
+        const float dU = (((c1.x * cs + c0.w) * cs + c0.z) * cs + c0.y) * cs + c0.x;
+        g = g + (-dU) * gs;
+        if (EV && slot == 0) {
+          const D3 a = sub64(to_d3(xj), to_d3(xi)), b = sub64(to_d3(xk), to_d3(xj)), cc = sub64(to_d3(xl), to_d3(xk));
+          const D3 m = cross64(a, b), nv = cross64(b, cc);
+          const double s = dot64(m, nv) / sqrt(dot64(m, m) * dot64(nv, nv));
+          ed += ((((double(c2.z) * s + double(c2.y)) * s + double(c2.x)) * s + double(c1.w)) * s + double(c1.z)) * s +
+                double(c1.y);
+        }
+      }
     }
+    float4 fo = f[i];
+    fo.x += g.x; fo.y += g.y; fo.z += g.z;
+    f[i] = fo;
   }
   if (EV) {
-    block_sum2_store<256>(eb, wb, pe_b, pw_b);
+    block_sum2_store<64>(eb, wb, pe_b, pw_b);
     __syncthreads();
-    block_sum2_store<256>(ea, 0.0, pe_a, pw_a);
+    block_sum2_store<64>(ea, 0.0, pe_a, pw_a);
     __syncthreads();
-    block_sum2_store<256>(ed, 0.0, pe_d, pw_d);
+    block_sum2_store<64>(ed, 0.0, pe_d, pw_d);
   }
 }
-
 }  // namespace
 
 extern "C" int nnpe_pair_table(nnpe_ctx* ctx, float cutoff, int nbin, const float* force, const float* potential) {
@@ -244,17 +331,22 @@ extern "C" int nnpe_pair_table(nnpe_ctx* ctx, float cutoff, int nbin, const floa
 
 int pair_enqueue(nnpe_ctx* ctx, bool ev) {
   const int N = ctx->natom, nb1 = ctx->pt_nbin + 1, nblk = blocks_for((long long)N * kPairG, 256);
+  PairEv64 P64;
+  P64.cut = double(ctx->pt_cut);
+  P64.rdr = double(ctx->pt_nbin) / P64.cut;
+  P64.vc = -1.0 / P64.cut;
+  P64.fc = P64.vc / P64.cut;
   const size_t lds = size_t(ctx->npair) * nb1 * sizeof(float);
   double* pe = ev_part(ctx, NNPE_TERM_PAIR, 0);
   double* pw = ev_part(ctx, NNPE_TERM_PAIR, 1);
   if (ev)
     hipLaunchKernelGGL((k_pair<true, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
                        ctx->cnt_long, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, ctx->ntype, ctx->npair, ctx->pt_cutsq,
-                       ctx->pt_cut, ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
+                       ctx->pt_rdr, ctx->pt_fc, P64, nb1, ctx->pt_tab, ctx->f, pe, pw);
   else
     hipLaunchKernelGGL((k_pair<false, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
                        ctx->cnt_long, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, ctx->ntype, ctx->npair, ctx->pt_cutsq,
-                       ctx->pt_cut, ctx->pt_rdr, ctx->pt_fc, ctx->pt_vc, nb1, ctx->pt_tab, ctx->f, pe, pw);
+                       ctx->pt_rdr, ctx->pt_fc, P64, nb1, ctx->pt_tab, ctx->f, pe, pw);
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_PAIR, nblk);
   return NNPE_OK;
@@ -267,77 +359,106 @@ extern "C" int nnpe_pair_compute(nnpe_ctx* ctx, int ev) {
   return pair_enqueue(ctx, ev != 0);
 }
 
+// kind 0 bonds, 1 angles, 2 dihedrals in the reference's cubic form, 3 dihedrals as an energy
+// polynomial of degree 5 in cos(phi) (Ryckaert-Bellemans); 3 replaces the dihedral terms.
 extern "C" int nnpe_bonded_set(nnpe_ctx* ctx, int kind, int ntyp, const float* coef, int nterm, const int* terms) {
   if (ctx) md_state_free(ctx);  // captured step graphs hold the old parameters
-  if (!ctx || kind < 0 || kind > 2 || ntyp < 0 || nterm < 0 || (nterm && (!coef || !terms)))
+  if (!ctx || kind < 0 || kind > 3 || ntyp < 0 || nterm < 0 || (nterm && (!coef || !terms)) || (ntyp && !coef))
     return ctx_fail(ctx, NNPE_ERR_ARG, "nnpe_bonded_set: bad argument");
   if (!ctx->have_system) return ctx_fail(ctx, NNPE_ERR_STATE, "no system");
-  const int w = 3 + kind;  // ints per term
-  // per-molecule offsets; terms must be grouped by molecule (autoGenerate order,
-  // src/bond.cpp:66-86); every atom of a term must belong to the same molecule
-  std::vector<int> hmol(ctx->natom);
-  {
-    std::vector<int4> hinfo(ctx->natom);
+  const int slot = kind == 3 ? 2 : kind;  // storage slot
+  const int na = 2 + slot;                // atoms per term
+  const int N = ctx->natom;
+  // every atom of a term must belong to one molecule (the unwrapped molecule coordinates)
+  std::vector<int> hmol(N);
+  if (nterm) {
+    std::vector<int4> hinfo(N);
     NNPE_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    NNPE_CHECK_HIP(ctx, hipMemcpy(hinfo.data(), ctx->info, sizeof(int4) * ctx->natom, hipMemcpyDeviceToHost));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(hinfo.data(), ctx->info, sizeof(int4) * N, hipMemcpyDeviceToHost));
     for (auto& in : hinfo) hmol[in.x] = in.y;
   }
-  std::vector<int> start(ctx->nmol + 1, 0);
-  int prev = -1;
+  std::vector<int> type(std::max(nterm, 1)), atoms(size_t(na) * std::max(nterm, 1)), start(N + 1, 0);
   for (int t = 0; t < nterm; ++t) {
-    const int* p = terms + w * t;
+    const int* p = terms + (na + 1) * t;
     if (p[0] < 0 || p[0] >= ntyp) return ctx_fail(ctx, NNPE_ERR_ARG, "auto generate error: term type out of range");
-    for (int k = 1; k < w; ++k)
-      if (p[k] < 0 || p[k] >= ctx->natom || hmol[p[k]] != hmol[p[1]]) return ctx_fail(ctx, NNPE_ERR_ARG, "term atoms outside one molecule");
-    int m = hmol[p[1]];
-    if (m < prev) return ctx_fail(ctx, NNPE_ERR_ARG, "terms not grouped by molecule");
-    prev = m;
-    start[m + 1]++;
+    for (int k = 1; k <= na; ++k)
+      if (p[k] < 0 || p[k] >= N || hmol[p[k]] != hmol[p[1]]) return ctx_fail(ctx, NNPE_ERR_ARG, "term atoms outside one molecule");
+    type[t] = p[0];
+    for (int k = 0; k < na; ++k) {
+      atoms[size_t(na) * t + k] = p[1 + k];
+      start[p[1 + k] + 1]++;
+    }
   }
-  for (int m = 0; m < ctx->nmol; ++m) start[m + 1] += start[m];
-  int** pterm = kind == 0 ? &ctx->bterm : kind == 1 ? &ctx->aterm : &ctx->dterm;
-  int** pstart = kind == 0 ? &ctx->bstart : kind == 1 ? &ctx->astart : &ctx->dstart;
-  float** pcoef = kind == 0 ? &ctx->bcoef : kind == 1 ? &ctx->acoef : &ctx->dcoef;
-  NNPE_CHECK_HIP(ctx, dev_alloc(*pterm, size_t(w) * std::max(nterm, 1)));
-  if (nterm) NNPE_CHECK_HIP(ctx, hipMemcpy(*pterm, terms, sizeof(int) * w * nterm, hipMemcpyHostToDevice));
-  NNPE_CHECK_HIP(ctx, dev_alloc(*pstart, start.size()));
-  NNPE_CHECK_HIP(ctx, hipMemcpy(*pstart, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
-  ctx->start_len[kind] = int(start.size());
-  NNPE_CHECK_HIP(ctx, dev_alloc(*pcoef, size_t(4) * std::max(ntyp, 1)));
-  if (ntyp) NNPE_CHECK_HIP(ctx, hipMemcpy(*pcoef, coef, sizeof(float) * 4 * ntyp, hipMemcpyHostToDevice));
-  if (kind == 0) { ctx->nbtyp = ntyp; ctx->nbond = nterm; }
-  else if (kind == 1) { ctx->natyp = ntyp; ctx->nangle = nterm; }
-  else { ctx->ndtyp = ntyp; ctx->ndih = nterm; }
+  for (int a = 0; a < N; ++a) start[a + 1] += start[a];
+  std::vector<int> ref(std::max(start[N], 1)), fill(start.begin(), start.end() - 1);
+  for (int t = 0; t < nterm; ++t)
+    for (int k = 0; k < na; ++k) ref[fill[atoms[size_t(na) * t + k]]++] = (t << 2) | k;
+  // coefficients in the kernel's form
+  const int cw = slot == 2 ? 12 : 4;
+  std::vector<float> c(size_t(cw) * std::max(ntyp, 1), 0.0f);
+  for (int y = 0; y < ntyp; ++y) {
+    float* o = c.data() + size_t(cw) * y;
+    if (slot < 2) {
+      for (int k = 0; k < 4; ++k) o[k] = coef[4 * y + k];
+    } else if (kind == 2) {
+      // stored (k0, k1, k2, k3) (src/dihedral.cpp:27-35): U = k0 + k1 s + k2/2 s^2 + k3/3 s^3
+      const float* k = coef + 4 * y;
+      const float d[5] = {k[1], k[2], k[3], 0.0f, 0.0f}, e[6] = {k[0], k[1], k[2] / 2.0f, k[3] / 3.0f, 0.0f, 0.0f};
+      for (int q = 0; q < 5; ++q) o[q] = d[q];
+      for (int q = 0; q < 6; ++q) o[5 + q] = e[q];
+    } else {
+      // U = C0 + C1 s + ... + C5 s^5
+      const float* e = coef + 6 * y;
+      for (int q = 0; q < 5; ++q) o[q] = float(q + 1) * e[q + 1];
+      for (int q = 0; q < 6; ++q) o[5 + q] = e[q];
+    }
+  }
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->bd_coef[slot], c.size()));
+  NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->bd_coef[slot], c.data(), c.size() * sizeof(float), hipMemcpyHostToDevice));
+  if (nterm) {
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->bd_type[slot], type.size()));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->bd_type[slot], type.data(), type.size() * sizeof(int), hipMemcpyHostToDevice));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->bd_atoms[slot], atoms.size()));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->bd_atoms[slot], atoms.data(), atoms.size() * sizeof(int), hipMemcpyHostToDevice));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->bd_start[slot], start.size()));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->bd_start[slot], start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+    NNPE_CHECK_HIP(ctx, dev_alloc(ctx->bd_ref[slot], ref.size()));
+    NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->bd_ref[slot], ref.data(), ref.size() * sizeof(int), hipMemcpyHostToDevice));
+    ctx->bd_start_len[slot] = N + 1;
+  } else {
+    dev_free(ctx->bd_type[slot]); dev_free(ctx->bd_atoms[slot]); dev_free(ctx->bd_start[slot]); dev_free(ctx->bd_ref[slot]);
+    ctx->bd_start_len[slot] = 0;
+  }
+  ctx->bd_ntyp[slot] = ntyp;
+  ctx->bd_nterm[slot] = nterm;
   return NNPE_OK;
 }
 
 int bonded_enqueue(nnpe_ctx* ctx, bool ev) {
-  // the kernel reads offsets [0, nmol]: kinds without terms get an all-zero table of that size
-  // (again when the molecule count changed); a kind whose terms were set for another molecule
-  // table is an error
-  for (int kind = 0; kind < 3; ++kind) {
-    int** pstart = kind == 0 ? &ctx->bstart : kind == 1 ? &ctx->astart : &ctx->dstart;
-    const int nterm = kind == 0 ? ctx->nbond : kind == 1 ? ctx->nangle : ctx->ndih;
-    if (*pstart && ctx->start_len[kind] == ctx->nmol + 1) continue;
-    if (nterm > 0) return ctx_fail(ctx, NNPE_ERR_STATE, "bonded terms were set for another molecule count");
-    std::vector<int> z(ctx->nmol + 1, 0);
-    NNPE_CHECK_HIP(ctx, dev_alloc(*pstart, z.size()));
-    NNPE_CHECK_HIP(ctx, hipMemcpy(*pstart, z.data(), sizeof(int) * z.size(), hipMemcpyHostToDevice));
-    ctx->start_len[kind] = int(z.size());
+  BondedArgs A;
+  int total = 0;
+  for (int k = 0; k < 3; ++k) {
+    const bool on = ctx->bd_nterm[k] > 0;
+    if (on && ctx->bd_start_len[k] != ctx->natom + 1)
+      return ctx_fail(ctx, NNPE_ERR_STATE, "bonded terms were set for another atom count");
+    A.start[k] = on ? ctx->bd_start[k] : nullptr;
+    A.ref[k] = ctx->bd_ref[k];
+    A.type[k] = ctx->bd_type[k];
+    A.atoms[k] = ctx->bd_atoms[k];
+    A.coef[k] = ctx->bd_coef[k];
+    total += ctx->bd_nterm[k];
   }
-  if (ctx->nbond + ctx->nangle + ctx->ndih == 0) return NNPE_OK;
-  const int nblk = blocks_for(ctx->nmol, 256);
+  if (total == 0) return NNPE_OK;
+  const int nblk = blocks_for(ctx->natom, 64);
   NNPE_CHECK_HIP(ctx, ev_reserve(ctx, nblk));
-  auto P = [&](int term, int w) { return ev_part(ctx, term, w); };
+  auto P = [&](int term, int w) { return ev ? ev_part(ctx, term, w) : nullptr; };
   if (ev)
-    hipLaunchKernelGGL(k_bonded<true>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->nmol, ctx->inv, ctx->xq, ctx->xmol_code,
-                       ctx->d_box, ctx->rbox, ctx->bstart, ctx->bterm, ctx->bcoef, ctx->astart, ctx->aterm, ctx->acoef,
-                       ctx->dstart, ctx->dterm, ctx->dcoef, ctx->f, P(NNPE_TERM_BOND, 0), P(NNPE_TERM_BOND, 1),
+    hipLaunchKernelGGL(k_bonded<true>, dim3(nblk), dim3(64), 0, ctx->stream, ctx->natom, A, ctx->info, ctx->inv, ctx->xq,
+                       ctx->xmol_code, ctx->d_box, ctx->rbox, ctx->f, P(NNPE_TERM_BOND, 0), P(NNPE_TERM_BOND, 1),
                        P(NNPE_TERM_ANGLE, 0), P(NNPE_TERM_ANGLE, 1), P(NNPE_TERM_DIHEDRAL, 0), P(NNPE_TERM_DIHEDRAL, 1));
   else
-    hipLaunchKernelGGL(k_bonded<false>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->nmol, ctx->inv, ctx->xq, ctx->xmol_code,
-                       ctx->d_box, ctx->rbox, ctx->bstart, ctx->bterm, ctx->bcoef, ctx->astart, ctx->aterm, ctx->acoef,
-                       ctx->dstart, ctx->dterm, ctx->dcoef, ctx->f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_bonded<false>, dim3(nblk), dim3(64), 0, ctx->stream, ctx->natom, A, ctx->info, ctx->inv, ctx->xq,
+                       ctx->xmol_code, ctx->d_box, ctx->rbox, ctx->f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) {
     int rc = ev_finish(ctx, NNPE_TERM_BOND, nblk);

@@ -129,12 +129,19 @@ struct nnpe_ctx {
   int pt_nbin = 0;
   float* pt_tab = nullptr;  // [npair][nbin+1] force then [npair][nbin+1] potential
 
-  // ---- bonded ------------------------------------------------------------------------------
-  int nbtyp = 0, natyp = 0, ndtyp = 0, nbond = 0, nangle = 0, ndih = 0;
-  float *bcoef = nullptr, *acoef = nullptr, *dcoef = nullptr;
-  int *bterm = nullptr, *aterm = nullptr, *dterm = nullptr;   // atom ids as given
-  int *bstart = nullptr, *astart = nullptr, *dstart = nullptr; // per-molecule offsets [nmol+1]
-  int start_len[3] = {0, 0, 0};  // entries of each offset table (nmol + 1 of the molecule table it was built for)
+  // ---- bonded (kind 0 bonds, 1 angles, 2 dihedrals) -----------------------------------------
+  // Terms as given: [nterm][2 + kind] atom ids, kind k has 2 + k atoms, and a type per term.
+  // Coefficients: 4 floats per type for bonds and angles (the classes' stored form); 12 per
+  // dihedral type (derivative polynomial d0..d4 and energy polynomial e0..e5 in cos phi, one pad).
+  // Atom membership: bd_start[k][atom id .. +1] delimits bd_ref[k], the (term << 2 | slot) of
+  // every term the atom belongs to, in term order.
+  int bd_ntyp[3] = {0, 0, 0}, bd_nterm[3] = {0, 0, 0};
+  float* bd_coef[3] = {nullptr, nullptr, nullptr};
+  int* bd_type[3] = {nullptr, nullptr, nullptr};
+  int* bd_atoms[3] = {nullptr, nullptr, nullptr};
+  int* bd_start[3] = {nullptr, nullptr, nullptr};
+  int* bd_ref[3] = {nullptr, nullptr, nullptr};
+  int bd_start_len[3] = {0, 0, 0};  // natom + 1 of the atom table each membership table was built for
 
   // ---- verlet --------------------------------------------------------------------------------
   double dt = 1.0;

@@ -111,9 +111,11 @@ class Engine:
 
     def bonded_set_from_system(self):
         s = self.sys
-        for kind, name, coef in ((0, "bond", s.bond_coef), (1, "angle", s.angle_coef), (2, "dihedral", s.dihedral_coef)):
+        rb = getattr(s, "dihedral_rb", None)
+        dih = (2, "dihedral", s.dihedral_coef) if rb is None else (3, "dihedral", rb)
+        for kind, name, coef in ((0, "bond", s.bond_coef), (1, "angle", s.angle_coef), dih):
             terms = s.expand(name)
-            coef = np.zeros((0, 4), np.float32) if coef is None else _f32(coef)
+            coef = np.zeros((0, 6 if kind == 3 else 4), np.float32) if coef is None else _f32(coef)
             if len(terms):
                 self._check(self.L.nnpe_bonded_set(self.h, kind, len(coef), _p(coef, C.c_float), len(terms),
                                                    _p(_i32(terms), C.c_int)))

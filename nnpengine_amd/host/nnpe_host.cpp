@@ -289,7 +289,7 @@ void LJTableCoulDSF::dumpForce(const char* fname, bool, bool append) {
   std::fclose(fp);
 }
 
-// ---- bonded: autoGenerate (src/bond.cpp:66-103) expands the molecule templates -----------------
+// ---- bonded: autoGenerate (src/bond.cpp:45-82) expands the molecule templates -----------------
 static std::vector<int> expand_terms(int kind) {
   std::vector<int> out;
   const int w = 3 + kind;

@@ -4,7 +4,7 @@ A :class:`System` holds everything ``ENVIRON::initialize`` and ``MOLECULE::from_
 (reference ``src/environ.cpp:22-141``, ``src/molecule.cpp:25-131``): per-atom type, mass,
 charge, position, molecule id and intra-molecule index, plus per-molecule-type topology and the
 special-pair class table (1 = 1-2, 2 = 1-3, 3 = 1-4).  Bonded instances are expanded per
-molecule exactly as ``BOND/ANGLE/DIHEDRAL::autoGenerate`` do (``src/bond.cpp:66-86``): the
+molecule exactly as ``BOND/ANGLE/DIHEDRAL::autoGenerate`` do (``src/bond.cpp:45-82``): the
 template's intra indices plus the atom id of the molecule's first atom.
 
 The builders here make the water-like and butane-like liquids of the benchmark workloads from a
@@ -78,10 +78,13 @@ class System:
     moltypes: List[MolType]
     bond_coef: np.ndarray = None      # [nbtyp, 4]  (b0, 2a, b, 2c)   src/bond.h:11
     angle_coef: np.ndarray = None     # [natyp, 4]  (cos t0, 2a, b, 2c) src/angle.h:11
-    dihedral_coef: np.ndarray = None  # [ndtyp, 4]  cubic-in-cos form  src/dihedral.cpp:46-54
+    dihedral_coef: np.ndarray = None  # [ndtyp, 4]  cubic-in-cos form  src/dihedral.cpp:27-35
     names: tuple = ()
     angle_params: list = None      # raw ANGLE::setParam arguments (theta0_deg, ka, kb, kc)
     dihedral_params: list = None   # raw DIHEDRAL::setParam arguments (c0, c1, c2, c3)
+    # Ryckaert-Bellemans dihedral [ndtyp, 6]: U = sum_n C_n cos^n(phi), phi = 0 cis (the reference
+    # measures phi that way).  When set it replaces dihedral_coef (nnpe_bonded_set kind 3).
+    dihedral_rb: np.ndarray = None
 
     @property
     def natom(self):
@@ -205,7 +208,7 @@ def water_box(nmol: int, bl: float, seed: int, temperature: float = 300.0, r0: f
     sites when nmol is not a cube), uniform random orientations, Maxwell-Boltzmann velocities.
 
     Bond: the reference's quartic form with a = 0, i.e. U = k (r - r0)^2.
-    Angle: the reference's form is harmonic in cos(theta) (src/angle.cpp:113-115),
+    Angle: the reference's form is harmonic in cos(theta) (src/angle.cpp:113-115, :131-134),
     U = c (cos t - cos t0)^2; c = k_angle / sin^2(t0) matches a harmonic-in-theta k_angle
     (kcal/mol/rad^2) at small deviations.
     """
@@ -243,7 +246,7 @@ def water_box(nmol: int, bl: float, seed: int, temperature: float = 300.0, r0: f
 
 
 def dihedral_coef_from_fourier(c0, c1, c2, c3):
-    """``DIHEDRAL::setParam(i,c0,c1,c2,c3)`` (src/dihedral.cpp:46-54): U = c0 + c1 cos(p) +
+    """``DIHEDRAL::setParam(i,c0,c1,c2,c3)`` (src/dihedral.cpp:27-35): U = c0 + c1 cos(p) +
     c2 cos(2p) + c3 cos(3p) rewritten as a cubic in cos(p); stored (k0, k1, 4*k2, 12*k3) so the
     force polynomial is (coef3 * c + coef2) * c + coef1."""
     return np.array([c0 - c2, c1 - 3 * c3, c2 * 2 * 2, c3 * 4 * 3], np.float32)
@@ -290,6 +293,21 @@ def butane_box(nmol: int, bl: float, seed: int, temperature: float = 300.0) -> S
                   dihedral_coef=dihedral_coef_from_fourier(0.0, 0.7055, -0.1355, 1.5725)[None, :],
                   names=("CH3", "CH2"), angle_params=[(114.0, 0.0, 0.0, c_ang)],
                   dihedral_params=[(0.0, 0.7055, -0.1355, 1.5725)])
+
+
+def rb_butane_coef():
+    """Ryckaert-Bellemans butane torsion, C0..C5 = 9.28, 12.16, -13.12, -3.06, 26.24, -31.5 kJ/mol
+    in cos(psi) with psi = phi - 180 deg (trans = 0), converted to kcal/mol and to cos(phi) of
+    the reference's convention (cos psi = -cos phi, so C_n picks up (-1)^n)."""
+    c = np.array([9.28, 12.16, -13.12, -3.06, 26.24, -31.5]) / 4.184
+    return (c * (-1.0) ** np.arange(6)).astype(np.float32)[None, :]
+
+
+def with_rb_dihedral(sysm: System) -> System:
+    """The system with its dihedral replaced by the Ryckaert-Bellemans quintic."""
+    s = sysm.copy()
+    s.dihedral_rb = rb_butane_coef()
+    return s
 
 
 # ---------------------------------------------------------------------------------------------

@@ -353,8 +353,8 @@ struct Eng {
     }
   }
 
-  // ---- BOND / ANGLE / DIHEDRAL (src/bond.cpp:108-137, src/angle.cpp:87-136,
-  //      src/dihedral.cpp:109-166); x_mol / f_mol indirection by atom id -----------------------
+  // ---- BOND / ANGLE / DIHEDRAL (src/bond.cpp:87-116, src/angle.cpp:87-136,
+  //      src/dihedral.cpp:90-147); x_mol / f_mol indirection by atom id -----------------------
   const T* xm(int aid) const { return &x[3 * xmol_row[aid]]; }
   T* fm(int aid) { return &f[3 * fmol_row[aid]]; }
 
@@ -399,10 +399,15 @@ struct Eng {
     }
   }
 
-  void dihedral_compute(int nd, const int* dih, const float* coef, bool ev, double* erg) {
+  // poly = false: the reference's stored cubic (src/dihedral.cpp:27-35);
+  // poly = true: coef [ntyp][6] = C0..C5 of U = sum_n C_n cos^n(phi) (Ryckaert-Bellemans, no
+  // counterpart in the reference), evaluated on the same geometry with dU/dcos = sum n C_n cos^(n-1)
+  void dihedral_compute(int nd, const int* dih, const float* coef, bool ev, double* erg, bool poly = false) {
     for (int a = 0; a < nd; ++a) {
       const int* p = dih + 5 * a;
-      T icf[4]; for (int k = 0; k < 4; ++k) icf[k] = T(coef[4 * p[0] + k]);
+      T icf[4] = {0, 0, 0, 0}, C[6] = {0, 0, 0, 0, 0, 0};
+      if (poly) for (int k = 0; k < 6; ++k) C[k] = T(coef[6 * p[0] + k]);
+      else for (int k = 0; k < 4; ++k) icf[k] = T(coef[4 * p[0] + k]);
       const T *xi = xm(p[1]), *xj = xm(p[2]), *xk = xm(p[3]), *xl = xm(p[4]);
       T *fi = fm(p[1]), *fj = fm(p[2]), *fk = fm(p[3]), *fl = fm(p[4]);
       T dx1 = xi[0] - xj[0], dy1 = xi[1] - xj[1], dz1 = xi[2] - xj[2];
@@ -417,7 +422,8 @@ struct Eng {
       T r3 = T(1) / std::sqrt(sqr(dx3) + sqr(dy3) + sqr(dz3));
       dx1 *= r1; dy1 *= r1; dz1 *= r1; dx3 *= r3; dy3 *= r3; dz3 *= r3;
       T cq = dx1 * dx3 + dy1 * dy3 + dz1 * dz3;
-      T ff = (icf[3] * cq + icf[2]) * cq + icf[1];
+      T ff = poly ? (((T(5) * C[5] * cq + T(4) * C[4]) * cq + T(3) * C[3]) * cq + T(2) * C[2]) * cq + C[1]
+                  : (icf[3] * cq + icf[2]) * cq + icf[1];
       r1 *= ff; r3 *= ff;
       T f1x = r1 * (cq * dx1 - dx3), f1y = r1 * (cq * dy1 - dy3), f1z = r1 * (cq * dz1 - dz3);
       T f3x = r3 * (cq * dx3 - dx1), f3y = r3 * (cq * dy3 - dy1), f3z = r3 * (cq * dz3 - dz1);
@@ -426,7 +432,8 @@ struct Eng {
       fl[0] += f3x; fl[1] += f3y; fl[2] += f3z;
       fj[0] += f2x - f1x; fj[1] += f2y - f1y; fj[2] += f2z - f1z;
       fk[0] -= f2x + f3x; fk[1] -= f2y + f3y; fk[2] -= f2z + f3z;
-      if (ev) *erg += double(((icf[3] / T(3) * cq + icf[2] / T(2)) * cq + icf[1]) * cq + icf[0]);
+      if (ev && poly) *erg += double(((((C[5] * cq + C[4]) * cq + C[3]) * cq + C[2]) * cq + C[1]) * cq + C[0]);
+      else if (ev) *erg += double(((icf[3] / T(3) * cq + icf[2] / T(2)) * cq + icf[1]) * cq + icf[0]);
     }
   }
 
@@ -1173,6 +1180,7 @@ int orc_pair_compute(OrcHandle* h, int ev, double* erg, double* vir) { ORC_DISPA
 int orc_bond_compute(OrcHandle* h, int n, const int* t, const float* c, int ev, double* erg, double* vir) { ORC_DISPATCH(h, bond_compute(n, t, c, ev != 0, erg, vir)); return 0; }
 int orc_angle_compute(OrcHandle* h, int n, const int* t, const float* c, int ev, double* erg) { ORC_DISPATCH(h, angle_compute(n, t, c, ev != 0, erg)); return 0; }
 int orc_dihedral_compute(OrcHandle* h, int n, const int* t, const float* c, int ev, double* erg) { ORC_DISPATCH(h, dihedral_compute(n, t, c, ev != 0, erg)); return 0; }
+int orc_dihedral_poly_compute(OrcHandle* h, int n, const int* t, const float* c, int ev, double* erg) { ORC_DISPATCH(h, dihedral_compute(n, t, c, ev != 0, erg, true)); return 0; }
 int orc_set_dt(OrcHandle* h, double dt) {
   if (h->prec) static_cast<Eng<double>*>(h->e)->set_dt(dt); else static_cast<Eng<float>*>(h->e)->set_dt(float(dt));
   return 0;

@@ -44,7 +44,7 @@ def lib():
                    "orc_isvalid", "orc_list_size", "orc_list_export", "orc_get", "orc_set",
                    "orc_get_cell_ids", "orc_get_xmol_codes", "orc_clear_f", "orc_pair_set",
                    "orc_pair_compute", "orc_bond_compute", "orc_angle_compute",
-                   "orc_dihedral_compute", "orc_set_dt", "orc_update_x", "orc_update_v",
+                   "orc_dihedral_compute", "orc_dihedral_poly_compute", "orc_set_dt", "orc_update_x", "orc_update_v",
                    "orc_compute_ke", "orc_gr_init", "orc_gr_tally", "orc_gr_get", "orc_sq",
                    "orc_nnp_tbl_init", "orc_nnp_tbl_tables", "orc_nnp_tbl_descriptor",
                    "orc_nnp_tbl_set_dedg", "orc_nnp_tbl_force", "orc_mlp_set", "orc_mlp_eval",
@@ -160,7 +160,11 @@ class Oracle:
         if len(angles):
             self.L.orc_angle_compute(self.h, len(angles), _p(angles, C.c_int), _p(_f32(s.angle_coef), C.c_float),
                                      int(ev), C.byref(e))
-        if len(dih):
+        rb = getattr(s, "dihedral_rb", None)
+        if len(dih) and rb is not None:
+            self.L.orc_dihedral_poly_compute(self.h, len(dih), _p(dih, C.c_int), _p(_f32(rb), C.c_float),
+                                             int(ev), C.byref(e))
+        elif len(dih):
             self.L.orc_dihedral_compute(self.h, len(dih), _p(dih, C.c_int), _p(_f32(s.dihedral_coef), C.c_float),
                                         int(ev), C.byref(e))
         return e.value, w.value

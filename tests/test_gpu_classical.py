@@ -2,12 +2,18 @@
 S(q)) against the oracle on identical seeded inputs.  Bars: bit-exact list sets and g(r)
 counts; forces/energies within 1e-5 relative RMS of the FP64 oracle (and ~1e-6 of the FP32
 oracle, which differs only in summation order)."""
+import os
+
 import numpy as np
 import pytest
 
 from nnpengine_amd import system as S
 from oracle import pyoracle
 from tests.conftest import rel_rms
+from tests.golden.make_golden import make_system
+from tests.golden.make_golden import tables as golden_tables
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -181,12 +187,47 @@ def test_call_order_errors(water_c0):
     assert np.array_equal(pyoracle.canonical(e.list_entries()), pyoracle.canonical(o.list_entries()))
 
 
-@pytest.mark.parametrize("case", ["c0", "ragged"])
+def _forces_case(case, water_c0):
+    """(system, pair cutoff, tables, reference fixture or None) of a force-parity case."""
+    if case == "c0":
+        sysm = water_c0
+    elif case == "ragged":
+        sysm = S.water_box(300, 21.0, seed=3)
+    else:
+        sysm = make_system("butane")
+        if case == "butane_rb":
+            sysm = S.with_rb_dihedral(sysm)
+    if case.startswith("butane"):
+        ft, ut = golden_tables("butane", sysm)
+        fix = np.load(os.path.join(GOLD, "butane_snap.npz")) if case == "butane" else None
+    else:
+        ft, ut = S.lj_table(sysm, RC, 900, S.WATER_LJ)
+        fix = np.load(os.path.join(GOLD, "water_c0_snap.npz")) if case == "c0" else None
+    return sysm, ft, ut, fix
+
+
+def _oracle_forces(sysm, ft, ut, prec, pair=True):
+    o = _oracle(sysm, prec)
+    o.rebuild(False)
+    o.pair_set(RC, ft, ut)
+    o.clear_f()
+    eb = o.bonded_compute()
+    ep = o.pair_compute() if pair else (0.0, 0.0)
+    return o.get("f"), ep, eb
+
+
+ENERGY_TERMS = ("pair", "bond", "angle", "dihedral")
+
+
+@pytest.mark.parametrize("case", ["c0", "ragged", "butane", "butane_rb"])
 def test_pair_bonded_forces(case, water_c0):
-    """Tabulated pair + bonded forces, energies and virials; "ragged" is 900 atoms, so the last
-    workgroup of the pair pass is partial."""
-    sysm = water_c0 if case == "c0" else S.water_box(300, 21.0, seed=3)
-    ft, ut = S.lj_table(sysm, RC, 900, S.WATER_LJ)
+    """Tabulated pair + bonded forces, energies and virials against the FP64 oracle (1e-5) and,
+    where the reference built from source wrote one, its own FP32 result.  "ragged" is 900
+    atoms, so the last workgroup of the pair pass is partial; "butane" is 64 united-atom chains
+    with bonds, angles and the reference's cubic dihedral; "butane_rb" the same chains with the
+    Ryckaert-Bellemans quintic dihedral."""
+    sysm, ft, ut, fix = _forces_case(case, water_c0)
+    print(f"[pair] case={case} natom={sysm.natom}", flush=True)
     e = _engine(sysm)
     e.build()
     e.pair_table(RC, ft, ut)
@@ -196,31 +237,54 @@ def test_pair_bonded_forces(case, water_c0):
     e.bonded_compute(True)
     fg = e.get("f")
     en = e.energy()
-    res = {}
-    for prec in (0, 1):
-        o = _oracle(sysm, prec)
-        o.rebuild(False)
-        o.pair_set(RC, ft, ut)
-        o.clear_f()
-        eb = o.bonded_compute()
-        ep = o.pair_compute()
-        res[prec] = (o.get("f"), ep, eb)
-    f32, ep32, eb32 = res[0]
-    f64, ep64, eb64 = res[1]
+    f32, ep32, eb32 = _oracle_forces(sysm, ft, ut, 0)
+    f64, ep64, eb64 = _oracle_forces(sysm, ft, ut, 1)
     print(f"[pair] {case} force rel rms vs fp32 oracle", rel_rms(fg, f32), "vs fp64", rel_rms(fg, f64))
     assert rel_rms(fg, f32) < 2e-6
     assert rel_rms(fg, f64) < 1e-5
-    e_gpu = en["pair"][0] + en["bond"][0] + en["angle"][0]
-    w_gpu = en["pair"][1] + en["bond"][1]
-    # The totals cancel (Coulomb pairs of both signs) and each pair term is FP32, as in the
-    # reference (src/pair.cpp:289-419), so their rounding shows in the sum: the bar is 1e-5 of
-    # the total plus twice the FP32 oracle's own distance from FP64 (2.4e-5 of the total in the
-    # 900-atom box).
+    if fix is not None:
+        print(f"[pair] {case} vs the reference's own FP32 forces", rel_rms(fg, fix["f"]))
+        assert rel_rms(fg, fix["f"]) < 5e-6
+    e_gpu = sum(en[t][0] for t in ENERGY_TERMS)
+    w_gpu = sum(en[t][1] for t in ENERGY_TERMS)
     for k in (0, 1):
-        r64, r32 = ep64[k] + eb64[k], ep32[k] + eb32[k]
+        r64 = ep64[k] + eb64[k]
         got = e_gpu if k == 0 else w_gpu
-        print(f"[pair] {case} {'energy' if k == 0 else 'virial'} {got:.6f} fp64 {r64:.6f} fp32 oracle {r32:.6f}")
-        assert abs(got - r64) < 1e-5 * abs(r64) + 2.0 * abs(r32 - r64)
+        print(f"[pair] {case} {'energy' if k == 0 else 'virial'} {got:.8f} fp64 {r64:.8f} rel {abs(got - r64) / abs(r64):.2e}")
+        assert abs(got - r64) < 1e-5 * abs(r64)
+
+
+@pytest.mark.parametrize("case", ["c0", "butane", "butane_rb"])
+def test_bonded_forces_alone(case, water_c0):
+    """The bonded pass alone (bond, angle, dihedral) against the FP64 oracle: the pair forces
+    would otherwise hide its error.  Every site is displaced by up to 0.05 A (seeded), so the
+    molecules leave their equilibrium geometry (the water box starts at it, with zero bonded
+    force)."""
+    sysm, ft, ut, _ = _forces_case(case, water_c0)
+    sysm = sysm.copy()
+    rng = np.random.default_rng(17)
+    sysm.x = (sysm.x + rng.uniform(-0.05, 0.05, sysm.x.shape)).astype(np.float32)
+    e = _engine(sysm)
+    e.build()
+    e.bonded_set_from_system()
+    e.clear_force()
+    e.energy_clear()
+    e.bonded_compute(True)
+    fg = e.get("f")
+    en = e.energy()
+    f64, _, eb64 = _oracle_forces(sysm, ft, ut, 1, pair=False)
+    err = rel_rms(fg, f64)
+    eg = en["bond"][0] + en["angle"][0] + en["dihedral"][0]
+    print(f"[bonded] {case} force rel rms vs fp64 {err:.2e}; energy {eg:.8f} vs {eb64[0]:.8f}; "
+          f"virial {en['bond'][1]:.6f} vs {eb64[1]:.6f}", flush=True)
+    assert err < 1e-5
+    assert abs(eg - eb64[0]) < 1e-5 * abs(eb64[0])
+    assert abs(en["bond"][1] - eb64[1]) < 1e-5 * max(abs(eb64[1]), 1e-3 * abs(eb64[0]))
+    if case.startswith("butane"):
+        assert abs(en["dihedral"][0]) > 0
+    # every atom's bonded force has one writer: a second pass adds exactly the same forces
+    e.bonded_compute(False)
+    assert np.array_equal(e.get("f"), 2 * fg)
 
 
 def test_md_steps_match_fp32_oracle(water_c0):
