@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <cstdlib>
+
 #include "internal.h"
 
 int ctx_fail(nnpe_ctx* ctx, int code, const std::string& msg) {
@@ -37,6 +39,7 @@ extern "C" int nnpe_create(nnpe_ctx** out, int device) {
   NNPE_CHECK_HIP(ctx, hipSetDevice(device));
   NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  if (const char* e = std::getenv("NNPE_MD_FORK")) ctx->md_fork = std::atoi(e) != 0;
   NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
   NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
   NNPE_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->counters), sizeof(DevCounters)));

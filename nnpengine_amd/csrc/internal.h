@@ -177,6 +177,7 @@ struct nnpe_ctx {
   // ---- MD driver -----------------------------------------------------------------------------
   long long md_step = 0;                // steps completed by nnpe_md_run / the step calls
   void* md = nullptr;                   // captured step graphs (md.hip)
+  bool md_fork = true;                  // NNP steps: classical terms on stream2 beside the NNP passes
   double* md_log = nullptr;             // device log records
   int md_log_cap = 0;
 };

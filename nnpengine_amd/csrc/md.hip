@@ -21,9 +21,19 @@ enum StepBits { kRebuild = 1, kGr = 2, kGrFull = 4, kLog = 8 };
 constexpr int kCopyProbeBytes = 64 << 20;  // each direction
 
 struct MdGraphs {
-  std::map<int, hipGraphExec_t> exec;
-  std::map<int, hipGraph_t> graph;
+  // one graph per step shape, and one per sequence of step shapes of a whole rebuild interval
+  std::map<std::vector<int>, hipGraphExec_t> exec;
+  std::map<std::vector<int>, hipGraph_t> graph;
 };
+
+// step shape of step s of a run with parameters P (first: the step that builds the first list)
+int step_key(const nnpe_md_params& P, long long s, bool first) {
+  int key = 0;
+  if (P.gr_every > 0 && s % P.gr_every == 0) key |= kGr | ((P.gr_long_every > 0 && s % P.gr_long_every == 0) ? kGrFull : 0);
+  if (P.log_every > 0 && s % P.log_every == 0) key |= kLog;
+  if (P.rebuild_every > 0 && (first || s % P.rebuild_every == 0)) key |= kRebuild;
+  return key;
+}
 
 __global__ void k_skin_check(int n, float dx2, const float4* xq, const float4* xprev, DevCounters* cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -100,7 +110,7 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   // NNP force, which adds into f, waits for them.  Captured, the step graph gets two branches.
   // The descriptors run first, alone: the classical kernels (VALU-bound) then overlap the network
   // (MFMA, latency-bound) and the adjoints instead of the equally VALU-bound descriptor pass.
-  const bool fork = P.nnp != 0;
+  const bool fork = P.nnp != 0 && ctx->md_fork;
   int rn = NNPE_OK;
   if (P.nnp == 1) rn = nnp_tbl_desc_enqueue(ctx);
   else if (P.nnp == 2) rn = sf_desc_enqueue(ctx);
@@ -190,12 +200,30 @@ extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, d
   NNPE_CHECK_LAUNCH(ctx);
   MdGraphs* G = graphs_of(ctx);
   int rc;
-  for (int k = 0; k < nsteps; ++k) {
+  // captures the steps of `keys` (consecutive, starting at the current step) into one graph, or
+  // replays the graph of that sequence
+  auto run_graph = [&](const std::vector<int>& keys) -> int {
+    auto it = G->exec.find(keys);
+    if (it == G->exec.end()) {
+      hipGraph_t g = nullptr;
+      NNPE_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int rcs = NNPE_OK;
+      for (size_t t = 0; t < keys.size() && !rcs; ++t) rcs = enqueue_step(ctx, P, keys[t], false);
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+      if (rcs) return rcs;
+      NNPE_CHECK_HIP(ctx, ce);
+      hipGraphExec_t ex = nullptr;
+      NNPE_CHECK_HIP(ctx, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      G->graph[keys] = g;
+      it = G->exec.emplace(keys, ex).first;
+    }
+    NNPE_CHECK_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
+    return NNPE_OK;
+  };
+  for (int k = 0; k < nsteps;) {
     const long long s = ctx->md_step;
     const bool first = s == 0 || !ctx->have_list;
-    int key = 0;
-    if (P.gr_every > 0 && s % P.gr_every == 0) key |= kGr | ((P.gr_long_every > 0 && s % P.gr_long_every == 0) ? kGrFull : 0);
-    if (P.log_every > 0 && s % P.log_every == 0) key |= kLog;
+    const int key = step_key(P, s, first);
     if (P.rebuild_every <= 0) {
       // NEIGHLIST::isvalid every step (src/run.cpp:236-246): host decision, eager launches
       if (!first && (rc = verlet_enqueue_x(ctx))) return rc;
@@ -205,35 +233,27 @@ extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, d
         if ((rc = nnpe_neigh_build(ctx))) return rc;
       }
       // the rest of the step without the position update
-      nnpe_md_params Q = P;
-      if ((rc = enqueue_step(ctx, Q, key, true))) return rc;
+      if ((rc = enqueue_step(ctx, P, key & ~kRebuild, true))) return rc;
+    } else if (first) {
+      // list first (with the capacity check and growth), then the step on the positions as read
+      if ((rc = nnpe_neigh_build(ctx))) return rc;
+      G = graphs_of(ctx);
+      if ((rc = enqueue_step(ctx, P, key & ~kRebuild, true))) return rc;
+    } else if (P.use_graph && s % P.rebuild_every == 0 && nsteps - k >= P.rebuild_every) {
+      // a whole rebuild interval as one graph: no launch gaps between its steps
+      std::vector<int> keys(P.rebuild_every);
+      for (int t = 0; t < P.rebuild_every; ++t) keys[t] = step_key(P, s + t, false);
+      if ((rc = run_graph(keys))) return rc;
+      ctx->md_step += P.rebuild_every;
+      k += P.rebuild_every;
+      continue;
+    } else if (P.use_graph) {
+      if ((rc = run_graph(std::vector<int>{key}))) return rc;
     } else {
-      if (first || s % P.rebuild_every == 0) key |= kRebuild;
-      if (first) {
-        // list first (with the capacity check and growth), then the step on the positions as read
-        if ((rc = nnpe_neigh_build(ctx))) return rc;
-        G = graphs_of(ctx);
-        if ((rc = enqueue_step(ctx, P, key & ~kRebuild, true))) return rc;
-      } else if (P.use_graph) {
-        auto it = G->exec.find(key);
-        if (it == G->exec.end()) {
-          hipGraph_t g = nullptr;
-          NNPE_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-          int rcs = enqueue_step(ctx, P, key, false);
-          hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
-          if (rcs) return rcs;
-          NNPE_CHECK_HIP(ctx, ce);
-          hipGraphExec_t ex = nullptr;
-          NNPE_CHECK_HIP(ctx, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-          G->graph[key] = g;
-          it = G->exec.emplace(key, ex).first;
-        }
-        NNPE_CHECK_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
-      } else {
-        if ((rc = enqueue_step(ctx, P, key, false))) return rc;
-      }
+      if ((rc = enqueue_step(ctx, P, key, false))) return rc;
     }
     ctx->md_step++;
+    ++k;
   }
   if ((rc = counters_read(ctx))) return rc;
   const DevCounters c = *ctx->h_counters;

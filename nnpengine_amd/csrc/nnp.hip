@@ -167,6 +167,13 @@ struct SfParams {
   // equally spaced centres mu_k = mu_s + (k - s) delta (s = kRadS):
   float rad_step_l2;      // 2 eta_r delta / ln 2
   float rad_rho[kRadS];   // exp(-eta_r delta^2 (2j + 1)), j = 0 .. kRadS-1
+  float rad_q[kNRad];     // Q_k = exp(-eta_r delta^2 (k - s)^2): g_k = g_s R^(k-s) Q_k
+  float rad_q_l2;         // -eta_r delta^2 / ln 2
+  float rad_delta;        // delta
+  // +1 and -1 as run-time values: a half-precision operand times one of them plus an FP32 value
+  // is a single mixed-precision FMA (the compiler would turn a literal +-1 into an add after a
+  // separate conversion)
+  float one, m_one;
 };
 
 // g[k] = exp(-eta_r (r - mu_k)^2), k = 0 .. kNRad-1.  RR (equally spaced centres): g at the
@@ -474,6 +481,11 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, int stride, const flo
     }
     if (a == b) dA[10] -= u0 + u1 + u2;
   }
+  // The slice stores the off-diagonal second-moment adjoints (components 7..9, each the adjoint
+  // of a product that appears twice in the symmetric matrix) halved, so the force pass forms
+  // M h with plain FMAs.
+#pragma unroll
+  for (int c = 7; c <= 9; ++c) dA[c] *= 0.5f;
   // slice float f of element a of row i at adj[((a * 16 + f / 4) * stride + i) * 4 + f % 4]
   auto at = [&](int f) -> float& { return adj[((size_t)(a * (kSlice / 4) + (f >> 2)) * stride + i) * 4 + (f & 3)]; };
 #pragma unroll
@@ -511,7 +523,8 @@ __global__ void __launch_bounds__(256) k_sf_adjoint(int n, int stride, const flo
 }
 
 // dE_owner/du for one neighbour at unit vector h (u = r h) of the given element slice of an
-// adjoint row: the angular part (vector) and the radial part's scalar folded into `along`
+// adjoint row (off-diagonal components 7..9 stored halved): the angular part (vector) and the
+// radial part's scalar folded into `along`
 __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const float* wv, const float* dwv, float ir,
                                             float hx, float hy, float hz, float& along, float& gx, float& gy, float& gz) {
 #pragma clang fp contract(fast)
@@ -520,9 +533,9 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
     const float* e = d + m * kNMom;
     const float w = wv[m], dw = dwv[m], wr = w * ir;
     const float p1 = e[1] * hx + e[2] * hy + e[3] * hz;
-    const float mrx = e[4] * hx + 0.5f * (e[7] * hy + e[8] * hz);
-    const float mry = e[5] * hy + 0.5f * (e[7] * hx + e[9] * hz);
-    const float mrz = e[6] * hz + 0.5f * (e[8] * hx + e[9] * hy);
+    const float mrx = e[4] * hx + e[7] * hy + e[8] * hz;
+    const float mry = e[5] * hy + e[7] * hx + e[9] * hz;
+    const float mrz = e[6] * hz + e[8] * hx + e[9] * hy;
     const float qq = mrx * hx + mry * hy + mrz * hz;
     along += dw * (e[0] + 2.0f * w * e[10] + p1 + qq) - wr * (p1 + 2.0f * qq);
     gx += wr * (e[1] + 2.0f * mrx);
@@ -534,12 +547,26 @@ __device__ __forceinline__ void sf_grad_ang(const float* __restrict__ d, const f
 // A group of G lanes per atom over the step's active lists (the pairs within the cutoff, one
 // list per neighbour element, compacted by the descriptor pass); the atom's own adjoint row is
 // staged in LDS (one row per group), each lane gathers the neighbour's element slice of its
-// pairs from global memory.
+// pairs from global memory.  Own and neighbour terms come from one evaluation: the force of the
+// pair on atom i is dE_i/du - dE_j/du' (u = x_j - x_i, u' = -u), and since the angular term is
+// linear in the adjoint and the neighbour's is taken at -h, where the moment components of odd
+// degree in h (1..3) change sign, one evaluation on own + neighbour for the even components and
+// own - neighbour for the odd ones gives it.
+// RR (equally spaced radial centres, within the range the host checked): the radial force is
+// a pair of polynomials in R = exp(2 eta_r delta (r - mu_s)) and 1/R.  With
+// g_{s+-j} = g_s R^(+-j) Q_j, the sum over centres of c_k d/dr[g_k fc] with c_k the two atoms'
+// radial adjoints is
+//   S0 (dfc - 2 eta_r fc (r - mu_s)) + 2 eta_r delta fc T,
+//   S0 = sum_k c_k g_k = g_s (U(R) + X L(X)),  T = sum_k (k - s) c_k g_k = g_s (R U'(R) - X (L + X L')),
+// X = 1/R, U and L polynomials with coefficients c_k Q_k, evaluated with their derivatives by
+// Horner's rule (two FMAs per centre).  The own adjoints are staged pre-multiplied by Q_k.
 // VV: this pass finishes each atom's force (the classical terms were added before), so it also
 // applies the FP64 velocity half-kick (vv_kick64) and stores the acceleration, in place of a
 // separate Verlet pass.
 // HALF (16-bit network forms): the neighbour's slice is gathered from its FP16 copy, 8 values
-// per 16-byte load (8 loads per pair instead of 15); the atom's own slice stays FP32.
+// per 16-byte load (8 loads per pair instead of 15); the atom's own slice stays FP32, and each
+// half value enters through a mixed-precision FMA with an FP32 operand.
+// EV: the virial of the pair, -(F_i . u), halved (each pair appears in both atoms' lists).
 template <int NT, bool EV, int G, bool RR, bool VV, bool HALF>
 // three waves per SIMD (at most 168 VGPRs): measured faster than the compiler's own 178-198
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_sf_force(int n, int stride, int cap, const uint32_t* __restrict__ nbr,
@@ -550,6 +577,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                                                   float4* __restrict__ f, double* pe, double* pw, VvArgs V) {
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kSlice;
+  constexpr int QR0 = kSliceRad / 4, QR1 = (kSliceRad + kNRad) / 4;  // radial chunks of a slice
   __shared__ __align__(16) float s_adj[256 / G][NMT];
   __shared__ float4 s_img[27];
   image_table_fill(s_img);
@@ -557,7 +585,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
   const float4* __restrict__ adj4 = reinterpret_cast<const float4*>(adj);
   if (i < n)
-    for (int q = lane; q < NMT / 4; q += G) reinterpret_cast<float4*>(s_adj[grp])[q] = adj4[(size_t)q * stride + i];
+    for (int q = lane; q < NMT / 4; q += G) {
+      float4 v = adj4[(size_t)q * stride + i];
+      const int qs = q % (kSlice / 4);
+      if (RR && qs >= QR0 && qs < QR1) {
+        // own radial adjoints times Q_k, Q_k = 2^(rad_q_l2 (k - s)^2)
+        const float d0 = float(4 * (qs - QR0) - kRadS);
+        v.x *= __builtin_amdgcn_exp2f(P.rad_q_l2 * d0 * d0);
+        v.y *= __builtin_amdgcn_exp2f(P.rad_q_l2 * (d0 + 1.0f) * (d0 + 1.0f));
+        v.z *= __builtin_amdgcn_exp2f(P.rad_q_l2 * (d0 + 2.0f) * (d0 + 2.0f));
+        v.w *= __builtin_amdgcn_exp2f(P.rad_q_l2 * (d0 + 3.0f) * (d0 + 3.0f));
+      }
+      reinterpret_cast<float4*>(s_adj[grp])[q] = v;
+    }
   __syncthreads();
   double wacc = 0;
   if (i < n) {
@@ -603,73 +643,74 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       }
       // the neighbour's slice for this atom's element, 16-byte chunk q at adj4[(ti*16 + q) * stride
       // + j]: the lanes' neighbours are rows of a few cells, so a wave's chunk loads share lines
-      float aj[kSliceRad + kNRad];
+      const float* as = ai + tj * kSlice;                      // own slice for the neighbour's element
+      // c[k] = own + neighbour (odd angular components own - neighbour); radial (RR) times Q_k
+      float e[kSliceRad + kNRad];
       if constexpr (HALF) {
+        _Float16 hv[kSlice];
 #pragma unroll
         for (int q = 0; q < kSlice / 8; ++q) {
           const uint4 v = adjh[(size_t)(ti * (kSlice / 8) + q) * stride + j];
-          _Float16 hv[8];
-          __builtin_memcpy(hv, &v, 16);
+          __builtin_memcpy(hv + 8 * q, &v, 16);
+        }
 #pragma unroll
-          for (int t = 0; t < 8; ++t)
-            if (8 * q + t < kSliceRad + kNRad) aj[8 * q + t] = (float)hv[t];
+        for (int k = 0; k < kSliceRad + kNRad; ++k) {
+          const int c = k % kNMom;
+          const float mul = k >= kSliceRad ? (RR ? P.rad_q[k - kSliceRad] : P.one)
+                                           : ((c >= 1 && c <= 3) ? P.m_one : P.one);
+          e[k] = fmaf((float)hv[k], mul, as[k]);
         }
       } else {
 #pragma unroll
         for (int q = 0; q < (kSliceRad + kNRad) / 4; ++q) {
           const float4 v = adj4[(size_t)(ti * (kSlice / 4) + q) * stride + j];
-          aj[4 * q] = v.x; aj[4 * q + 1] = v.y; aj[4 * q + 2] = v.z; aj[4 * q + 3] = v.w;
+          const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int k = 4 * q + t, c = k % kNMom;
+            if (k >= kSliceRad) e[k] = RR ? fmaf(vv[t], P.rad_q[k - kSliceRad], as[k]) : as[k] + vv[t];
+            else e[k] = (c >= 1 && c <= 3) ? as[k] - vv[t] : as[k] + vv[t];
+          }
         }
       }
-      const float* as = ai + tj * kSlice;                      // own slice for the neighbour's element
-      // radial: sum_n g_n'(r) (dR_i[tj][n] + dR_j[ti][n])
-      float rad = 0.0f, radi = 0.0f;
-      const float* ri = as + kSliceRad;
-      const float* rj = aj + kSliceRad;
-      float gk[kNRad];
-      sf_gauss<RR>(r, P, gk);
+      const float* ce = e + kSliceRad;  // radial coefficients
       const float m2efc = -2.0f * P.eta_r * fc;
+      float rad;
+      if constexpr (RR) {
+        const float d = r - P.mu[kRadS];
+        const float gs = __builtin_amdgcn_exp2f(P.eta_r_l2 * d * d);
+        const float l = P.rad_step_l2 * d;
+        const float R = __builtin_amdgcn_exp2f(l), X = __builtin_amdgcn_exp2f(-l);
+        // U(R) = sum_{j=0}^{7} c_{s+j} R^j and U'(R)
+        float up = ce[kNRad - 1], ud = 0.0f;
 #pragma unroll
-      for (int k = 0; k < kNRad; ++k) {
-        const float d = r - P.mu[k];
-        const float g = gk[k] * (m2efc * d + dfc);  // d/dr of g_k fc
-        rad += (ri[k] + rj[k]) * g;
-        if (EV) radi += ri[k] * g;
-      }
-      if constexpr (EV) {
-        // own adjoint, direction +h
-        float al_i = 0, gix = 0, giy = 0, giz = 0;
-        sf_grad_ang(as, wv, dwv, ir, hx, hy, hz, al_i, gix, giy, giz);
-        // neighbour's adjoint, direction -h
-        float al_j = 0, gjx = 0, gjy = 0, gjz = 0;
-        sf_grad_ang(aj, wv, dwv, ir, -hx, -hy, -hz, al_j, gjx, gjy, gjz);
-        // F_i += dE_i/du - dE_j/du' with u' = -u; radial pieces are along h for i and -h for j
-        const float gxi = gix + (al_i)*hx, gyi = giy + (al_i)*hy, gzi = giz + (al_i)*hz;
-        const float gxj = gjx - al_j * hx, gyj = gjy - al_j * hy, gzj = gjz - al_j * hz;
-        fx += gxi - gxj + rad * hx;
-        fy += gyi - gyj + rad * hy;
-        fz += gzi - gzj + rad * hz;
-        // W_i = -(dE_i/du . u), own descriptor only
-        vir -= (gxi * ux + gyi * uy + gzi * uz) + radi * r;
+        for (int k = kNRad - 2; k >= kRadS; --k) {
+          ud = fmaf(ud, R, up);
+          up = fmaf(up, R, ce[k]);
+        }
+        // L(X) = sum_{i=0}^{7} c_{s-1-i} X^i and L'(X)
+        float lp = ce[0], ld = 0.0f;
+#pragma unroll
+        for (int k = 1; k < kRadS; ++k) {
+          ld = fmaf(ld, X, lp);
+          lp = fmaf(lp, X, ce[k]);
+        }
+        const float S0 = gs * fmaf(X, lp, up);
+        const float T = gs * (R * ud - X * fmaf(X, ld, lp));
+        rad = S0 * fmaf(m2efc, d, dfc) - m2efc * P.rad_delta * T;
       } else {
-        // the angular term is linear in the adjoint and the neighbour's is taken at -h, where
-        // the moment components of odd degree in h (1..3) change sign: one evaluation on
-        // own + neighbour for the even components and own - neighbour for the odd ones gives
-        // dE_i/du - dE_j/du' of the two separate evaluations
-        float e[kNEta * kNMom];
+        // sum_n g_n'(r) (dR_i[tj][n] + dR_j[ti][n])
+        float gk[kNRad];
+        sf_gauss<false>(r, P, gk);
+        rad = 0.0f;
 #pragma unroll
-        for (int m = 0; m < kNEta; ++m)
-#pragma unroll
-          for (int c = 0; c < kNMom; ++c) {
-            const int k = m * kNMom + c;
-            e[k] = (c >= 1 && c <= 3) ? as[k] - aj[k] : as[k] + aj[k];
-          }
-        float al = 0, gx = 0, gy = 0, gz = 0;
-        sf_grad_ang(e, wv, dwv, ir, hx, hy, hz, al, gx, gy, gz);
-        fx += gx + (al + rad) * hx;
-        fy += gy + (al + rad) * hy;
-        fz += gz + (al + rad) * hz;
+        for (int k = 0; k < kNRad; ++k) rad += ce[k] * (gk[k] * (m2efc * (r - P.mu[k]) + dfc));
       }
+      float al = 0, gx = 0, gy = 0, gz = 0;
+      sf_grad_ang(e, wv, dwv, ir, hx, hy, hz, al, gx, gy, gz);
+      const float px = gx + (al + rad) * hx, py = gy + (al + rad) * hy, pz = gz + (al + rad) * hz;
+      fx += px; fy += py; fz += pz;
+      if (EV) vir -= 0.5f * (px * ux + py * uy + pz * uz);
     }
     }
     }
@@ -699,6 +740,7 @@ struct NnpState {
   // symmetry functions
   bool sf = false;
   bool rad_recur = false;  // equally spaced radial centres: Gaussians by recurrence
+  bool rad_horner = false; // ... and in the range where the force pass sums them as polynomials
   SfParams P{};
   float* M = nullptr;    // moments [N][NMT]
   float* adj = nullptr;  // [N][ntype][kSlice]
@@ -873,6 +915,18 @@ extern "C" int nnpe_sf_set(nnpe_ctx* ctx, float rc, int nrad, const float* mu, f
   S.rad_recur = delta > 0 && dev <= 1e-6 * std::max(1.0, std::fabs(double(mu[kNRad - 1])));
   S.P.rad_step_l2 = float(2.0 * double(eta_r) * delta * l2e);
   for (int j = 0; j < kRadS; ++j) S.P.rad_rho[j] = float(std::exp(-double(eta_r) * delta * delta * (2 * j + 1)));
+  // the force pass's radial polynomials in R and 1/R (coefficients c_k Q_k): Q_k must stay a
+  // normal float, and the largest power term times its Q, exp(a z^2) with a = eta_r delta^2 and
+  // z the farthest distance (in centre spacings) of any r in [0, rc] from the middle centre,
+  // must stay far from the float range
+  const double a_q = double(eta_r) * delta * delta;
+  const double zmax = std::max(std::fabs(double(mu[kRadS])), std::fabs(double(rc) - double(mu[kRadS]))) / std::max(delta, 1e-30);
+  S.rad_horner = S.rad_recur && a_q <= 1.0 && a_q * zmax * zmax <= 75.0;
+  for (int k = 0; k < kNRad; ++k) S.P.rad_q[k] = float(std::exp(-a_q * double(k - kRadS) * double(k - kRadS)));
+  S.P.rad_q_l2 = float(-a_q * l2e);
+  S.P.rad_delta = float(delta);
+  S.P.one = 1.0f;
+  S.P.m_one = -1.0f;
   S.nmt = ctx->ntype * kNEta * kNMom + ctx->ntype * kNRad;
   NNPE_CHECK_HIP(ctx, dev_alloc(S.M, size_t(ctx->natom) * S.nmt));
   NNPE_CHECK_HIP(ctx, dev_alloc(S.adj, size_t(ctx->stride) * ctx->ntype * kSlice));
@@ -957,7 +1011,7 @@ int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick) {
                      S.act_cap, S.act, S.act_cnt, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, S.P, S.adj, S.adjh, ctx->f, pe, pw, V)
 #define NNPE_SF_FORCE2(NT, EV, RR, VV) if (half) NNPE_SF_FORCE3(NT, EV, RR, VV, true); else NNPE_SF_FORCE3(NT, EV, RR, VV, false)
 #define NNPE_SF_FORCE1(NT, EV, RR) if (kick) NNPE_SF_FORCE2(NT, EV, RR, true); else NNPE_SF_FORCE2(NT, EV, RR, false)
-#define NNPE_SF_FORCE(NT, EV) if (S.rad_recur) NNPE_SF_FORCE1(NT, EV, true); else NNPE_SF_FORCE1(NT, EV, false)
+#define NNPE_SF_FORCE(NT, EV) if (S.rad_horner) NNPE_SF_FORCE1(NT, EV, true); else NNPE_SF_FORCE1(NT, EV, false)
   switch (ctx->ntype) {
     case 1: if (ev) NNPE_SF_FORCE(1, true); else NNPE_SF_FORCE(1, false); break;
     case 2: if (ev) NNPE_SF_FORCE(2, true); else NNPE_SF_FORCE(2, false); break;
