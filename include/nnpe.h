@@ -97,6 +97,13 @@ int nnpe_bonded_set(nnpe_ctx* ctx, int kind, int ntyp, const float* coef, int nt
 /* BOND/ANGLE/DIHEDRAL::compute<ev> (src/bond.cpp:87-116, src/angle.cpp:87-136,
  * src/dihedral.cpp:90-147) on unwrapped molecule positions.  Adds to the force array. */
 int nnpe_bonded_compute(nnpe_ctx* ctx, int ev);
+/* No counterpart in the reference: nnpe_bonded_compute for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same forces and energy/virial sums, bit for bit; the per-workgroup partial sums live in a
+ * fixed device array (768 KiB per device, held until the process ends) instead of the
+ * context's buffer, so the call allocates nothing.  At most 1,048,576 atoms
+ * (NNPE_ERR_UNSUPPORTED beyond).  Calls that overlap on one device share that array and race. */
+int nnpe_bonded_compute_noalloc(nnpe_ctx* ctx, int ev);
 
 /* Energy and virial sums (FP64) of the ev compute calls since the last clear:
  * out[2*term] = energy, out[2*term+1] = virial, term = nnpe_term. */

@@ -37,6 +37,7 @@ SIGNATURES = [
     ("nnpe_pair_compute", _I, [_P, _I]),
     ("nnpe_bonded_set", _I, [_P, _I, _I, _PF, _I, _PI]),
     ("nnpe_bonded_compute", _I, [_P, _I]),
+    ("nnpe_bonded_compute_noalloc", _I, [_P, _I]),
     ("nnpe_energy_get", _I, [_P, _PD]),
     ("nnpe_energy_clear", _I, [_P]),
     ("nnpe_verlet_set", _I, [_P, _D, _I]),

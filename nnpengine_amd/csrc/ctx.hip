@@ -48,6 +48,11 @@ extern "C" int nnpe_create(nnpe_ctx** out, int device) {
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->ev_sum, 2 * NNPE_NTERM));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->ev_sum, 0, sizeof(double) * 2 * NNPE_NTERM));
   NNPE_CHECK_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->h_ev), sizeof(double) * 2 * NNPE_NTERM, hipHostMallocDefault));
+  const int rcs = bonded_scratch_setup(ctx);  // the allocation-free bonded entry point's device array
+  if (rcs) {
+    nnpe_destroy(ctx);
+    return rcs;
+  }
   *out = ctx;
   return NNPE_OK;
 }
@@ -214,8 +219,11 @@ __global__ void k_ev_finish(const double* part_e, const double* part_w, int nblo
 }
 
 int ev_finish(nnpe_ctx* ctx, int term, int nblocks) {
-  hipLaunchKernelGGL(k_ev_finish, dim3(1), dim3(64), 0, ctx->stream, ev_part(ctx, term, 0), ev_part(ctx, term, 1),
-                     nblocks, ctx->ev_sum + 2 * term);
+  return ev_finish_from(ctx, term, ev_part(ctx, term, 0), ev_part(ctx, term, 1), nblocks);
+}
+
+int ev_finish_from(nnpe_ctx* ctx, int term, const double* part_e, const double* part_w, int nblocks) {
+  hipLaunchKernelGGL(k_ev_finish, dim3(1), dim3(64), 0, ctx->stream, part_e, part_w, nblocks, ctx->ev_sum + 2 * term);
   NNPE_CHECK_LAUNCH(ctx);
   return NNPE_OK;
 }

@@ -15,6 +15,7 @@
 // atom's bonded force has one writer and a fixed summation order, and all rows work at once.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "internal.h"
 
@@ -473,4 +474,69 @@ extern "C" int nnpe_bonded_compute(nnpe_ctx* ctx, int ev) {
   if (!ctx) return NNPE_ERR_ARG;
   if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "bonded compute needs a list build (unwrapped images)");
   return bonded_enqueue(ctx, ev != 0);
+}
+
+// ---- the bonded pass without allocation ---------------------------------------------------------
+// The per-workgroup energy/virial partials of the three bonded terms live in one file-scope
+// device array instead of the context's partial buffer (which nnpe_bonded_compute may grow).
+// Each device holds its own copy at its own address, found once per device when a context is
+// created on it.  Every call on a device shares that device's array: calls that overlap on one
+// device race, and each call overwrites every partial it reads.
+namespace {
+constexpr int kScratchBlocks = 1 << 14;  // 64-atom workgroups: at most 1,048,576 atoms
+constexpr int kScratchDevices = 64;
+__device__ __align__(256) double scratch[6 * kScratchBlocks];  // [bond e, bond w, angle e, angle w, dihedral e, dihedral w]
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};  // device address of scratch, per device
+}  // namespace
+
+int bonded_scratch_setup(nnpe_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kScratchDevices) return NNPE_OK;  // the entry point refuses such devices
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[ctx->device]) return NNPE_OK;
+  void* p = nullptr;
+  NNPE_CHECK_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // with the context's device current
+  g_scratch_ptr[ctx->device] = static_cast<double*>(p);
+  return NNPE_OK;
+}
+
+extern "C" int nnpe_bonded_compute_noalloc(nnpe_ctx* ctx, int ev) {
+  if (!ctx) return NNPE_ERR_ARG;
+  if (!ctx->have_list) return ctx_fail(ctx, NNPE_ERR_STATE, "bonded compute needs a list build (unwrapped images)");
+  double* sc = nullptr;
+  if (ctx->device >= 0 && ctx->device < kScratchDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    sc = g_scratch_ptr[ctx->device];
+  }
+  if (!sc) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "no scratch array on this device");
+  const int nblk = blocks_for(ctx->natom, 64);
+  if (nblk > kScratchBlocks)
+    return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "nnpe_bonded_compute_noalloc: more than 1048576 atoms");
+  BondedArgs A;
+  int total = 0;
+  for (int k = 0; k < 3; ++k) {
+    const bool on = ctx->bd_nterm[k] > 0;
+    if (on && ctx->bd_start_len[k] != ctx->natom + 1)
+      return ctx_fail(ctx, NNPE_ERR_STATE, "bonded terms were set for another atom count");
+    A.start[k] = on ? ctx->bd_start[k] : nullptr;
+    A.ref[k] = ctx->bd_ref[k];
+    A.type[k] = ctx->bd_type[k];
+    A.atoms[k] = ctx->bd_atoms[k];
+    A.coef[k] = ctx->bd_coef[k];
+    total += ctx->bd_nterm[k];
+  }
+  if (total == 0) return NNPE_OK;
+  auto P = [&](int term, int w) { return sc + (size_t)(2 * (term - NNPE_TERM_BOND) + w) * kScratchBlocks; };
+  if (ev)
+    hipLaunchKernelGGL(k_bonded<true>, dim3(nblk), dim3(64), 0, ctx->stream, ctx->natom, A, ctx->info, ctx->inv, ctx->xq,
+                       ctx->xmol_code, ctx->d_box, ctx->rbox, ctx->f, P(NNPE_TERM_BOND, 0), P(NNPE_TERM_BOND, 1),
+                       P(NNPE_TERM_ANGLE, 0), P(NNPE_TERM_ANGLE, 1), P(NNPE_TERM_DIHEDRAL, 0), P(NNPE_TERM_DIHEDRAL, 1));
+  else
+    hipLaunchKernelGGL(k_bonded<false>, dim3(nblk), dim3(64), 0, ctx->stream, ctx->natom, A, ctx->info, ctx->inv, ctx->xq,
+                       ctx->xmol_code, ctx->d_box, ctx->rbox, ctx->f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  NNPE_CHECK_LAUNCH(ctx);
+  if (!ev) return NNPE_OK;
+  int rc = NNPE_OK;
+  for (int t = NNPE_TERM_BOND; t <= NNPE_TERM_DIHEDRAL && !rc; ++t) rc = ev_finish_from(ctx, t, P(t, 0), P(t, 1), nblk);
+  return rc;
 }

@@ -436,6 +436,11 @@ struct VvArgs {
 };
 
 int ev_finish(nnpe_ctx* ctx, int term, int nblocks);  // add partials of a term into ev_sum
+// add per-block partials held elsewhere (the same fixed-order reduction) into ev_sum
+int ev_finish_from(nnpe_ctx* ctx, int term, const double* part_e, const double* part_w, int nblocks);
+// the allocation-free bonded entry point: address of its device array on the context's device
+// (once per device, at context creation; forces.hip)
+int bonded_scratch_setup(nnpe_ctx* ctx);
 double* ev_part(nnpe_ctx* ctx, int term, int which);  // partial array for term (0 energy, 1 virial)
 int counters_read(nnpe_ctx* ctx);
 // upload ctx->h_box and a cell -> box map for the single-box layout

@@ -123,6 +123,10 @@ class Engine:
     def bonded_compute(self, ev=True):
         self._check(self.L.nnpe_bonded_compute(self.h, int(ev)))
 
+    def bonded_compute_noalloc(self, ev=True):
+        """nnpe_bonded_compute with the partial sums in the library's fixed device array."""
+        self._check(self.L.nnpe_bonded_compute_noalloc(self.h, int(ev)))
+
     def clear_force(self):
         self._check(self.L.nnpe_clear_force(self.h))
 
