@@ -28,6 +28,13 @@ WIDTHS = [80, 128, 128, 128, 1]
 DT = 0.5
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # TFLOP/s dense
+# non-packed FP32 vector rate: 256 CUs x 64 lanes x 2 (FMA) x 2.4 GHz (the library builds without
+# packed FP32 instructions)
+VALU_PEAK_TFLOPS = 78.6
+# algorithmic FP32 operations per directed pair within the symmetry-function cutoff
+SF_FORCE_FLOP_PAIR = 405   # geometry 16, cutoff 32, angular weights 21, radial polynomials 74,
+                           # adjoint combination 60, angular gradient 4 x 48, force 10
+SF_DESC_FLOP_PAIR = 205    # geometry 16, cutoff 18, radial Gaussians and sums 67, angular moments 4 x 26
 
 
 def log(*a):
@@ -51,6 +58,22 @@ def make_engine(sysm, par, norm, prec):
     return e
 
 
+def gpu_standardisation(sysm):
+    """Per-element input standardisation of the network from the starting snapshot's
+    descriptors, computed by the engine."""
+    from nnpengine_amd import model as M
+    from nnpengine_amd.engine import Engine
+    sfp = M.sf_params(RC_NNP)
+    e = Engine(sysm)
+    e.neigh_init(RC_PAIR, SKIN, R_GR, RC_NNP)
+    e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    e.build()
+    e.sf_descriptor()
+    G0 = e.nnp_get("G").astype(np.float64)
+    e.close()
+    return M.standardisation(G0, sysm.typ, sysm.ntype)
+
+
 MD_KW = dict(dt=DT, fp64=True, rebuild_every=10, pair=True, bonded=True, nnp=2, gr_every=10, gr_long_every=10,
              sq=True, use_graph=True)
 
@@ -64,9 +87,41 @@ def time_steps(e, steps, warmup):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(sysm, par, norm):
-    """The oracle's float32 restatement of the same step on the host (one thread), on a bounded
-    sample: 5 steps of the 24k-atom workload (about 10 s)."""
+def cpu_reference(sysm):
+    """The reference's own classical MD loop, compiled from its sources (oracle/_ref/ref_harness:
+    refold/sort/ghosts/NEIGHLIST::build on NEIGHLIST::isvalid, BOND/ANGLE/DIHEDRAL::compute,
+    LJTableCoulDSF::compute, VERLET, src/run.cpp:204-394) with its 27 OpenMP threads
+    (src/environ.h:85-90), on the same 24000-atom snapshot and velocities at dt = 0.5 fs.  The
+    time per step is the difference of the MD-loop wall times of a 10-step and a 30-step run.
+    Returns None when the harness is not built."""
+    from nnpengine_amd import system as S
+    from oracle import refrun
+    if not refrun.available():
+        return None
+    ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
+    t = {}
+    for n in (10, 30):
+        r = refrun.run(sysm, RC_PAIR, SKIN, RC_PAIR, ft, ut, mode="md", nsteps=n, dt=DT)
+        t[n] = (r["t_loop"], int(r["nbuild"]), int(r["threads"]))
+    per_step = (t[30][0] - t[10][0]) / 20
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    threads = t[30][2]
+    return {"value": sysm.natom / per_step, "unit": "atom-steps/s", "cores": cores, "threads": threads, "kind": "reference",
+            "ms_per_step": per_step * 1e3,
+            "oversubscribed": threads > cores,
+            "sample": f"the reference's classical step compiled from its sources, {threads} OpenMP threads on {cores} "
+                      f"host cores: tabulated LJ + shifted-force Coulomb 9 A, bonded terms, lists 11 A rebuilt when an "
+                      f"atom moved > skin/2 ({t[30][1]} builds in 30 steps), FP32 Verlet, 24000-atom water at dt 0.5 fs; "
+                      f"per-step time from 10- and 30-step runs ({t[10][0]:.2f} s, {t[30][0]:.2f} s).  The NNP and g(r) "
+                      f"are not in it: pair_nnp.cpp and grsq.cpp include the TensorFlow C header, absent here"}
+
+
+def cpu_port(sysm, par, norm):
+    """The oracle's float32 restatement of the whole NNP step on the host (one thread), on a
+    bounded sample: 5 steps of the 24k-atom workload (about 10 s)."""
     from nnpengine_amd import model as M
     from nnpengine_amd import system as S
     from oracle import pyoracle
@@ -104,7 +159,7 @@ def cpu_baseline(sysm, par, norm):
     # the workload rebuilds lists and tallies g(r)/S(q) every 10th step
     per_step = (t_step + (t_reb + t_gr) / 10) / nsteps
     total = t_reb + t_gr + t_step
-    return {"value": sysm.natom / per_step, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+    return {"value": sysm.natom / per_step, "unit": "atom-steps/s", "cores": 1, "threads": 1, "kind": "port",
             "sample": f"{nsteps} steps of the 24000-atom workload ({total:.1f} s): forces (pair, bonded, NNP "
                       f"descriptors, FP64 MLP, NNP forces) + Verlet every step, list rebuild and 12 A g(r) + S(q) "
                       f"charged at 1/10 per step; oracle float32 restatement, 1 thread"}
@@ -198,10 +253,14 @@ def nve_drift(sysm, par, norm, prec, ps, log_every=200):
     e.close()
     et = log[:, [2 * k for k in E_TERMS]].sum(1) / sysm.natom
     t_ns = np.arange(len(et)) * log_every * DT * 1e-6
-    slope = float(np.polyfit(t_ns, et, 1)[0]) if len(et) > 2 else float("nan")
+    # the step-0 record holds the starting velocities, which the integrator's start-up takes as
+    # half-step velocities (the reference's setDt(-1) / updateX start, src/run.cpp:193): its
+    # kinetic energy is not on the trajectory's energy surface, so the fit starts at record 1
+    slope = float(np.polyfit(t_ns[1:], et[1:], 1)[0]) if len(et) > 3 else float("nan")
     ke = log[:, 14] / sysm.natom
     return {"ps": steps * DT / 1000, "steps": steps, "drift_kcal_mol_atom_per_ns": slope,
-            "e_tot_per_atom_first_last": [float(et[0]), float(et[-1])], "e_tot_per_atom_std": float(et.std()),
+            "e_tot_per_atom_first_last": [float(et[1]), float(et[-1])], "e_tot_per_atom_std": float(et[1:].std()),
+            "e_tot_per_atom_step0": float(et[0]),
             "ke_per_atom_mean": float(ke.mean()), "skin_violations": int(st["skin_violations"]),
             "repeated_chunks": int(redo), "wall_s": wall}, x
 
@@ -287,6 +346,69 @@ def valu_busy(stage):
     return None
 
 
+def cpu_baseline_leg(sysm, par, norm, x_end, drift, do_cpu, do_err):
+    """The host side of the run: the reference's own CPU step and the oracle's restatement of the
+    NNP step timed on the host cores, and the oracle as the checker of the GPU forces (FP64
+    reference forces on the starting snapshot and on the liquid after the NVE run)."""
+    cpu = errs = None
+    if do_err:
+        errs = {"start": force_errors(sysm, par, norm)}
+        if x_end is not None:
+            # the liquid after the NVE run of the timed form (positions as the engine holds them)
+            snap = sysm.copy()
+            snap.x = np.ascontiguousarray(x_end, np.float32)
+            errs[f"after_{drift[HEAD]['ps']:g}ps_{HEAD}_nve"] = force_errors(snap, par, norm)
+    if do_cpu:
+        ref = cpu_reference(sysm)
+        port = cpu_port(sysm, par, norm)
+        if ref is not None:
+            cpu = dict(ref)
+            cpu["port_nnp_step"] = port
+        else:
+            cpu = port
+    return cpu, errs
+
+
+def refinement(steps, target_ps):
+    """The butane refinement workload (4000 chains, 16 000 sites, Ryckaert-Bellemans torsion):
+    the target S(q) from an MD run with LJ epsilon x 1.1 (target_ps of FP64-Verlet MD, classical
+    terms), then one refinement iteration of `steps` MD steps with the BF16 NNP on, g(r) + S(q) +
+    RMDF every 10 steps and the pair table rewritten every 100 steps, timed."""
+    from nnpengine_amd import workloads as W
+    c = W.BUTANE_C3
+    t0 = time.perf_counter()
+    sysm = W.butane_c3_system()
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    target_steps = int(round(target_ps * 1000 / c["dt"]))
+    target_steps -= target_steps % c["rebuild_every"]
+    s_exp, _ = W.butane_target(sysm, target_steps)
+    t_target = time.perf_counter() - t0
+    ft, ut = W.butane_tables(sysm)
+    e = W.classical_engine(sysm, ft, ut, nnp=True)
+    W.nnp_attach(e, sysm, HEAD)
+    e.rmdf_init(W.rmdf_data(s_exp, sysm.ntype, c["q_max"]), 1.0, 0.5, target=0)
+    kw = dict(dt=c["dt"], fp64=True, rebuild_every=c["rebuild_every"], pair=True, bonded=True, nnp=2,
+              gr_every=c["gr_every"], gr_long_every=c["gr_every"], sq=True, rmdf=True, log_every=100)
+    e.md_run(200, **kw)  # warm-up: graphs captured, the refinement under way
+    e.sync()
+    t0 = time.perf_counter()
+    lg = e.md_run(steps, **kw)
+    e.sync()
+    t = time.perf_counter() - t0
+    st = e.md_stats()
+    from nnpengine_amd import _lib
+    k = _lib.TERMS.index("rmdf")
+    res = lg[:, 2 * k]
+    e.close()
+    return {"workload": f"butane refinement: {sysm.natom} sites ({sysm.nmol} chains), L={sysm.bl:.1f} A, RB torsion, "
+                        f"BF16 NNP ({HEAD}), g(r) + S(q) + RMDF every 10 steps, table rewrite every 100 steps",
+            "steps": steps, "seconds": t, "ms_per_step": t / steps * 1e3, "atom_steps_per_s": sysm.natom * steps / t,
+            "target_ps": target_steps * c["dt"] / 1000, "target_seconds": t_target, "build_seconds": t_build,
+            "residual_first_last": [float(res[0]), float(res[-1])] if len(res) else None,
+            "list_builds": int(st["nbuild"]), "skin_violations": int(st["skin_violations"])}
+
+
 def force_errors(sysm, par, norm):
     """NNP and total force relative RMS vs the FP64 oracle on one snapshot."""
     from nnpengine_amd import model as M
@@ -350,6 +472,8 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--no-sqd", action="store_true")
     ap.add_argument("--drift-ps", type=float, default=20.0, help="NVE run per precision form for the drift (0: skip)")
+    ap.add_argument("--refine-steps", type=int, default=2000, help="steps of the timed butane refinement iteration (0: skip)")
+    ap.add_argument("--refine-target-ps", type=float, default=20.0, help="MD run that makes the refinement target S(q)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a sharded run is not "
@@ -357,19 +481,11 @@ def main():
         sys.exit(2)
     from nnpengine_amd import model as M
     from nnpengine_amd import system as S
-    from oracle import pyoracle
 
     sysm = S.water_box(NMOL, BL, seed=7)
-    # network: seeded weights, descriptor standardisation from the starting snapshot (FP64 oracle)
-    sfp = M.sf_params(RC_NNP)
-    o = pyoracle.Oracle(sysm, 1)
-    o.neigh_init(RC_PAIR, SKIN, R_GR)
-    o.rebuild(True)
-    o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
-    G0 = o.sf_descriptor()
-    norm = M.standardisation(G0, sysm.typ, sysm.ntype)
+    # network: seeded weights, descriptor standardisation from the starting snapshot's descriptors
+    norm = gpu_standardisation(sysm)
     par = M.mlp_init(WIDTHS, sysm.ntype, 2024)
-    del o
     log(f"[bench] system {sysm.natom} atoms")
 
     res = {}
@@ -390,6 +506,11 @@ def main():
         eb.md_run(1, log_every=0, **MD_KW)
         lat.append((time.perf_counter() - t0) * 1e3)
     lat = np.array(lat)
+    # list sizes of the step as run (before the stage profile, whose Verlet stage moves the atoms
+    # without new forces): entries of the pair and short lists, pairs within the SF cutoff
+    n_long, n_ghost = eb.list_size(0)
+    n_short, _ = eb.list_size(1)
+    n_act = active_pairs(eb, sysm.bl, RC_NNP)
 
     # per-stage profile (HIP events on the engine stream) of the BF16 form
     stages = {}
@@ -402,9 +523,6 @@ def main():
 
     # roofline of the dominant kernel: the largest share of the step (stage time x launches per
     # step; rebuild and the full g(r) tally run every 10th step)
-    n_long, n_ghost = eb.list_size(0)
-    n_short, _ = eb.list_size(1)
-    n_act = active_pairs(eb, sysm.bl, RC_NNP)
     N = sysm.natom
     flop_atom = 4 * sum(WIDTHS[i] * WIDTHS[i + 1] for i in range(len(WIDTHS) - 1))
     # (the full g(r) tally runs on the rebuild steps, folded into the rebuild: stage rebuild_gr)
@@ -441,6 +559,15 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": alg_bytes[dom]}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = hbm_traffic(dom)
+    # the symmetry-function passes against the vector-ALU roofline: algorithmic FP32 operations
+    # per directed pair within the cutoff (FMA = 2, exp / rsqrt = 1; counted in the design notes)
+    # over the non-packed FP32 rate
+    alg_flops = {"sf_force": SF_FORCE_FLOP_PAIR * n_act, "sf_desc": SF_DESC_FLOP_PAIR * n_act}
+    if dom in alg_flops:
+        ach = alg_flops[dom] / (stages[dom] * 1e-3) / 1e12
+        roof["valu"] = {"algorithmic_flop": alg_flops[dom], "flop_per_pair": alg_flops[dom] / n_act,
+                        "active_pairs": n_act, "achieved_tflops": ach, "peak_tflops": VALU_PEAK_TFLOPS,
+                        "frac_valu": ach / VALU_PEAK_TFLOPS}
     # the SF and pair kernels are bound by their vector-ALU issue and gather latency rather than
     # by HBM: the committed VALU counter pass gives the kernel's VALU busy share beside frac
     roof["valu_busy"] = valu_busy(dom)
@@ -458,6 +585,8 @@ def main():
         ach = alg_bytes[st] / (stages[st] * 1e-3) / 1e9
         stage_roof[st] = {"algorithmic_bytes": alg_bytes[st], "achieved_gbs": ach, "frac_hbm": ach / HBM_PEAK_GBS,
                           "traffic": hbm_traffic(st), "valu_busy": valu_busy(st)}
+        if st in alg_flops:
+            stage_roof[st]["frac_valu"] = alg_flops[st] / (stages[st] * 1e-3) / 1e12 / VALU_PEAK_TFLOPS
 
     sqd = None if args.no_sqd else sqd_timing(eb)
 
@@ -474,15 +603,11 @@ def main():
             if prec == HEAD:
                 x_end = xe
             log(f"[bench] NVE {prec}: {drift[prec]}")
-    errs = None
-    if not args.no_err:
-        errs = {"start": force_errors(sysm, par, norm)}
-        if x_end is not None:
-            # the liquid after the NVE run of the timed form (positions as the engine holds them)
-            snap = sysm.copy()
-            snap.x = np.ascontiguousarray(x_end, np.float32)
-            errs[f"after_{drift[HEAD]['ps']:g}ps_{HEAD}_nve"] = force_errors(snap, par, norm)
-    cpu = None if args.no_cpu else cpu_baseline(sysm, par, norm)
+    refine = None
+    if args.refine_steps > 0:
+        refine = refinement(args.refine_steps, args.refine_target_ps)
+        log("[bench] refinement:", refine)
+    cpu, errs = cpu_baseline_leg(sysm, par, norm, x_end, drift, not args.no_cpu, not args.no_err)
 
     t_b, t_f = res[HEAD], res["fp32"]
     value = N * args.steps / t_b
@@ -498,6 +623,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
+        "precision_form": "NNP hidden layers in BF16 on MFMA (one BF16 term per value, FP32 accumulation, "
+                          "FP32 bias/tanh/epilogues); descriptors, tabulated pair terms and the own-atom "
+                          "adjoints FP32; the neighbours' adjoints (dE/dmoment) gathered in FP16 by the SF "
+                          "force pass; Verlet, energy/virial sums and S(q) FP64",
         "data": "synthetic (seeded water lattice, Maxwell-Boltzmann 300 K, random-init NNP weights)",
         "config": {"workload": "24000-atom water NNP-MD step (8000 molecules, L=62.1 A, LJ+Coulomb table 9 A, "
                                "symmetry-function NNP 6 A D=80, MLP 3x128 tanh, lists 11 A rebuilt every 10 steps, "
@@ -519,6 +648,7 @@ def main():
         "roofline": roof,
         "stage_roofline": stage_roof,
         "sqd": sqd,
+        "refinement": refine,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
