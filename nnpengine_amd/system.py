@@ -430,10 +430,81 @@ def write_classical_input(d, sysm, pair_cutoff, skin, gr_cutoff, force_tbl, pot_
     strength, gamma, files, delta = rmdf if rmdf else (0.0, 1.0, [], "*")
     lines += [f"{nbin_gr} {nq} {q_max:.9g}", f"{strength:.9g} {gamma:.9g}", " ".join([str(len(files))] + list(files)),
               delta, f"{temp[0]:.9g} {temp[1]:.9g}", " ".join(str(s) for s in steps)]
-    o = outputs or {}
-    lines += [f"{o.get('xyz', '*')} 0", f"{o.get('gr', '*')} 0", f"{o.get('sq', '*')} 0",
-              f"{o.get('sqd', '*')} 0 {sqd[0]:.9g} {sqd[1]}", f"{o.get('sqx', '*')} 0", f"{o.get('force', '*')} 0"]
+    lines += _output_lines(outputs, sqd, force=True)
     path = os.path.join(d, "in.classical")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
+def _output_lines(outputs, sqd, force):
+    """The dump lines at the end of the reference's inputs: name and append flag for the xyz,
+    g(r), S(q), SQD (+ q_max, nq), sqx and (classical) refined-table dumps; an output may be
+    given as a name or as (name, append)."""
+    o = outputs or {}
+
+    def nm(k):
+        v = o.get(k, "*")
+        return (v, 0) if isinstance(v, str) else (v[0], int(v[1]))
+
+    out = [f"{nm('xyz')[0]} {nm('xyz')[1]}", f"{nm('gr')[0]} {nm('gr')[1]}", f"{nm('sq')[0]} {nm('sq')[1]}",
+           f"{nm('sqd')[0]} {nm('sqd')[1]} {sqd[0]:.9g} {sqd[1]}", f"{nm('sqx')[0]} {nm('sqx')[1]}"]
+    if force:
+        out.append(f"{nm('force')[0]} {nm('force')[1]}")
+    return out
+
+
+def write_nnp_input(d, sysm, pair_cutoff, skin, gr_cutoff, nbin_tbl, tbl_coef, widths, params, out_scale, precision,
+                    nbin_gr, nq, q_max, temp=(300.0, 300.0), steps=(100, 10, 10, 10, 10), outputs=None, rmdf=None,
+                    sqd=(10.0, 10)):
+    """The positional input file of ``nnp_run`` (src/run.cpp:968-1100) and the files it names:
+    configuration and molecules, the parameter file (its first token names the network file;
+    the reference names a TF SavedModel there), the network file (text: nlayer, widths,
+    out_scale, precision, then per element and layer W [in][out] and b), and the binary weight
+    file the reference reads (src/pair_nnp.cpp:505-517): bond, angle and dihedral coefficients
+    as ``setParam()`` expects them (the factor-2 terms not yet doubled), then coef_p
+    [npair][4][6], coef_w, coef_d, coef_r [npair][80] as float32.  Returns the input path."""
+    import os
+    mols = []
+    for k, mt in enumerate(sysm.moltypes):
+        p = os.path.join(d, f"mol{k}.txt")
+        write_molecule(p, mt)
+        mols.append(p)
+    cfg = os.path.join(d, "config.txt")
+    write_config(cfg, sysm, mols)
+    net = os.path.join(d, "net.txt")
+    with open(net, "w") as fh:
+        fh.write(f"{len(widths) - 1} " + " ".join(str(w) for w in widths) + f" {out_scale:.9g} {precision}\n")
+        fh.write("\n".join(f"{v:.9e}" for v in np.asarray(params, np.float32)) + "\n")
+    prm = os.path.join(d, "nnp_param.txt")
+    with open(prm, "w") as fh:
+        fh.write(net + "\n")
+    wgt = os.path.join(d, "weights.bin")
+
+    def undouble(coef, cols):
+        c = np.array(coef, np.float32).reshape(-1, 4).copy()
+        for k in cols:
+            c[:, k] /= 2
+        return c
+
+    cp, cw, cd, cr = tbl_coef
+    dih = np.zeros((0, 4), np.float32) if sysm.dihedral_coef is None else np.asarray(sysm.dihedral_coef, np.float32)
+    # DIHEDRAL::setParam() multiplies coef[2] by 2 and coef[3] by 3 (src/dihedral.cpp:37-44)
+    dih_file = dih.reshape(-1, 4).copy()
+    dih_file[:, 2] /= 2
+    dih_file[:, 3] /= 3
+    parts = [undouble(sysm.bond_coef, (1, 3)), undouble(sysm.angle_coef, (1, 3)), dih_file,
+             np.asarray(cp, np.float32), np.asarray(cw, np.float32), np.asarray(cd, np.float32), np.asarray(cr, np.float32)]
+    with open(wgt, "wb") as fh:
+        for a in parts:
+            fh.write(np.ascontiguousarray(a, np.float32).tobytes())
+    strength, gamma, files, delta = rmdf if rmdf else (0.0, 1.0, [], "*")
+    lines = [cfg, f"{pair_cutoff:.9g} {skin:.9g} {gr_cutoff:.9g}", str(len(sysm.bond_coef)), str(len(sysm.angle_coef)),
+             str(len(dih)), prm, wgt, str(nbin_tbl), f"{nbin_gr} {nq} {q_max:.9g}", f"{strength:.9g} {gamma:.9g}",
+             " ".join([str(len(files))] + list(files)), delta, f"{temp[0]:.9g} {temp[1]:.9g}",
+             " ".join(str(s) for s in steps)]
+    lines += _output_lines(outputs, sqd, force=False)
+    path = os.path.join(d, "in.nnp")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     return path
