@@ -6,6 +6,10 @@
 // a_0 = G_i.  Backward (hand-written): d_L = s_out * w_out * (1 - a_L^2),
 // d_{l-1} = (d_l W_l^T) * (1 - a_{l-1}^2),  dE/dG = d_1 W_1^T.
 //
+// Two kernels of the same shape, picked once in nnpe_mlp_set: k_mlp for any widths and every
+// form, and k_mlp_w, the 16-bit forms' kernel for hidden layers all 128 wide and descriptors at
+// most 128 wide (the production network), trimmed for VALU work and with its weight loads
+// issued one product ahead.
 // Kernel shape: one workgroup of kMlpWaves waves owns a tile of 32 atoms of one element and runs
 // the whole forward and backward for them; within a layer wave w computes the 32-column output
 // blocks w, w + kMlpWaves, ... and a barrier separates the layers.  Activations live in the
@@ -429,6 +433,226 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
   }
 }
 
+// Uniform-width fast path of the 16-bit forms (FP16, one-term BF16): every hidden layer 128
+// wide (one 32-column block per wave in every product), descriptors at most 128 wide (D a
+// multiple of 4).  Same data layout, LDS slabs and per-lane fragment mapping as k_mlp; what the
+// fixed width buys is VALU economy, which bounds this kernel (the MFMA work of a 32-row tile is a
+// fifth of its VALU time): compile-time strides and trip counts (no per-fragment bounds tests or
+// address arithmetic), descriptor staging with one column group per thread (no index division),
+// and a cheaper epilogue.  With e = exp(-2|z|) and r = 1/(1 + e): a = sign(z) (r - e r) and the
+// stored derivative is t' = e r^2 = (1 - a^2) / 4; the factor 4 is restored in the power-of-two
+// scale constants of the backward products (exact), so each value costs one multiply less than
+// forming 1 - a^2 = 4 e r^2 and (1 - e) r separately.
+constexpr int kW = kMaxWidth;   // the hidden width of the fast path
+constexpr int kLdW = kW + 8;    // its LDS row stride (16-bit elements)
+
+// The B fragments of one wave's 32-column block cb of fragment-ordered weights bm [n][K]
+// (K <= 128, a multiple of 16), one 16-byte load per fragment from L2
+template <int KIND>
+__device__ __forceinline__ void load_b(typename Prec<KIND>::F (&b)[8], const typename Prec<KIND>::T* bm, int K, int cb,
+                                       int lane) {
+  const typename Prec<KIND>::T* bp = bm + (size_t)cb * (K / 16) * 512 + lane * 8;
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+    if (s * 16 < K) b[s] = Prec<KIND>::ld(bp + s * 512);
+}
+
+// 32 x 32 output block of A [32 x K] (LDS, row stride lda) times the B fragments b: the A
+// fragments in two halves (register budget), then the MFMA chain
+template <int KIND>
+__device__ __forceinline__ f32x16 gemm_b(const typename Prec<KIND>::T* A, int lda, const typename Prec<KIND>::F (&b)[8],
+                                         int K, int lane) {
+  typedef Prec<KIND> PR;
+  typedef typename PR::F F;
+  f32x16 acc;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+  const typename PR::T* arow = A + (lane & 31) * lda + 8 * (lane >> 5);
+#pragma unroll
+  for (int h = 0; h < 8; h += 4) {
+    F a[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if ((h + s) * 16 < K) a[s] = PR::ld(arow + (h + s) * 16);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if ((h + s) * 16 < K) acc = PR::mma(b[h + s], a[s], acc);
+  }
+  return acc;
+}
+
+// (three waves per SIMD: registers for the fragments in flight beside the live ones)
+template <int KIND, int NH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_mlp_w(MlpDev P, const float* __restrict__ G, float* __restrict__ dG, float* __restrict__ e_atom,
+        const int* __restrict__ type_idx) {
+  typedef Prec<KIND> PR;
+  typedef typename PR::T T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
+  const int tid = threadIdx.x, lane = tid & 63, cb = tid >> 6;
+  const int tile = blockIdx.x;
+  int e = 0;
+  while (e + 1 < kMaxType && tile >= P.tile_off[e + 1]) ++e;
+  const int r0 = (tile - P.tile_off[e]) * 32;
+  const int nrows = min(32, P.count[e] - r0);
+  const int* rows = type_idx + P.type_off[e] + r0;
+  const int D = P.D, Dp = P.Dp, ldg = P.ld[0];
+  T* Gs = lds;
+  T* act = lds + P.slab_off[1];  // slab s = 1 .. NH at act + (s - 1) * 32 * kLdW
+
+  // B fragments of the next product, issued one product ahead: layer 0's before the staging,
+  // each later product's right after the previous MFMA chain, so the L2 round trip overlaps an
+  // epilogue and a barrier instead of stalling the chain
+  typename PR::F b[8];
+  load_b<KIND>(b, reinterpret_cast<const T*>(P.Wt[e][0]), Dp, cb, lane);
+
+  __shared__ int srow[32];
+  // the epilogues work on y = kz, k = 2 / ln 2, so that exp(-2|z|) = 2^(-|y|) is one instruction:
+  // biases staged as k b; output weights staged twice, for the energy and pre-multiplied by the
+  // scale constant of d_L
+  constexpr float kTanh = 2.8853900817779268f;
+  __shared__ __align__(16) float s_wout[kW];
+  __shared__ __align__(16) float s_wd[kW];
+  __shared__ __align__(16) float s_bias[NH][kW];
+  __shared__ float s_epart[4][32];
+  if (tid < 32) srow[tid] = tid < nrows ? rows[tid] : -1;
+  if (tid < kW) {
+    const float w = P.wout[e][tid];
+    s_wout[tid] = w;
+    s_wd[tid] = 4.0f * P.out_scale * P.s_d[NH - 1] * w;
+  }
+#pragma unroll
+  for (int l = 0; l < NH; ++l)
+    if (tid < kW) s_bias[l][tid] = kTanh * P.b[e][l][tid];
+  __syncthreads();
+  // standardised descriptors: thread -> column group c4 = tid % 32 of rows tid / 32 + 8 i
+  {
+    const int c4 = tid & 31, D4 = D >> 2;
+    if (c4 < (Dp >> 2)) {
+      const int cc = min(c4, D4 - 1);
+      const float4 sh = reinterpret_cast<const float4*>(P.shift[e])[cc];
+      const float4 sc = reinterpret_cast<const float4*>(P.scale[e])[cc];
+      const float4* G4 = reinterpret_cast<const float4*>(G);
+      float4 g[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = srow[(tid >> 5) + 8 * i];
+        g[i] = G4[(size_t)(row >= 0 ? row : srow[0]) * D4 + cc];  // in-bounds clamp; padding zeroed below
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = (tid >> 5) + 8 * i;
+        float4 v = make_float4((g[i].x - sh.x) * sc.x, (g[i].y - sh.y) * sc.y, (g[i].z - sh.z) * sc.z,
+                               (g[i].w - sh.w) * sc.w);
+        if (c4 >= D4 || srow[r] < 0) v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const T o[4] = {PR::cvt(v.x), PR::cvt(v.y), PR::cvt(v.z), PR::cvt(v.w)};
+        store4(Gs + r * ldg + 4 * c4, o);
+      }
+    }
+  }
+  __syncthreads();
+
+  const int row = lane & 31, f0 = cb * 32 + 4 * (lane >> 5);  // this lane: features f0 + 8q + t of row `row`
+  float tv[NH > 1 ? NH - 1 : 1][16];                           // t' of the hidden layers 0 .. NH-2
+  // forward
+#pragma unroll
+  for (int l = 0; l < NH; ++l) {
+    const T* src = l == 0 ? Gs : act + (l - 1) * 32 * kLdW;
+    T* dst = act + l * 32 * kLdW;
+    const f32x16 acc = gemm_b<KIND>(src, l == 0 ? ldg : kLdW, b, l == 0 ? Dp : kW, lane);
+    if (l + 1 < NH)
+      load_b<KIND>(b, reinterpret_cast<const T*>(P.Wt[e][l + 1]), kW, cb, lane);
+    else if (NH > 1)
+      load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][NH - 1]), kW, cb, lane);
+    else if (cb < (Dp >> 5))
+      load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][0]), kW, cb, lane);
+    const float kun = kTanh / P.s_w[l];
+    float es = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 bq = *reinterpret_cast<const float4*>(&s_bias[l][f0 + 8 * q]);
+      const float bv[4] = {bq.x, bq.y, bq.z, bq.w};
+      float wq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, wdq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (l + 1 == NH) {
+        const float4 w4 = *reinterpret_cast<const float4*>(&s_wout[f0 + 8 * q]);
+        const float4 d4 = *reinterpret_cast<const float4*>(&s_wd[f0 + 8 * q]);
+        wq[0] = w4.x; wq[1] = w4.y; wq[2] = w4.z; wq[3] = w4.w;
+        wdq[0] = d4.x; wdq[1] = d4.y; wdq[2] = d4.z; wdq[3] = d4.w;
+      }
+      T o[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float y = fmaf(acc[4 * q + t], kun, bv[t]);  // k z
+        const float ex = __builtin_amdgcn_exp2f(-fabsf(y));  // exp(-2|z|)
+        const float r = __builtin_amdgcn_rcpf(1.0f + ex);
+        const float er = ex * r;
+        // sign of y by integer ops (r - er >= 0; a copysign here is moved past the 16-bit rounding
+        // below by the compiler, which then repacks the halves)
+        const float a = __uint_as_float(__float_as_uint(r - er) | (__float_as_uint(y) & 0x80000000u));
+        const float td = er * r;  // (1 - a^2) / 4
+        if (l + 1 < NH) {
+          o[t] = PR::cvt(a);
+          tv[l < NH - 1 ? l : 0][4 * q + t] = td;
+        } else {
+          // d_L = s_out w_out (1 - a_L^2) = 4 s_out w_out t', scaled by s_d[NH-1]; energy from the FP32 a_L
+          es += a * wq[t];
+          o[t] = PR::cvt(wdq[t] * td);
+        }
+      }
+      store4(dst + row * kLdW + f0 + 8 * q, o);
+    }
+    if (l + 1 == NH) {
+      es += __shfl_xor(es, 32, 64);  // lanes l and l + 32 hold the two halves of row l % 32
+      if (lane < 32) s_epart[cb][lane] = es;
+    }
+    __syncthreads();
+  }
+  if (tid < nrows) {
+    float s = 0.0f;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s += s_epart[w][tid];
+    e_atom[srow[tid]] = P.out_scale * (s + P.bout[e]);
+  }
+  // backward: d_{l-1} = (d_l W_l^T) (1 - a_{l-1}^2) = 4 (d_l W_l^T) t'_{l-1}
+#pragma unroll
+  for (int l = NH - 1; l >= 1; --l) {
+    const f32x16 acc = gemm_b<KIND>(act + l * 32 * kLdW, kLdW, b, kW, lane);
+    if (l > 1)
+      load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][l - 1]), kW, cb, lane);
+    else if (cb < (Dp >> 5))
+      load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][0]), kW, cb, lane);
+    const float c = 4.0f / (P.s_d[l] * P.s_w[l]) * P.s_d[l - 1];
+    T* prev = act + (l - 1) * 32 * kLdW + row * kLdW + f0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T o[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) o[t] = PR::cvt(acc[4 * q + t] * c * tv[l - 1][4 * q + t]);
+      store4(prev + 8 * q, o);
+    }
+    __syncthreads();
+  }
+  // dE/dG = (d_1 W_1^T) * scale: column block cb of the padded descriptor width
+  if (cb < (Dp >> 5)) {
+    const f32x16 acc = gemm_b<KIND>(act, kLdW, b, kW, lane);
+    const float un = 1.0f / (P.s_d[0] * P.s_w[0]);
+    if (row < nrows) {
+      float* out = dG + (size_t)srow[row] * D;
+      const float* scv = P.scale[e];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int f = f0 + 8 * q;
+        if (f < D) {  // D a multiple of 4: the 4 features are all in or all out
+          const float4 sc = *reinterpret_cast<const float4*>(scv + f);
+          *reinterpret_cast<float4*>(out + f) = make_float4(acc[4 * q] * sc.x * un, acc[4 * q + 1] * sc.y * un,
+                                                            acc[4 * q + 2] * sc.z * un, acc[4 * q + 3] * sc.w * un);
+        }
+      }
+    }
+  }
+}
+
 // rows of each element, wave-aggregated: one atomic per wave and element (a per-row atomic on
 // a handful of counters serialises)
 __global__ void __launch_bounds__(256) k_type_index(int n, int ntype, const int* typ, const int* off, int* fill,
@@ -466,6 +690,7 @@ struct MlpState {
   int nl = 0;
   std::vector<int> widths;
   size_t lds_bytes = 0;
+  bool uniform = false;  // k_mlp_w: 16-bit form, every hidden layer kMaxWidth wide, D <= kMaxWidth, D % 4 == 0
   int ntiles = 0;
   std::vector<void*> bufs;
   int* d_type_off = nullptr;
@@ -551,6 +776,8 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
     for (int s = 1; s <= d.nh; ++s) d.lo_off[s] = (s % 2) ? odd : 0;
   }
   S->lds_bytes = size_t(off) * esz;
+  S->uniform = (precision == 2 || precision == 3) && d.Dp <= kMaxWidth && d.D % 4 == 0;
+  for (int l = 0; l < d.nh; ++l) S->uniform = S->uniform && d.H[l] == kMaxWidth;
   if (S->lds_bytes > 64 * 1024) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "MLP tile does not fit the 64 KiB LDS budget");
   // weights: per element, per layer W [in][out] then b [out]
   size_t pofs = 0;
@@ -718,7 +945,19 @@ int mlp_enqueue(nnpe_ctx* ctx, bool ev) {
     case 4: NNPE_MLP(K, 4); break;            \
     default: NNPE_MLP(K, 5); break;           \
   }
-    if (S->kind == 0) { NNPE_MLP_K(0) }
+    if (S->uniform) {
+#define NNPE_MLP_W(K)                                                                                         \
+  switch (d.nh) {                                                                                             \
+    case 1: hipLaunchKernelGGL((k_mlp_w<K, 1>), dim3(S->ntiles), dim3(256), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx); break; \
+    case 2: hipLaunchKernelGGL((k_mlp_w<K, 2>), dim3(S->ntiles), dim3(256), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx); break; \
+    case 3: hipLaunchKernelGGL((k_mlp_w<K, 3>), dim3(S->ntiles), dim3(256), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx); break; \
+    case 4: hipLaunchKernelGGL((k_mlp_w<K, 4>), dim3(S->ntiles), dim3(256), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx); break; \
+    default: hipLaunchKernelGGL((k_mlp_w<K, 5>), dim3(S->ntiles), dim3(256), S->lds_bytes, ctx->stream, d, ctx->G, ctx->dG, ctx->e_atom, ctx->type_idx); break; \
+  }
+      if (S->kind == 3) { NNPE_MLP_W(3) }
+      else { NNPE_MLP_W(2) }
+#undef NNPE_MLP_W
+    } else if (S->kind == 0) { NNPE_MLP_K(0) }
     else if (S->kind == 1) { NNPE_MLP_K(1) }
     else if (S->kind == 3) { NNPE_MLP_K(3) }
     else { NNPE_MLP_K(2) }

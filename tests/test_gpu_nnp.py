@@ -37,9 +37,13 @@ def _setup(sysm, widths, seed=11, out_scale=0.5):
     return e, o, par, G64, norm
 
 
-@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3), ("bf16x1", 2e-2, 3e-2)])
-def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
-    widths = [80, 64, 64, 1]
+PRECS = [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3), ("bf16x1", 2e-2, 3e-2)]
+
+
+# hidden widths all 128: the uniform-width kernel of the 16-bit forms; 64: the general kernel
+@pytest.mark.parametrize("widths", [[80, 64, 64, 1], [80, 128, 128, 1]], ids=["w64", "w128"])
+@pytest.mark.parametrize("prec,tol_e,tol_g", PRECS)
+def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g, widths):
     e, o, par, G64, norm = _setup(water_c0, widths)
     e.sf_descriptor()
     G = e.nnp_get("G").astype(np.float64)
@@ -58,13 +62,14 @@ def test_mlp_matches_fp64(water_c0, prec, tol_e, tol_g):
     assert abs(en["nn"][0] - et) < max(tol_e, 1e-6) * (np.abs(ea64).sum() + 1e-9) * 10
 
 
-@pytest.mark.parametrize("prec,tol_e,tol_g", [("fp32", 1e-5, 1e-5), ("bf16", 2e-2, 3e-2), ("fp16", 5e-3, 5e-3), ("bf16x1", 2e-2, 3e-2)])
-def test_mlp_ragged_tiles_mixed_widths(prec, tol_e, tol_g):
-    """300 O and 600 H atoms: the last 32-row tile of each element is partial (12 and 24 rows),
-    and hidden widths 128, 96, 32 leave one, then three, of the four column blocks idle in the
-    forward and backward passes."""
+@pytest.mark.parametrize("widths", [[80, 128, 96, 32, 1], [80, 128, 128, 128, 1]], ids=["mixed", "w128x3"])
+@pytest.mark.parametrize("prec,tol_e,tol_g", PRECS)
+def test_mlp_ragged_tiles_mixed_widths(prec, tol_e, tol_g, widths):
+    """300 O and 600 H atoms: the last 32-row tile of each element is partial (12 and 24 rows);
+    hidden widths 128, 96, 32 leave one, then three, of the four column blocks idle in the
+    forward and backward passes, and 3 x 128 is the uniform-width kernel (16-bit forms) on
+    ragged tiles."""
     sysm = S.water_box(300, 21.0, seed=3)
-    widths = [80, 128, 96, 32, 1]
     e, o, par, G64, norm = _setup(sysm, widths, seed=5)
     e.sf_descriptor()
     G = e.nnp_get("G").astype(np.float64)
