@@ -319,6 +319,94 @@ def test_md_steps_match_fp32_oracle(water_c0):
     assert rel_rms(e.get("v"), o.get("v")) < 1e-4
 
 
+def _trajectory(e, o, nstep):
+    """The reference MD loop's call sequence (src/run.cpp:230-346) on the engine and the oracle
+    side by side; returns the number of rebuilds (equal on both sides)."""
+    nb = 0
+    for step in range(nstep + 1):
+        if step > 0:
+            e.update_x()
+            o.update_x()
+        vg, vo = e.isvalid(), o.isvalid()
+        assert vg == vo, f"step {step}: rebuild decision differs"
+        if not vg:
+            e.build(); o.rebuild(False)
+            nb += 1
+        else:
+            o.update_ghost_pos()
+        e.pair_compute(False); e.bonded_compute(False)
+        o.clear_f(); o.bonded_compute(False); o.pair_compute(False)
+        e.update_v(); o.update_v()
+    return nb
+
+
+def test_md_steps_fp64_form_match_fp64_oracle(water_c0):
+    """The FP64 integration form (positions and velocities in FP64) and the FP32 form against
+    the FP64 oracle's trajectory over 40 steps with rebuilds.  Both sides differ by the FP32
+    force evaluation on the engine (the chaotic growth of its ~1e-6 relative error gives ~1e-8
+    in x over 20 fs); the FP32 form adds FP32 rounding of x and v on every step and lands
+    several times further away."""
+    sysm = water_c0
+    ft, ut = S.lj_table(sysm, RC, 900, S.WATER_LJ)
+    errs = {}
+    for fp64 in (True, False):
+        e = _engine(sysm)
+        e.pair_table(RC, ft, ut)
+        e.bonded_set_from_system()
+        e.verlet_set(0.5, fp64=fp64)
+        o = _oracle(sysm, 1)
+        o.pair_set(RC, ft, ut)
+        o.set_dt(0.5)
+        nb = _trajectory(e, o, 40)
+        assert nb >= 3
+        errs[fp64] = (rel_rms(e.get("x"), o.get("x")), rel_rms(e.get("v"), o.get("v")))
+        e.close()
+    print(f"[md] 40 steps vs FP64 oracle: FP64 form x {errs[True][0]:.2e} v {errs[True][1]:.2e}; "
+          f"FP32 form x {errs[False][0]:.2e} v {errs[False][1]:.2e}", flush=True)
+    assert errs[True][0] < 2e-7 and errs[True][1] < 5e-5
+    assert errs[True][0] < 0.5 * errs[False][0]
+
+
+def test_fp64_verlet_is_the_fp64_restatement(water_c0):
+    """The FP64 form's update itself (src/verlet.cpp:29-97 restated in double) against the FP64
+    oracle's update given the same forces: every step the engine's FP32 forces, widened, are
+    the oracle's forces.  Masses 16 and 1 (their reciprocals exact in FP32) give both sides the
+    same 1/m, so the two integrations are the same double arithmetic: positions and velocities
+    agree to 1e-14 over 40 steps with list rebuilds and re-sorts on the engine."""
+    import dataclasses
+    sysm = dataclasses.replace(water_c0, mass=np.where(water_c0.typ == 0, 16.0, 1.0).astype(np.float32))
+    ft, ut = S.lj_table(sysm, RC, 900, S.WATER_LJ)
+    e = _engine(sysm)
+    e.pair_table(RC, ft, ut)
+    e.bonded_set_from_system()
+    e.verlet_set(0.5, fp64=True)
+    o = pyoracle.Oracle(sysm, 1)
+    o.set_dt(0.5)
+    o.set("x", e.get("x"))
+    o.set("v", e.get("v"))
+    o.set("f_prev", e.get("f_prev"))
+    worst, nb = [0.0, 0.0], 0
+    for step in range(41):
+        if step > 0:
+            e.update_x()
+            o.update_x()
+        if not e.isvalid():
+            e.build()
+            nb += 1
+        e.pair_compute(False)
+        e.bonded_compute(False)
+        o.set("f", e.get("f"))
+        e.update_v()
+        o.update_v()
+        for k, w in enumerate(("x", "v")):
+            a, b = e.get(w), o.get(w)
+            worst[k] = max(worst[k], float(np.abs(a - b).max() / np.abs(b).max()))
+    print(f"[md] FP64 update vs FP64 restatement, same forces: max rel x {worst[0]:.1e} v {worst[1]:.1e}; "
+          f"{nb} builds", flush=True)
+    assert nb >= 3
+    assert worst[0] <= 1e-14 and worst[1] <= 1e-14
+
+
 def test_gr_counts_bit_exact(water_c1):
     sysm = water_c1
     e = _engine(sysm, RC, 2.0, 12.0)
