@@ -194,6 +194,73 @@ def test_precision_forms_100_snapshots():
     assert max(errs["bf16x1"]) < 1e-2
 
 
+def _nnp_forces_fp64(sysm, widths, par, norm, sfp):
+    o = pyoracle.Oracle(sysm, 1)
+    o.neigh_init(RC_PAIR, 2.0, RC_PAIR)
+    o.rebuild(True)
+    o.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    G64 = o.sf_descriptor()
+    if norm is None:
+        norm = M.standardisation(G64, sysm.typ, sysm.ntype)
+    o.mlp_set(widths, par, 0.5, *norm)
+    _, _, dG64 = o.mlp_eval(G64)
+    o.clear_f()
+    o.sf_force(dG64)
+    return o.get("f"), norm
+
+
+def test_precision_forms_24k_snapshots():
+    """The same bars on the benchmark's own size and network: 24000-atom water (L = 62.1 A), the
+    3 x 128 tanh network, NNP forces of every precision form vs the FP64 oracle on 8 snapshots --
+    4 seeded lattice starts and 4 liquid configurations from 1 ps of MD (FP64 Verlet, classical
+    + FP32 NNP forces) of the first.  One standardisation (first snapshot's descriptors)."""
+    from nnpengine_amd.engine import Engine
+    widths = [80, 128, 128, 128, 1]
+    sfp = M.sf_params(RC_NNP)
+    par = M.mlp_init(widths, 2, 2024)
+    norm = None
+    snaps = [S.water_box(8000, 62.1, seed=7 + 100 * k) for k in range(4)]
+    # liquid snapshots: MD of the first start (pair table, bonded terms, the FP32 network)
+    f0, norm = _nnp_forces_fp64(snaps[0], widths, par, norm, sfp)
+    ft, ut = S.lj_table(snaps[0], RC_PAIR, 900, S.WATER_LJ)
+    md = Engine(snaps[0])
+    md.neigh_init(RC_PAIR, 2.0, RC_PAIR, RC_NNP)
+    md.pair_table(RC_PAIR, ft, ut)
+    md.bonded_set_from_system()
+    md.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    md.mlp_set(widths, par, 0.5, "fp32", *norm)
+    for _ in range(4):
+        md.md_run(500, dt=0.5, fp64=True, rebuild_every=10, pair=True, bonded=True, nnp=2, log_every=0)
+        liq = snaps[0].copy()
+        liq.x = np.ascontiguousarray(md.get("x"), np.float32)
+        snaps.append(liq)
+    md.close()
+    errs = {"fp32": [], "bf16": [], "fp16": [], "bf16x1": []}
+    for k, sysm in enumerate(snaps):
+        print(f"[snapshots24k] snapshot {k}", flush=True)
+        f64 = f0 if k == 0 else _nnp_forces_fp64(sysm, widths, par, norm, sfp)[0]
+        e = Engine(sysm)
+        e.neigh_init(RC_PAIR, 2.0, RC_PAIR, RC_NNP)
+        e.build()
+        e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+        for prec in errs:
+            e.mlp_set(widths, par, 0.5, prec, *norm)
+            e.sf_descriptor()
+            e.clear_force()
+            e.mlp_eval(False)
+            e.sf_force(False)
+            errs[prec].append(rel_rms(e.get("f"), f64))
+        e.close()
+    for prec, v in errs.items():
+        v = np.array(v)
+        print(f"[snapshots24k] {prec}: NNP force rel-rms vs fp64 over {len(v)} snapshots: "
+              + " ".join(f"{x:.2e}" for x in v), flush=True)
+    assert max(errs["fp32"]) < 1e-5
+    assert max(errs["bf16"]) < 1e-2
+    assert max(errs["fp16"]) < 5e-3
+    assert max(errs["bf16x1"]) < 1e-2
+
+
 def test_tbl_descriptor_and_force(water_c0):
     sysm = water_c0
     from nnpengine_amd.engine import Engine
