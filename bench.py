@@ -31,6 +31,7 @@ MFMA_PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # TFLOP/s dense
 # non-packed FP32 vector rate: 256 CUs x 64 lanes x 2 (FMA) x 2.4 GHz (the library builds without
 # packed FP32 instructions)
 VALU_PEAK_TFLOPS = 78.6
+CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 # algorithmic FP32 operations per directed pair within the symmetry-function cutoff
 SF_FORCE_FLOP_PAIR = 405   # geometry 16, cutoff 32, angular weights 21, radial polynomials 74,
                            # adjoint combination 60, angular gradient 4 x 48, force 10
@@ -309,7 +310,7 @@ def hbm_traffic(stage):
 HEAD = "bf16x1"
 
 STAGE_KERNEL = {"pair": "k_pair<false, 32>", "bonded": "k_bonded<false>", "sf_desc": "k_sf_desc<2, 16, true>",
-                "mlp": "k_mlp<3, 3>", "sf_adjoint": "k_sf_adjoint<2, true>",
+                "mlp": "k_mlp_w<3, 3>", "sf_adjoint": "k_sf_adjoint<2, true>",
                 "sf_force": "k_sf_force<2, false, 16, true, false, true>",
                 "verlet": "k_update_x64"}
 
@@ -327,6 +328,22 @@ def active_pairs(e, bl, rc):
     d = img - x[i]
     rsq = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
     return int(np.count_nonzero(rsq < np.float32(rc) * np.float32(rc)))
+
+
+def mfma_busy_cycles(kernel):
+    """SQ_VALU_MFMA_BUSY_CYCLES per launch of the kernel (summed over the SIMDs) from the
+    committed MFMA counter summary (the file states the command), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_mfma_water24k_mlp.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 9 and cells[0] == f"`{kernel}`":
+            try:
+                return float(cells[2]) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def valu_busy(stage):
@@ -369,11 +386,12 @@ def cpu_baseline_leg(sysm, par, norm, x_end, drift, do_cpu, do_err):
     return cpu, errs
 
 
-def refinement(steps, target_ps):
+def refinement(steps, target_ps, iters):
     """The butane refinement workload (4000 chains, 16 000 sites, Ryckaert-Bellemans torsion):
     the target S(q) from an MD run with LJ epsilon x 1.1 (target_ps of FP64-Verlet MD, classical
-    terms), then one refinement iteration of `steps` MD steps with the BF16 NNP on, g(r) + S(q) +
-    RMDF every 10 steps and the pair table rewritten every 100 steps, timed."""
+    terms), then `iters` refinement iterations of `steps` MD steps each with the BF16 NNP on,
+    g(r) + S(q) + RMDF every 10 steps and the pair table rewritten every 100 steps (each
+    iteration continues from the previous one's positions and refined table), each timed."""
     from nnpengine_amd import workloads as W
     c = W.BUTANE_C3
     t0 = time.perf_counter()
@@ -384,6 +402,7 @@ def refinement(steps, target_ps):
     target_steps -= target_steps % c["rebuild_every"]
     s_exp, _ = W.butane_target(sysm, target_steps)
     t_target = time.perf_counter() - t0
+    log(f"[bench] refinement target: {target_steps} steps in {t_target:.1f} s")
     ft, ut = W.butane_tables(sysm)
     e = W.classical_engine(sysm, ft, ut, nnp=True)
     W.nnp_attach(e, sysm, HEAD)
@@ -392,20 +411,27 @@ def refinement(steps, target_ps):
               gr_every=c["gr_every"], gr_long_every=c["gr_every"], sq=True, rmdf=True, log_every=100)
     e.md_run(200, **kw)  # warm-up: graphs captured, the refinement under way
     e.sync()
-    t0 = time.perf_counter()
-    lg = e.md_run(steps, **kw)
-    e.sync()
-    t = time.perf_counter() - t0
-    st = e.md_stats()
     from nnpengine_amd import _lib
     k = _lib.TERMS.index("rmdf")
-    res = lg[:, 2 * k]
+    it_s, res = [], []
+    for it in range(iters):
+        t0 = time.perf_counter()
+        lg = e.md_run(steps, **kw)
+        e.sync()
+        it_s.append(time.perf_counter() - t0)
+        if len(lg):
+            res.append((float(lg[0, 2 * k]), float(lg[-1, 2 * k])))
+    st = e.md_stats()
     e.close()
+    t = float(sum(it_s))
     return {"workload": f"butane refinement: {sysm.natom} sites ({sysm.nmol} chains), L={sysm.bl:.1f} A, RB torsion, "
                         f"BF16 NNP ({HEAD}), g(r) + S(q) + RMDF every 10 steps, table rewrite every 100 steps",
-            "steps": steps, "seconds": t, "ms_per_step": t / steps * 1e3, "atom_steps_per_s": sysm.natom * steps / t,
+            "iterations": iters, "steps_per_iteration": steps, "seconds": t,
+            "iteration_seconds_min_max": [min(it_s), max(it_s)] if it_s else None,
+            "ms_per_step": t / max(1, iters * steps) * 1e3, "atom_steps_per_s": sysm.natom * iters * steps / t if t else None,
             "target_ps": target_steps * c["dt"] / 1000, "target_seconds": t_target, "build_seconds": t_build,
-            "residual_first_last": [float(res[0]), float(res[-1])] if len(res) else None,
+            "rmdf_residual_first_iteration": list(res[0]) if res else None,
+            "rmdf_residual_last_iteration": list(res[-1]) if res else None,
             "list_builds": int(st["nbuild"]), "skin_violations": int(st["skin_violations"])}
 
 
@@ -472,8 +498,9 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--no-sqd", action="store_true")
     ap.add_argument("--drift-ps", type=float, default=20.0, help="NVE run per precision form for the drift (0: skip)")
-    ap.add_argument("--refine-steps", type=int, default=2000, help="steps of the timed butane refinement iteration (0: skip)")
-    ap.add_argument("--refine-target-ps", type=float, default=20.0, help="MD run that makes the refinement target S(q)")
+    ap.add_argument("--refine-steps", type=int, default=2000, help="steps per butane refinement iteration (0: skip)")
+    ap.add_argument("--refine-iters", type=int, default=50, help="refinement iterations")
+    ap.add_argument("--refine-target-ps", type=float, default=200.0, help="MD run that makes the refinement target S(q)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a sharded run is not "
@@ -577,6 +604,12 @@ def main():
                 "flop_per_atom": flop_atom}
     mlp_roof["frac_mfma_bf16"] = mlp_roof["achieved_tflops_bf16"] / MFMA_PEAK["bf16"]
     mlp_roof["frac_mfma_fp32"] = mlp_roof["achieved_tflops_fp32"] / MFMA_PEAK["fp32"]
+    # MFMA busy share by counter: the per-launch SQ_VALU_MFMA_BUSY_CYCLES of the committed MFMA
+    # pass over the SIMD cycles of this run's stage time
+    busy = {"bf16": mfma_busy_cycles(STAGE_KERNEL["mlp"]), "fp32": mfma_busy_cycles("k_mlp<0, 3>")}
+    mlp_roof["mfma_busy_share"] = {
+        "bf16": busy["bf16"] / (1024 * stages["mlp"] * 1e-3 * CLOCK_HZ) if busy["bf16"] else None,
+        "fp32": busy["fp32"] / (1024 * ms32_mlp * 1e-3 * CLOCK_HZ) if busy["fp32"] else None}
     # every stage against the HBM roofline (algorithmic bytes / live stage time), with the HBM
     # traffic and VALU busy share of its kernel from the committed counter summaries where the
     # stage is one kernel (the rebuild is a dozen launches: its time and bytes cover them all)
@@ -605,7 +638,7 @@ def main():
             log(f"[bench] NVE {prec}: {drift[prec]}")
     refine = None
     if args.refine_steps > 0:
-        refine = refinement(args.refine_steps, args.refine_target_ps)
+        refine = refinement(args.refine_steps, args.refine_target_ps, args.refine_iters)
         log("[bench] refinement:", refine)
     cpu, errs = cpu_baseline_leg(sysm, par, norm, x_end, drift, not args.no_cpu, not args.no_err)
 
