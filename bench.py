@@ -32,6 +32,8 @@ MFMA_PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # TFLOP/s dense
 # packed FP32 instructions)
 VALU_PEAK_TFLOPS = 78.6
 CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+COPY_PROBE_BYTES = 64 << 20  # the copy stage: 64 MiB read and 64 MiB written
+MFMA_PROBE_FLOP = 2048 * 2048 * 4 * 2 * 32 * 32 * 16  # the mfma stage: waves x rounds x chains x flop per MFMA
 # algorithmic FP32 operations per directed pair within the symmetry-function cutoff
 SF_FORCE_FLOP_PAIR = 405   # geometry 16, cutoff 32, angular weights 21, radial polynomials 74,
                            # adjoint combination 60, angular gradient 4 x 48, force 10
@@ -542,8 +544,15 @@ def main():
     # per-stage profile (HIP events on the engine stream) of the BF16 form
     stages = {}
     for st in ("pair", "bonded", "sf_desc", "mlp", "sf_adjoint", "sf_force", "rebuild", "gr_full", "rebuild_gr", "verlet", "sq",
-               "copy"):
+               "copy", "mfma"):
         stages[st] = eb.profile(st, reps=args.stage_reps)
+    # the peaks measured on this GPU beside the published ones: a 64 MiB streaming copy (read +
+    # write) and an MFMA issue-rate probe (register operands, bf16 32x32x16)
+    measured = {"hbm_copy_gbs": 2 * COPY_PROBE_BYTES / (stages["copy"] * 1e-3) / 1e9,
+                "mfma_bf16_tflops": MFMA_PROBE_FLOP / (stages["mfma"] * 1e-3) / 1e12}
+    measured["hbm_copy_frac_of_published"] = measured["hbm_copy_gbs"] / HBM_PEAK_GBS
+    measured["mfma_bf16_frac_of_published"] = measured["mfma_bf16_tflops"] / MFMA_PEAK["bf16"]
+    log("[bench] measured peaks:", measured)
     ms32_mlp = engines["fp32"].profile("mlp", reps=args.stage_reps)
     ms_split_mlp = engines["bf16"].profile("mlp", reps=args.stage_reps)
     log("[bench] stage ms:", {k: round(v, 4) for k, v in stages.items()}, "fp32 mlp", round(ms32_mlp, 4))
@@ -680,6 +689,7 @@ def main():
         "mlp": mlp_roof,
         "roofline": roof,
         "stage_roofline": stage_roof,
+        "measured_peaks": measured,
         "sqd": sqd,
         "refinement": refine,
         "cpu_baseline": cpu,

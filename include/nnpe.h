@@ -296,7 +296,9 @@ enum nnpe_stage { NNPE_STAGE_PAIR = 0, NNPE_STAGE_BONDED = 1, NNPE_STAGE_SF_DESC
                   NNPE_STAGE_GR_FULL = 7, NNPE_STAGE_GR_LIST = 8, NNPE_STAGE_VERLET = 9, NNPE_STAGE_SQ = 10,
                   NNPE_STAGE_TBL_DESC = 11, NNPE_STAGE_TBL_FORCE = 12,
                   NNPE_STAGE_COPY = 13 /* 64 MiB read + 64 MiB write streaming copy */,
-                  NNPE_STAGE_REBUILD_GR = 14 /* rebuild with the full g(r) tally folded in */ };
+                  NNPE_STAGE_REBUILD_GR = 14 /* rebuild with the full g(r) tally folded in */,
+                  NNPE_STAGE_MFMA = 15 /* MFMA issue-rate probe: 2048 waves x 2048 rounds x 4
+                                          v_mfma_f32_32x32x16_bf16 = 5.50e11 flop */ };
 int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* avg_ms);
 
 #ifdef __cplusplus

@@ -86,7 +86,7 @@ SIGNATURES = [
 
 STAGES = {"pair": 0, "bonded": 1, "sf_desc": 2, "mlp": 3, "sf_adjoint": 4, "sf_force": 5, "rebuild": 6,
           "gr_full": 7, "gr_list": 8, "verlet": 9, "sq": 10, "tbl_desc": 11, "tbl_force": 12,
-          "copy": 13, "rebuild_gr": 14}
+          "copy": 13, "rebuild_gr": 14, "mfma": 15}
 
 
 class MdParams(C.Structure):

@@ -78,6 +78,36 @@ __global__ void __launch_bounds__(256) k_copy_probe(int n, const float4* __restr
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
 }
 
+// MFMA issue-rate probe: every wave runs kMfmaProbeIters rounds of four independent
+// v_mfma_f32_32x32x16_bf16 chains on register operands, then stores one value per lane so the
+// chains stay live.  kMfmaProbeBlocks workgroups of 4 waves: two waves per SIMD on 256 CUs.
+// Flops per launch: 4 kMfmaProbeBlocks x kMfmaProbeIters x 4 x 2 x 32 x 32 x 16.
+constexpr int kMfmaProbeIters = 2048;
+constexpr int kMfmaProbeBlocks = 512;
+__global__ void __launch_bounds__(256) k_mfma_probe(float* __restrict__ out, float seed) {
+  typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+  typedef float f32x16 __attribute__((ext_vector_type(16)));
+  bf16x8 a, b;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = (__bf16)(seed * float((threadIdx.x + k) & 15));
+    b[k] = (__bf16)(seed * float((k + 3 * threadIdx.x) & 15));
+  }
+  f32x16 c0, c1, c2, c3;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) c0[j] = c1[j] = c2[j] = c3[j] = 0.0f;
+  for (int it = 0; it < kMfmaProbeIters; ++it) {
+    c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, a, c2, 0, 0, 0);
+    c3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, b, c3, 0, 0, 0);
+  }
+  float sum = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) sum += (c0[j] + c1[j]) + (c2[j] + c3[j]);
+  out[blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
 MdGraphs* graphs_of(nnpe_ctx* ctx) {
   if (!ctx->md) ctx->md = new MdGraphs;
   return static_cast<MdGraphs*>(ctx->md);
@@ -293,7 +323,8 @@ extern "C" int nnpe_md_stats(nnpe_ctx* ctx, long long* out) {
 // accumulate (forces += ...) are timed as launched, their results are not meant to be used.
 extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* avg_ms) {
   if (!ctx || reps <= 0 || !avg_ms) return NNPE_ERR_ARG;
-  if (!ctx->have_list && stage != NNPE_STAGE_COPY) return ctx_fail(ctx, NNPE_ERR_STATE, "profile needs a built list");
+  if (!ctx->have_list && stage != NNPE_STAGE_COPY && stage != NNPE_STAGE_MFMA)
+    return ctx_fail(ctx, NNPE_ERR_STATE, "profile needs a built list");
   float4* probe = nullptr;
   auto launch = [&](int st) -> int {
     switch (st) {
@@ -321,6 +352,13 @@ extern "C" int nnpe_profile_stage(nnpe_ctx* ctx, int stage, int reps, double* av
           if (hipMemsetAsync(probe, 0, 2 * size_t(kCopyProbeBytes), ctx->stream) != hipSuccess) return NNPE_ERR_HIP;
         }
         hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, ctx->stream, n, probe, probe + n);
+        return hipGetLastError() == hipSuccess ? NNPE_OK : NNPE_ERR_HIP;
+      }
+      case NNPE_STAGE_MFMA: {
+        static_assert(size_t(kMfmaProbeBlocks) * 256 * sizeof(float) <= 2 * size_t(kCopyProbeBytes), "probe output");
+        if (!probe && hipMalloc(reinterpret_cast<void**>(&probe), 2 * size_t(kCopyProbeBytes)) != hipSuccess) return NNPE_ERR_HIP;
+        hipLaunchKernelGGL(k_mfma_probe, dim3(kMfmaProbeBlocks), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(probe),
+                           1.0f / 64.0f);
         return hipGetLastError() == hipSuccess ? NNPE_OK : NNPE_ERR_HIP;
       }
       default: return NNPE_ERR_ARG;
