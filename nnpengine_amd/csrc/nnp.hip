@@ -442,82 +442,94 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   }
 }
 
-// dE/d(moments) of each atom from dE/dG and its moments.  Lane (atom i, element a, exponent m)
-// forms the 11 adjoints of moment group (a, m) -- summing the partner elements b in order, each
-// pair's terms in the order of the pair loop (x <= y) of the descriptor definition, the
-// self pair twice -- and copies four of element a's sixteen radial dE/dG into the slice.
+// dE/d(moments) of each atom from dE/dG and its moments.  A workgroup takes 32 atoms; lane
+// (atom, element a, exponent m) forms the 11 adjoints of moment group (a, m) -- summing the
+// partner elements b in order, each pair's terms in the order of the pair loop (x <= y) of the
+// descriptor definition, the self pair twice -- and copies four of element a's sixteen radial
+// dE/dG into the slice.  The slices are assembled in LDS and stored chunk by chunk with the atom
+// index fastest across the lanes, so each store instruction writes whole runs of the
+// structure-of-arrays chunk rows (instead of 4-byte pieces of 16-byte chunks).
 template <int NT, bool HALF>
-__global__ void __launch_bounds__(256) k_sf_adjoint(int n, int stride, const float* __restrict__ M,
-                                                    const float* __restrict__ dG, float* __restrict__ adj,
-                                                    uint4* __restrict__ adjh) {
+__global__ void __launch_bounds__(32 * kNEta * NT) k_sf_adjoint(int n, int stride, const float* __restrict__ M,
+                                                               const float* __restrict__ dG, float* __restrict__ adj,
+                                                               uint4* __restrict__ adjh) {
 #pragma clang fp contract(fast)
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int D = NT * kNRad + NT * (NT + 1) / 2 * kNEta * 4;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * NT * kNEta) return;
-  const int m = t % kNEta, a = (t / kNEta) % NT, i = t / (kNEta * NT);
-  const float* mi = M + (size_t)i * NMT;
-  const float* g = dG + (size_t)i * D;
-  float dA[kNMom];
-#pragma unroll
-  for (int c = 0; c < kNMom; ++c) dA[c] = 0.0f;
-#pragma unroll
-  for (int b = 0; b < NT; ++b) {
-    const int x = min(a, b), y = max(a, b);
-    const int pp = x * NT - (x * (x - 1)) / 2 + (y - x);  // index of the pair (x, y), x <= y
-    const float* gg = g + NT * kNRad + (pp * kNEta + m) * 4;
-    const float u0 = gg[0] + gg[1] + 0.5f * (gg[2] + gg[3]);
-    const float u1 = gg[0] - gg[1] + gg[2] - gg[3];
-    const float u2 = 0.5f * (gg[2] + gg[3]);
-    const float* Z = mi + (b * kNEta + m) * kNMom;  // the partner's moments
-    for (int rep = 0; rep < (a == b ? 2 : 1); ++rep) {
-      dA[0] += u0 * Z[0];
-#pragma unroll
-      for (int c = 1; c <= 3; ++c) dA[c] += u1 * Z[c];
-#pragma unroll
-      for (int c = 4; c <= 6; ++c) dA[c] += u2 * Z[c];
-#pragma unroll
-      for (int c = 7; c <= 9; ++c) dA[c] += 2.0f * u2 * Z[c];
-    }
-    if (a == b) dA[10] -= u0 + u1 + u2;
-  }
-  // The slice stores the off-diagonal second-moment adjoints (components 7..9, each the adjoint
-  // of a product that appears twice in the symmetric matrix) halved, so the force pass forms
-  // M h with plain FMAs.
-#pragma unroll
-  for (int c = 7; c <= 9; ++c) dA[c] *= 0.5f;
-  // slice float f of element a of row i at adj[((a * 16 + f / 4) * stride + i) * 4 + f % 4]
-  auto at = [&](int f) -> float& { return adj[((size_t)(a * (kSlice / 4) + (f >> 2)) * stride + i) * 4 + (f & 3)]; };
-#pragma unroll
-  for (int c = 0; c < kNMom; ++c) at(m * kNMom + c) = dA[c];
+  constexpr int NTH = 32 * kNEta * NT;
+  constexpr int kRow = kSlice + 4;  // padded LDS row: the store phase's lanes read different banks
+  __shared__ __align__(16) float s32[32 * NT][kRow];
+  __shared__ __align__(16) _Float16 s16[HALF ? 32 * NT : 1][kSlice + 8];
+  const int tid = threadIdx.x;
+  const int m = tid % kNEta, a = (tid / kNEta) % NT, il = tid / (kNEta * NT);
+  const int i0 = blockIdx.x * 32, i = i0 + il;
   static_assert(kSliceRad % 4 == 0 && kNRad == 4 * kNEta, "one radial chunk per exponent lane");
-  const float* gr = g + a * kNRad + 4 * m;
-  reinterpret_cast<float4*>(adj)[(size_t)(a * (kSlice / 4) + kSliceRad / 4 + m) * stride + i] =
-      make_float4(gr[0], gr[1], gr[2], gr[3]);
-  if (m == 0)
-    reinterpret_cast<float4*>(adj)[(size_t)(a * (kSlice / 4) + (kSliceRad + kNRad) / 4) * stride + i] =
-        make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if constexpr (HALF) {
-    // slice value f of element a of row i at half f % 8 of adjh[(a * 8 + f / 8) * stride + i]:
-    // the four lanes of (i, a) -- one wave -- assemble the slice in LDS, then each stores two
-    // whole 16-byte chunks
-    static_assert(kSlice / 8 == 2 * kNEta, "two 8-half chunks per exponent lane");
-    __shared__ __align__(16) _Float16 s_h[256 / kNEta][kSlice];
-    _Float16* sh = s_h[threadIdx.x / kNEta];
-    // saturate instead of rounding to infinity (|adjoint| beyond 65504 would be a broken network)
-    auto h16 = [](float x) { return (_Float16)fminf(fmaxf(x, -65504.0f), 65504.0f); };
+  if (i < n) {
+    const float* mi = M + (size_t)i * NMT;
+    const float* g = dG + (size_t)i * D;
+    float dA[kNMom];
 #pragma unroll
-    for (int c = 0; c < kNMom; ++c) sh[m * kNMom + c] = h16(dA[c]);
+    for (int c = 0; c < kNMom; ++c) dA[c] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) sh[kSliceRad + 4 * m + k] = h16(gr[k]);
+    for (int b = 0; b < NT; ++b) {
+      const int x = min(a, b), y = max(a, b);
+      const int pp = x * NT - (x * (x - 1)) / 2 + (y - x);  // index of the pair (x, y), x <= y
+      const float* gg = g + NT * kNRad + (pp * kNEta + m) * 4;
+      const float u0 = gg[0] + gg[1] + 0.5f * (gg[2] + gg[3]);
+      const float u1 = gg[0] - gg[1] + gg[2] - gg[3];
+      const float u2 = 0.5f * (gg[2] + gg[3]);
+      const float* Z = mi + (b * kNEta + m) * kNMom;  // the partner's moments
+      for (int rep = 0; rep < (a == b ? 2 : 1); ++rep) {
+        dA[0] += u0 * Z[0];
+#pragma unroll
+        for (int c = 1; c <= 3; ++c) dA[c] += u1 * Z[c];
+#pragma unroll
+        for (int c = 4; c <= 6; ++c) dA[c] += u2 * Z[c];
+#pragma unroll
+        for (int c = 7; c <= 9; ++c) dA[c] += 2.0f * u2 * Z[c];
+      }
+      if (a == b) dA[10] -= u0 + u1 + u2;
+    }
+    // The slice stores the off-diagonal second-moment adjoints (components 7..9, each the
+    // adjoint of a product that appears twice in the symmetric matrix) halved, so the force pass
+    // forms M h with plain FMAs.
+#pragma unroll
+    for (int c = 7; c <= 9; ++c) dA[c] *= 0.5f;
+    const float* gr = g + a * kNRad + 4 * m;
+    float* sl = s32[il * NT + a];
+#pragma unroll
+    for (int c = 0; c < kNMom; ++c) sl[m * kNMom + c] = dA[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sl[kSliceRad + 4 * m + k] = gr[k];
     if (m == 0)
 #pragma unroll
-      for (int k = kSliceRad + kNRad; k < kSlice; ++k) sh[k] = (_Float16)0.0f;
-    __builtin_amdgcn_wave_barrier();  // the group lives in one wave: its LDS writes are ordered
+      for (int k = kSliceRad + kNRad; k < kSlice; ++k) sl[k] = 0.0f;
+    if constexpr (HALF) {
+      // saturate instead of rounding to infinity (|adjoint| beyond 65504 would be a broken network)
+      auto h16 = [](float x) { return (_Float16)fminf(fmaxf(x, -65504.0f), 65504.0f); };
+      _Float16* sh = s16[il * NT + a];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int q = 2 * m + h;
-      adjh[(size_t)(a * (kSlice / 8) + q) * stride + i] = *reinterpret_cast<const uint4*>(sh + 8 * q);
+      for (int c = 0; c < kNMom; ++c) sh[m * kNMom + c] = h16(dA[c]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sh[kSliceRad + 4 * m + k] = h16(gr[k]);
+      if (m == 0)
+#pragma unroll
+        for (int k = kSliceRad + kNRad; k < kSlice; ++k) sh[k] = (_Float16)0.0f;
+    }
+  }
+  __syncthreads();
+  // chunk q of element e of row j at adj4[(e * 16 + q) * stride + j] and (FP16, 8 values per
+  // chunk) adjh[(e * 8 + q) * stride + j]
+  const int nat = min(32, n - i0);
+  float4* adj4 = reinterpret_cast<float4*>(adj);
+  for (int idx = tid; idx < 32 * NT * (kSlice / 4); idx += NTH) {
+    const int j = idx & 31, rest = idx >> 5, q = rest % (kSlice / 4), e = rest / (kSlice / 4);
+    if (j < nat) adj4[(size_t)(e * (kSlice / 4) + q) * stride + i0 + j] = reinterpret_cast<const float4*>(s32[j * NT + e])[q];
+  }
+  if constexpr (HALF) {
+    for (int idx = tid; idx < 32 * NT * (kSlice / 8); idx += NTH) {
+      const int j = idx & 31, rest = idx >> 5, q = rest % (kSlice / 8), e = rest / (kSlice / 8);
+      if (j < nat) adjh[(size_t)(e * (kSlice / 8) + q) * stride + i0 + j] = reinterpret_cast<const uint4*>(s16[j * NT + e])[q];
     }
   }
 }
@@ -979,11 +991,11 @@ static bool sf_half_gather(nnpe_ctx* ctx) { return ctx->dG_kind != 0; }
 
 int sf_adjoint_enqueue(nnpe_ctx* ctx) {
   NnpState& S = nnp_state(ctx);
-  const int nblk = blocks_for((long long)ctx->natom * ctx->ntype * kNEta, 256);
+  const int nblk = blocks_for(ctx->natom, 32);  // 32 atoms per workgroup
   const bool half = sf_half_gather(ctx);
-#define NNPE_SF_ADJ1(NT, H)                                                                                   \
-  hipLaunchKernelGGL((k_sf_adjoint<NT, H>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, S.M, \
-                     nnp_dG(ctx), S.adj, S.adjh)
+#define NNPE_SF_ADJ1(NT, H)                                                                                         \
+  hipLaunchKernelGGL((k_sf_adjoint<NT, H>), dim3(nblk), dim3(32 * kNEta * NT), 0, ctx->stream, ctx->natom, ctx->stride, \
+                     S.M, nnp_dG(ctx), S.adj, S.adjh)
 #define NNPE_SF_ADJ(NT) if (half) NNPE_SF_ADJ1(NT, true); else NNPE_SF_ADJ1(NT, false)
   switch (ctx->ntype) {
     case 1: NNPE_SF_ADJ(1); break;
