@@ -38,7 +38,9 @@ __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
   return __mul24(i, ntype) - (__mul24(i, i - 1) >> 1) + (j - i);
 }
 
-template <bool EV, int G>
+// TYE: the neighbour's element comes from its list entry (at most NNPE_ENTRY_MAX_TYPES
+// elements) instead of a second gather, typ[j], per entry
+template <bool EV, int G, bool TYE>
 __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __restrict__ nbr,
                                               const int* __restrict__ cnt, const float4* __restrict__ xq,
                                               const int* __restrict__ typ, const BoxPar* __restrict__ boxes,
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
       for (int u = 0; u < kPairU; ++u) {
         const int j = k0 + u * G < m ? (int)NNPE_ENTRY_J(ec[u]) : i;
         xr[u] = xq[j];
-        tr[u] = typ[j];
+        tr[u] = TYE ? NNPE_ENTRY_T(ec[u]) : typ[j];
       }
 #pragma unroll
       for (int u = 0; u < kPairU; ++u) {
@@ -340,14 +342,17 @@ int pair_enqueue(nnpe_ctx* ctx, bool ev) {
   const size_t lds = size_t(ctx->npair) * nb1 * sizeof(float);
   double* pe = ev_part(ctx, NNPE_TERM_PAIR, 0);
   double* pw = ev_part(ctx, NNPE_TERM_PAIR, 1);
-  if (ev)
-    hipLaunchKernelGGL((k_pair<true, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
-                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, ctx->ntype, ctx->npair, ctx->pt_cutsq,
-                       ctx->pt_rdr, ctx->pt_fc, P64, nb1, ctx->pt_tab, ctx->f, pe, pw);
-  else
-    hipLaunchKernelGGL((k_pair<false, kPairG>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long,
-                       ctx->cnt_long, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, ctx->ntype, ctx->npair, ctx->pt_cutsq,
-                       ctx->pt_rdr, ctx->pt_fc, P64, nb1, ctx->pt_tab, ctx->f, pe, pw);
+#define NNPE_PAIR(EV, TYE)                                                                                           \
+  hipLaunchKernelGGL((k_pair<EV, kPairG, TYE>), dim3(nblk), dim3(256), lds, ctx->stream, N, ctx->cap_long, ctx->nbr_long, \
+                     ctx->cnt_long, ctx->xq, ctx->typ, ctx->d_box, ctx->rbox, ctx->ntype, ctx->npair, ctx->pt_cutsq,         \
+                     ctx->pt_rdr, ctx->pt_fc, P64, nb1, ctx->pt_tab, ctx->f, pe, pw)
+  const bool tye = ctx->ntype <= NNPE_ENTRY_MAX_TYPES;
+  if (ev) {
+    if (tye) NNPE_PAIR(true, true); else NNPE_PAIR(true, false);
+  } else {
+    if (tye) NNPE_PAIR(false, true); else NNPE_PAIR(false, false);
+  }
+#undef NNPE_PAIR
   NNPE_CHECK_LAUNCH(ctx);
   if (ev) return ev_finish(ctx, NNPE_TERM_PAIR, nblk);
   return NNPE_OK;

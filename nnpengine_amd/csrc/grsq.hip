@@ -22,8 +22,6 @@ constexpr int kHistLdsMax = 48 * 1024;
 constexpr int kHistLdsFused = 16 * 1024;  // histogram added to the rebuild kernel's LDS
 constexpr int kGrG = 16;  // lanes per atom, inner tally
 
-constexpr int kGrShellCopies = 32;
-
 __device__ __forceinline__ int npair_of(int ntype) { return ntype * (ntype + 1) / 2; }
 
 __device__ __forceinline__ int pair_index(int a, int b, int ntype) {
@@ -71,7 +69,7 @@ __global__ void __launch_bounds__(256) k_gr_inner(int n, int cap, const uint32_t
       if (!(rsq < rmaxsq)) continue;
       const int b = gr_bin(rsq, rdr);
       if (b >= nbin_inner) continue;
-      const int p = pair_index(ti, typ[j], ntype);
+      const int p = pair_index(ti, ntype <= NNPE_ENTRY_MAX_TYPES ? NNPE_ENTRY_T(e) : typ[j], ntype);
       if (kLds) atomicAdd(&s_hist[p * nbin_inner + b], 1u);
       else atomicAdd(&count[(size_t)p * nbin + b], 1ull);
     }
@@ -199,8 +197,8 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
 // atom, or that hold no candidate, are dropped; the wave then takes its remaining items one at a
 // time, its 64 lanes walking the item cell's real rows above the atom (weight 2) and its ghost
 // images (weight 1) -- 2 per unordered pair, the reference's rule.  Counts go to
-// kGrShellCopies private copies (the few bins near r_max take every add), folded into the tally
-// by k_gr_fold_shell.
+// kGrShellCopies private copies (the few bins near r_max take every add), the same copies the
+// rebuild kernel added its workgroup histograms to, folded into the tally by k_gr_fold_shell.
 // Positions and cells are those of the rebuild that just ran, so the cell boxes bound the atoms.
 __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* __restrict__ outer,
                                                   const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
@@ -266,7 +264,7 @@ __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* _
   }
 }
 
-// count += the shell copies (integer sums: order-free), then h(r) as k_gr_reduce
+// count += the copies (integer sums: order-free), then h(r) as k_gr_reduce
 __global__ void k_gr_fold_shell(int total, unsigned long long* count, const unsigned long long* copies, const double* norm,
                                 double* h) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

@@ -9,8 +9,8 @@
 //    its position is xq[src] + image shift computed in FP32 exactly as the reference's
 //    x_ghost = x + (+-L) (src/environ.cpp:431-433, :505-507);
 //  * neighbour lists are padded rows: entry k of atom i at nbr[i * cap + k], read by a group of
-//    lanes per atom (consecutive words); an entry packs (j row : 25 bits, image code : 5 bits,
-//    special class : 2 bits).
+//    lanes per atom (consecutive words); an entry packs (j row : 23 bits, element of j : 2 bits,
+//    image code : 5 bits, special class : 2 bits).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,11 +21,17 @@
 
 #include "../../include/nnpe.h"
 
-#define NNPE_ENTRY_J(e) ((int)((e) & 0x1FFFFFFu))
+// neighbour-list entry: row j (23 bits) | element of j (2 bits) | image code (5) | special class (2)
+#define NNPE_ENTRY_J(e) ((int)((e) & 0x7FFFFFu))
+#define NNPE_ENTRY_T(e) ((int)(((e) >> 23) & 3u))
 #define NNPE_ENTRY_CODE(e) ((int)(((e) >> 25) & 31u))
 #define NNPE_ENTRY_SP(e) ((int)((e) >> 30))
 #define NNPE_MAKE_ENTRY(j, code, sp) ((uint32_t)(j) | ((uint32_t)(code) << 25) | ((uint32_t)(sp) << 30))
-#define NNPE_MAX_ATOMS (1 << 25)
+#define NNPE_MAKE_ENTRY_T(j, t, code, sp) (NNPE_MAKE_ENTRY(j, code, sp) | ((uint32_t)(t) << 23))
+#define NNPE_MAX_ATOMS (1 << 23)
+// entries carry the neighbour's element when there are at most this many elements (else 0, and
+// the kernels that need it read typ[j])
+#define NNPE_ENTRY_MAX_TYPES 4
 
 // device counters, one small block read back by the host when needed
 struct DevCounters {
@@ -193,6 +199,9 @@ int gr_enqueue(nnpe_ctx* ctx, bool full);
 // full g(r) tally folded into a list rebuild (neigh.hip, grsq.hip): the rebuild counts the pairs
 // of its candidates, the outer cells follow in a shell pass
 bool gr_fusable(nnpe_ctx* ctx);
+// private copies of the full tally's counts that a tally folded into a rebuild adds into (the
+// rebuild kernel's workgroup histograms and the shell pass), summed by the fold kernel
+constexpr int kGrShellCopies = 32;
 int gr_shell_size(nnpe_ctx* ctx);  // entries of the shell copies (cleared by the rebuild's reset)
 int gr_outer_refresh(nnpe_ctx* ctx);  // device outer stencil for the current grid and g(r) range
 int gr_shell_finish_enqueue(nnpe_ctx* ctx);

@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
                                                      uint32_t* __restrict__ nbr_s, int* __restrict__ cnt_s_out,
                                                      float4* __restrict__ xprev, DevCounters* counters,
                                                      float gr_rmaxsq, float gr_rdr, int gr_nbin, int npair,
-                                                     unsigned long long* __restrict__ gr_count) {
+                                                     unsigned long long* __restrict__ gr_copies) {
   // kGr: the full g(r) tally of the rebuild's positions rides along -- a real candidate pair
   // within r_max adds 2 from its lower row, a ghost image 1 (2 per unordered pair, the
   // reference's rule), into the workgroup's LDS histogram
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
   __shared__ int r_begin[kBuildMaxRanges], r_pref[kBuildMaxRanges + 1];
   __shared__ unsigned char r_flag[kBuildMaxRanges];  // bit 0: ghost, bit 1: no distance check
   __shared__ float4 c_pos[kBuildBatch];              // x, y, z, molecule id (int bits)
-  __shared__ uint32_t c_ent[kBuildBatch];            // row | code << 25 | nocheck << 31
+  __shared__ uint32_t c_ent[kBuildBatch];            // entry (special class 0) | nocheck << 31
   __shared__ uint32_t c_aux[kBuildBatch];            // intra id | element << 24
   __shared__ int h_nl[kBuildHome], h_ns[kBuildHome][kShortMaxType];
   __shared__ int s_wsum[kBuildThreads / 64 + 1];
@@ -469,9 +469,10 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
         if (fl & 1) { row = gs_src[pos]; code = gs_code[pos]; p = image_pos(xq[row], code, bl, m_bl); }
         else { row = pos; code = 13; p = xq[pos]; }
         const int4 in = info[row];
+        const int tr = typ[row];
         c_pos[u] = make_float4(p.x, p.y, p.z, __int_as_float(in.y));
-        c_ent[u] = NNPE_MAKE_ENTRY(row, code, 0) | ((fl & 2) ? 0x80000000u : 0u);
-        c_aux[u] = (uint32_t)in.z | ((uint32_t)typ[row] << 24);
+        c_ent[u] = NNPE_MAKE_ENTRY_T(row, ntype <= NNPE_ENTRY_MAX_TYPES ? tr : 0, code, 0) | ((fl & 2) ? 0x80000000u : 0u);
+        c_aux[u] = (uint32_t)in.z | ((uint32_t)tr << 24);
       }
       __syncthreads();
       for (int h = h0 + wave; h < h1; h += kBuildThreads / 64) {
@@ -527,10 +528,14 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
     if (kShort) atomicMax(&counters->max_short, s_max[1]);
   }
   if (kGr) {
+    // the workgroup histogram goes into one of kGrShellCopies private copies of the counts (the
+    // hardware block index picks it, so each copy takes the adds of workgroups of one XCD):
+    // one copy would take every workgroup's device-scope add to the same few hundred bins
     __syncthreads();  // every wave's histogram adds are done
+    unsigned long long* dst = gr_copies + (size_t)(blockIdx.x % kGrShellCopies) * (npair * gr_nbin);
     for (int k = tid; k < npair * gr_nbin; k += blockDim.x) {
       const unsigned int v = s_gh[k];
-      if (v) atomicAdd(&gr_count[k], (unsigned long long)v);
+      if (v) atomicAdd(&dst[k], (unsigned long long)v);
     }
   }
 }
@@ -748,7 +753,7 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
                      ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,      \
                      ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, SHORT ? ctx->cap_short : 0,          \
                      ctx->nbr_long, ctx->cnt_long, SHORT ? ctx->nbr_short : nullptr, SHORT ? ctx->cnt_short : nullptr,      \
-                     ctx->xprev, ctx->counters, ctx->gr_rmaxsq, ctx->gr_rdr, ctx->gr_nbin, ctx->npair, ctx->gr_count)
+                     ctx->xprev, ctx->counters, ctx->gr_rmaxsq, ctx->gr_rdr, ctx->gr_nbin, ctx->npair, ctx->gr_shell)
   if (ctx->cap_short) {
     if (gr_tally) NNPE_BUILD(true, true); else NNPE_BUILD(true, false);
   } else {

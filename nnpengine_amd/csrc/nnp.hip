@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_tbl_desc(int n, int cap, const uint32_t
     const int t1 = int(ipw1);
     const float ipw2 = ipw1 - float(t1);
     ipw1 = 1.0f - ipw2;
-    const int p = pair_index(ti, typ[j], ntype);
+    const int p = pair_index(ti, ntype <= NNPE_ENTRY_MAX_TYPES ? NNPE_ENTRY_T(ent) : typ[j], ntype);
     const float* t = tdesc + ((size_t)(p * 4 + NNPE_ENTRY_SP(ent)) * nb1 + t1) * kND;
 #pragma unroll
     for (int k = 0; k < kND; ++k) acc[k] += t[k] * ipw1 + t[kND + k] * ipw2;
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(256) k_tbl_force(int n, int cap, const uint32_
       const int t1 = int(ipw1), t2 = t1 + 1;
       const float ipw2 = ipw1 - float(t1);
       ipw1 = 1.0f - ipw2;
-      const int p = pair_index(ti, typ[j], ntype);
+      const int p = pair_index(ti, ntype <= NNPE_ENTRY_MAX_TYPES ? NNPE_ENTRY_T(ent) : typ[j], ntype);
       const float* fp = tfpair + (size_t)(p * 4 + sp) * nb1;
       float fpair = fp[t1] * ipw1 + fp[t2] * ipw2;
       const float* t = tfdesc + ((size_t)(p * 4 + sp) * nb1 + t1) * kND;
