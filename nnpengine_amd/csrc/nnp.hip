@@ -241,18 +241,6 @@ __device__ __forceinline__ int sf_half_centre(int hg, int t) {
   else return kHRad * hg + t;
 }
 
-// sums over the 8 lanes of each half of a 16-lane group, N values at once (DPP inside a row:
-// half-row mirror, then pair swaps at distance 2 and 1); every lane of the half gets the sum
-template <int N>
-__device__ __forceinline__ void half_group_sum_n(float* v) {
-#pragma unroll
-  for (int k = 0; k < N; ++k) v[k] += dpp_f32<0x141>(v[k]);
-#pragma unroll
-  for (int k = 0; k < N; ++k) v[k] += dpp_f32<0x4E>(v[k]);
-#pragma unroll
-  for (int k = 0; k < N; ++k) v[k] += dpp_f32<0xB1>(v[k]);
-}
-
 // fc = (cos(pi r/rc) + 1)/2 and dfc/dr for 0 <= r <= rc.  With y = pi (r/rc - 1/2), |y| <= pi/2:
 // cos(pi r/rc) = -sin(y), sin(pi r/rc) = cos(y), both from their Taylor series about 0 (sin to
 // y^11, cos to y^12; truncation below 6e-8 on the range), evaluated by Horner's rule.
@@ -285,18 +273,19 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
   constexpr int NMT = NT * kNEta * kNMom + NT * kNRad;
   constexpr int RAD0 = NT * kNEta * kNMom;
   constexpr int D = NT * kNRad + NP * kNEta * 4;
-  __shared__ float s_mom[256 / G][NMT];
+  __shared__ __align__(16) float s_mom[256 / G][NMT];
   __shared__ float4 s_pair[256 / G][kSfBuf];
   __shared__ float4 s_img[27];
   image_table_fill(s_img);
   __syncthreads();
   static_assert(G == 16, "the moment sums split the 16 lanes of an atom into two halves");
-  const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
+  static_assert(NMT % 4 == 0, "moment rows are whole 16-byte chunks");
+  const int blk = xcd_block(gridDim.x);
+  const int i = (blk * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
   const int hg = lane >> 3, hl = lane & 7;  // half of the group and lane within it
   const bool live = i < n;
   float* sm = s_mom[grp];
-  float* mo = Mout + (size_t)(live ? i : 0) * NMT;
   if (live) {
     const float4 xi = xq[i];
     const float bl = boxes[rbox[i]].bl;
@@ -400,26 +389,47 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
         __builtin_amdgcn_wave_barrier();  // buffer consumed before the next scan overwrites it
       }
       if (lane == 0) act_cnt[(size_t)a * stride + i] = na;
-      half_group_sum_n<kHEta * kNMom>(&A[0][0]);
-      half_group_sum_n<kHRad>(R);
-      // value v of this half (angular m*11 + c, then radial) is stored by its lane v mod 8
+      // The half's 30 sums (angular m*11 + c, then the 8 radial centres; 2 zero pads) are
+      // reduce-scattered over its 8 lanes in three exchange steps (half-row mirror, then
+      // distance 2 and 1 in the quad): at each step a lane keeps one half of its block and adds
+      // the partner's copy of it, so lane hl ends with the full sums of values 4 hl .. 4 hl + 3
+      // and writes just those to the LDS moment row (45 DPP adds and 4 stores per lane instead of
+      // a 120-add butterfly and a conditional store per value)
+      constexpr int NV = kHEta * kNMom + kHRad;
+      static_assert(NV <= 32 && G == 16, "a half's sums fit 8 lanes x 4 values");
+      float v[32];
 #pragma unroll
       for (int m = 0; m < kHEta; ++m)
 #pragma unroll
-        for (int c = 0; c < kNMom; ++c)
-          if ((m * kNMom + c) % (G / 2) == hl) {
-            const int idx = (a * kNEta + kHEta * hg + m) * kNMom + c;
-            mo[idx] = A[m][c]; sm[idx] = A[m][c];
-          }
+        for (int c = 0; c < kNMom; ++c) v[m * kNMom + c] = A[m][c];
 #pragma unroll
-      for (int k = 0; k < kHRad; ++k)
-        if ((kHEta * kNMom + k) % (G / 2) == hl) {
-          const int idx = RAD0 + a * kNRad + sf_half_centre<RR>(hg, k);
-          mo[idx] = R[k]; sm[idx] = R[k];
-        }
+      for (int k = 0; k < kHRad; ++k) v[kHEta * kNMom + k] = R[k];
+#pragma unroll
+      for (int k = NV; k < 32; ++k) v[k] = 0.0f;
+      const bool up4 = (hl & 4) != 0, up2 = (hl & 2) != 0, up1 = (hl & 1) != 0;
+      float w[16], x[8], y[4];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) w[t] = (up4 ? v[16 + t] : v[t]) + dpp_f32<0x141>(up4 ? v[t] : v[16 + t]);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) x[t] = (up2 ? w[8 + t] : w[t]) + dpp_f32<0x4E>(up2 ? w[t] : w[8 + t]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) y[t] = (up1 ? x[4 + t] : x[t]) + dpp_f32<0xB1>(up1 ? x[t] : x[4 + t]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int vv = 4 * hl + t;
+        if (vv < kHEta * kNMom) sm[(a * kNEta + kHEta * hg) * kNMom + vv] = y[t];
+        else if (vv < NV) sm[RAD0 + a * kNRad + sf_half_centre<RR>(hg, vv - kHEta * kNMom)] = y[t];
+      }
     }
   }
   __syncthreads();
+  // the workgroup's moment rows, contiguous in LDS and in Mout, copied out whole
+  {
+    const int i0 = blk * (256 / G), nl = min(256 / G, n - i0);
+    const float4* src = reinterpret_cast<const float4*>(&s_mom[0][0]);
+    float4* dst = reinterpret_cast<float4*>(Mout + (size_t)i0 * NMT);
+    for (int q = threadIdx.x; q < nl * (NMT / 4); q += 256) dst[q] = src[q];
+  }
   if (!live) return;
   float* g = G_out + (size_t)i * D;
   for (int k = lane; k < NT * kNRad; k += G) g[k] = sm[RAD0 + k];
