@@ -50,7 +50,8 @@ int nnpe_synchronize(nnpe_ctx* ctx);
  * molecule files (MOLECULE::from_file, src/molecule.cpp:25-131).
  * x, v: [natom][3]; q, mass, typ, mol_id, intra_id: [natom]; mol_type: [nmol];
  * mt_special: concatenated [mt_natom[t]][mt_natom[t]] special classes at mt_special_off[t].
- * v may be NULL (zero velocities). */
+ * v may be NULL (zero velocities).  natom < 8 388 608 (a neighbour-list entry holds a 23-bit
+ * row); larger systems return NNPE_ERR_ARG. */
 int nnpe_system_set(nnpe_ctx* ctx, int natom, int ntype, float bl,
                     const float* x, const float* v, const float* q, const float* mass,
                     const int* typ, const int* mol_id, const int* intra_id,
