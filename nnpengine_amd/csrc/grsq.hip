@@ -194,9 +194,9 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
 // own candidates (the pair stencil); the pairs within r_max whose cells lie outside it are in
 // the outer stencil cells (on the device only those a point of the home cell can reach).
 // Lane per (atom, outer cell): cells whose box lies farther than r_max (plus a margin) from the
-// atom, or that hold no candidate, are dropped; the wave then takes its remaining items one at a
-// time, its 64 lanes walking the item cell's real rows above the atom (weight 2) and its ghost
-// images (weight 1) -- 2 per unordered pair, the reference's rule.  Counts go to
+// atom, or that hold no candidate, are dropped; the wave then takes its remaining items four at a
+// time, 16 lanes per item walking the item cell's real rows above the atom (weight 2) and its
+// ghost images (weight 1) -- 2 per unordered pair, the reference's rule.  Counts go to
 // kGrShellCopies private copies (the few bins near r_max take every add), the same copies the
 // rebuild kernel added its workgroup histograms to, folded into the tally by k_gr_fold_shell.
 // Positions and cells are those of the rebuild that just ran, so the cell boxes bound the atoms.
@@ -236,17 +236,27 @@ __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* _
              (cstart[nc + 1] > max(cstart[nc], i + 1) || gstart[nc + 1] > gstart[nc]);
     }
   }
+  // the wave takes its surviving items four at a time, one per 16-lane quarter: an item is one
+  // cell's rows above the atom and its ghost images, about a dozen candidates, so a whole wave
+  // per item would leave most lanes idle
   unsigned long long bal = __ballot(keep);
+  const int quarter = lane >> 4, ql = lane & 15;
   while (bal) {  // wave-uniform: every lane holds the same mask
-    const int src = __ffsll((long long)bal) - 1;
-    bal &= bal - 1ull;
-    const int ii = __shfl(i, src, 64), cc = __shfl(nc, src, 64);
+    int src = -1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int s = bal ? __ffsll((long long)bal) - 1 : -1;
+      bal &= bal - 1ull;  // (0 stays 0)
+      if (q == quarter) src = s;
+    }
+    const int ii = __shfl(i, max(src, 0), 64), cc = __shfl(nc, max(src, 0), 64);
+    if (src < 0) continue;
     const float4 xi = xq[ii];
     const int ti = typ[ii];
     const BoxPar& B = boxes[rbox[ii]];
     const int j0 = max(cstart[cc], ii + 1), nr = max(0, cstart[cc + 1] - j0);
     const int g0 = gstart[cc], ng = gstart[cc + 1] - g0;
-    for (int k = lane; k < nr + ng; k += 64) {
+    for (int k = ql; k < nr + ng; k += 16) {
       float4 xj;
       int tj;
       unsigned long long wgt;
