@@ -194,8 +194,8 @@ __global__ void __launch_bounds__(256) k_gr_cells(int ncell, const BoxPar* __res
 // own candidates (the pair stencil); the pairs within r_max whose cells lie outside it are in
 // the outer stencil cells (on the device only those a point of the home cell can reach).
 // Lane per (atom, outer cell): cells whose box lies farther than r_max (plus a margin) from the
-// atom, or that hold no candidate, are dropped; the wave then takes its remaining items four at a
-// time, 16 lanes per item walking the item cell's real rows above the atom (weight 2) and its
+// atom, or that hold no candidate, are dropped; the wave then takes its remaining items eight at
+// a time, 8 lanes per item walking the item cell's real rows above the atom (weight 2) and its
 // ghost images (weight 1) -- 2 per unordered pair, the reference's rule.  Counts go to
 // kGrShellCopies private copies (the few bins near r_max take every add), the same copies the
 // rebuild kernel added its workgroup histograms to, folded into the tally by k_gr_fold_shell.
@@ -236,18 +236,18 @@ __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* _
              (cstart[nc + 1] > max(cstart[nc], i + 1) || gstart[nc + 1] > gstart[nc]);
     }
   }
-  // the wave takes its surviving items four at a time, one per 16-lane quarter: an item is one
+  // the wave takes its surviving items eight at a time, one per 8-lane part: an item is one
   // cell's rows above the atom and its ghost images, about a dozen candidates, so a whole wave
-  // per item would leave most lanes idle
+  // per item would leave most lanes idle (16 lanes per item measured 2.5 us slower per rebuild)
   unsigned long long bal = __ballot(keep);
-  const int quarter = lane >> 4, ql = lane & 15;
+  const int part = lane >> 3, ql = lane & 7;
   while (bal) {  // wave-uniform: every lane holds the same mask
     int src = -1;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 8; ++q) {
       const int s = bal ? __ffsll((long long)bal) - 1 : -1;
       bal &= bal - 1ull;  // (0 stays 0)
-      if (q == quarter) src = s;
+      if (q == part) src = s;
     }
     const int ii = __shfl(i, max(src, 0), 64), cc = __shfl(nc, max(src, 0), 64);
     if (src < 0) continue;
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(256) k_gr_shell(int n, int nout, const int4* _
     const BoxPar& B = boxes[rbox[ii]];
     const int j0 = max(cstart[cc], ii + 1), nr = max(0, cstart[cc + 1] - j0);
     const int g0 = gstart[cc], ng = gstart[cc + 1] - g0;
-    for (int k = ql; k < nr + ng; k += 16) {
+    for (int k = ql; k < nr + ng; k += 8) {
       float4 xj;
       int tj;
       unsigned long long wgt;
