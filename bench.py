@@ -311,7 +311,7 @@ def hbm_traffic(stage):
 # hi + lo form ("bf16") is timed beside it
 HEAD = "bf16x1"
 
-STAGE_KERNEL = {"pair": "k_pair<false, 32>", "bonded": "k_bonded<false>", "sf_desc": "k_sf_desc<2, 16, true>",
+STAGE_KERNEL = {"pair": "k_pair<false, 32, true>", "bonded": "k_bonded<false>", "sf_desc": "k_sf_desc<2, 16, true>",
                 "mlp": "k_mlp_w<3, 3>", "sf_adjoint": "k_sf_adjoint<2, true>",
                 "sf_force": "k_sf_force<2, false, 16, true, false, true>",
                 "verlet": "k_update_x64"}
