@@ -189,7 +189,9 @@ struct nnpe_ctx {
 };
 
 // enqueue helpers (no host synchronisation; safe inside stream capture)
-int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally = false);
+// skin_accum: the MD step's skin check ran before this rebuild; its flag is added to the
+// skin-violation count and cleared by the rebuild's reset kernel
+int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally = false, bool skin_accum = false);
 int pair_enqueue(nnpe_ctx* ctx, bool ev);
 int bonded_enqueue(nnpe_ctx* ctx, bool ev);
 int verlet_enqueue_x(nnpe_ctx* ctx);
@@ -224,6 +226,9 @@ void mlp_state_free(nnpe_ctx* ctx);
 int nnp_desc_reserve(nnpe_ctx* ctx, int D);
 int sf_dim(int ntype);
 int type_index_enqueue(nnpe_ctx* ctx);
+// element offsets into type_idx on the device when the network keeps a row index per element
+// (the rebuild then fills type_idx in its copy-back kernel), else nullptr
+const int* mlp_type_off(nnpe_ctx* ctx);
 // NNP pair tables being refined: force / potential [npair][4][nbin+1]; nonzero if not set
 int nnp_tbl_tables(nnpe_ctx* ctx, float** ftab, float** utab, int* nbin, float* dr);
 // symmetry-function active lists sized to the short lists' capacity

@@ -46,11 +46,6 @@ __global__ void k_skin_check(int n, float dx2, const float4* xq, const float4* x
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&cnt->invalid, 1);
 }
 
-__global__ void k_skin_accum(DevCounters* cnt) {
-  if (cnt->invalid) cnt->skin_violations += 1;
-  cnt->invalid = 0;
-}
-
 // the RMDF slot is not cleared: it holds the residual of the last refinement step
 __global__ void k_log_append(double* ev_sum, double* log, int cap, int nval, DevCounters* cnt) {
   const int slot = cnt->log_count;
@@ -120,12 +115,11 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   // a full g(r) tally on a rebuild step rides on the rebuild (same positions, fresh cells)
   const bool gr_fused = (key & kRebuild) && (key & kGr) && (key & kGrFull) && gr_fusable(ctx);
   if (key & kRebuild) {
-    if (!first && P.rebuild_every > 0) {
+    const bool skin = !first && P.rebuild_every > 0;
+    if (skin)
       hipLaunchKernelGGL(k_skin_check, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, s, ctx->natom, ctx->dx2, ctx->xq,
                          ctx->xprev, ctx->counters);
-      hipLaunchKernelGGL(k_skin_accum, dim3(1), dim3(1), 0, s, ctx->counters);
-    }
-    if ((rc = neigh_enqueue_build(ctx, gr_fused))) return rc;
+    if ((rc = neigh_enqueue_build(ctx, gr_fused, skin))) return rc;
   }
   if (key & kGr) {
     if (!gr_fused && (rc = gr_enqueue(ctx, (key & kGrFull) != 0))) return rc;

@@ -723,7 +723,10 @@ static float pow2_scale(double maxabs) {
   return float(std::ldexp(1.0, -int(std::floor(std::log2(maxabs)))));
 }
 
-// rows of each element (rebuilt after every re-sort)
+const int* mlp_type_off(nnpe_ctx* ctx) { return ctx->type_idx && ctx->mlp ? ctx->mlp->d_type_off : nullptr; }
+
+// rows of each element (rebuilt after every re-sort; the list rebuild does it in its copy-back
+// kernel)
 int type_index_enqueue(nnpe_ctx* ctx) {
   if (!ctx->type_idx) return NNPE_OK;
   MlpState* S = ctx->mlp;

@@ -19,17 +19,25 @@ namespace {
 
 constexpr int kStencilNoCheck = 1;
 
-// the rebuild's counters and per-cell counts cleared in one launch, and the g(r) counts of a
-// tally folded into the rebuild (gr_n == 0 otherwise)
+// the rebuild's counters and per-cell counts cleared in one launch, the g(r) counts of a
+// tally folded into the rebuild (gr_n == 0 otherwise), the fill counters of the per-element row
+// index (nt_fill == 0 when there is none), and, after an MD step's skin check, its flag added to
+// the skin-violation count and cleared
 __global__ void k_build_reset(int G, int* cell_cnt, int* cell_fill, int* gcell_cnt, int* gcell_fill, DevCounters* c,
-                              int gr_n, unsigned long long* gr_count, int gr_shell_n, unsigned long long* gr_shell) {
+                              int gr_n, unsigned long long* gr_count, int gr_shell_n, unsigned long long* gr_shell,
+                              int nt_fill, int* type_fill, int skin_accum) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < G) { cell_cnt[i] = 0; cell_fill[i] = 0; gcell_cnt[i] = 0; gcell_fill[i] = 0; }
   if (i < gr_n) gr_count[i] = 0ull;
   if (i < gr_shell_n) gr_shell[i] = 0ull;
+  if (i < nt_fill) type_fill[i] = 0;
   if (i == 0) {
     c->nghost = 0; c->overflow_long = 0; c->overflow_short = 0; c->ghost_overflow = 0;
     c->cell_range = 0; c->max_long = 0; c->max_short = 0;
+    if (skin_accum) {
+      if (c->invalid) c->skin_violations += 1;
+      c->invalid = 0;
+    }
   }
 }
 
@@ -139,15 +147,35 @@ __global__ void k_gather(int n, bool fp64, const int* order, const int* cell_old
   if (fp64) { xd2[i] = xd[o]; vd2[i] = vd[o]; ad2[i] = ad[o]; }
 }
 
+// ... and, when the network keeps a row index per element (type_idx), the sorted rows of each
+// element appended to it, wave-aggregated: one atomic per wave and element (fill counters
+// cleared by the rebuild's reset)
 __global__ void k_copy_back(int n, bool fp64, const float4* xq2, const float4* v2, const float4* fprev2, const float* rm2,
                             const int* typ2, const int4* info2, const double4* xd2, const double4* vd2, const double4* ad2,
                             const int* rbox2, float4* xq, float4* v, float4* fprev, float* rm, int* typ, int4* info,
-                            double4* xd, double4* vd, double4* ad, int* rbox) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  xq[i] = xq2[i]; v[i] = v2[i]; fprev[i] = fprev2[i]; rm[i] = rm2[i]; typ[i] = typ2[i]; info[i] = info2[i];
-  rbox[i] = rbox2[i];
-  if (fp64) { xd[i] = xd2[i]; vd[i] = vd2[i]; ad[i] = ad2[i]; }
+                            double4* xd, double4* vd, double4* ad, int* rbox, int ntype, const int* type_off,
+                            int* type_fill, int* type_idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = i < n ? typ2[i] : -1;
+  if (i < n) {
+    xq[i] = xq2[i]; v[i] = v2[i]; fprev[i] = fprev2[i]; rm[i] = rm2[i]; typ[i] = t; info[i] = info2[i];
+    rbox[i] = rbox2[i];
+    if (fp64) { xd[i] = xd2[i]; vd[i] = vd2[i]; ad[i] = ad2[i]; }
+  }
+  if (!type_idx) return;  // uniform
+  const int lane = threadIdx.x & 63;
+  for (int e = 0; e < ntype; ++e) {
+    const unsigned long long bal = __ballot(t == e);
+    if (!bal) continue;
+    const int first = __ffsll((long long)bal) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(&type_fill[e], __popcll(bal));
+    base = __shfl(base, first, 64);
+    if (t == e) {
+      const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+      type_idx[type_off[e] + base + rank] = i;
+    }
+  }
 }
 
 // LOCAL::update_ghost_and_info x_mol part (src/environ.cpp:461-469): the image of an atom whose
@@ -710,7 +738,7 @@ int box_upload_single(nnpe_ctx* ctx) {
 }
 
 // Enqueue the rebuild kernels (no host synchronisation; graph-capturable).
-int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
+int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum) {
   const int N = ctx->natom, G = ctx->num_grid, bs = 256;
   hipStream_t s = ctx->stream;
   if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
@@ -719,9 +747,10 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
   gr_tally = gr_tally && gr_fusable(ctx);
   const int gr_n = gr_tally ? ctx->npair * ctx->gr_nbin : 0;
   const int gr_shell_n = gr_tally ? gr_shell_size(ctx) : 0;
+  const int* type_off = mlp_type_off(ctx);  // the per-element row index is refilled by k_copy_back
   hipLaunchKernelGGL(k_build_reset, dim3(blocks_for(std::max(G, gr_shell_n), bs)), dim3(bs), 0, s, G, ctx->cell_cnt,
                      ctx->cell_fill, ctx->gcell_cnt, ctx->gcell_fill, ctx->counters, gr_n, ctx->gr_count, gr_shell_n,
-                     ctx->gr_shell);
+                     ctx->gr_shell, type_off ? ctx->ntype : 0, ctx->type_fill, int(skin_accum));
   hipLaunchKernelGGL(k_refold, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->d_box, ctx->rbox, ctx->xq, ctx->xd,
                      ctx->t_cell, ctx->cell_cnt, ctx->counters);
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->cell_cnt, ctx->cell_start, G);
@@ -736,7 +765,8 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
   // fixed pointers)
   hipLaunchKernelGGL(k_copy_back, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->t_xq, ctx->t_v, ctx->t_fprev,
                      ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->t_rbox, ctx->xq, ctx->v,
-                     ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->rbox);
+                     ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->rbox, ctx->ntype, type_off,
+                     ctx->type_fill, type_off ? ctx->type_idx : nullptr);
   NNPE_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_ghost_gen, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->d_box, ctx->rbox, ctx->ghost_cap, ctx->xq,
                      ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_cnt, ctx->counters, ctx->info, ctx->mol_first,
@@ -766,7 +796,7 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally) {
     const int rc = gr_shell_finish_enqueue(ctx);
     if (rc) return rc;
   }
-  return type_index_enqueue(ctx);  // rows of each element moved with the sort
+  return NNPE_OK;
 }
 
 // Check the build's counters; grow capacities and rebuild on list/ghost overflow.
