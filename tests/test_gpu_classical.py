@@ -2,6 +2,7 @@
 S(q)) against the oracle on identical seeded inputs.  Bars: bit-exact list sets and g(r)
 counts; forces/energies within 1e-5 relative RMS of the FP64 oracle (and ~1e-6 of the FP32
 oracle, which differs only in summation order)."""
+import dataclasses
 import os
 
 import numpy as np
@@ -456,3 +457,58 @@ def test_gr_fused_with_rebuild_equals_standalone(water_c1):
     print(f"[gr fused] total counts {int(fused.sum())} vs {int(alone.sum())}", flush=True)
     assert fused.sum() > 0
     np.testing.assert_array_equal(fused, alone)
+
+
+def _many_element_box(base, nt):
+    """The water box with its atoms relabelled into nt elements (oxygens 0/1 by molecule,
+    hydrogens 2 .. nt-1 by molecule and position) and a distinct LJ pair for every element pair.
+    Four elements is the most a list entry names (the pair pass and the inner tally take the
+    neighbour's element from the entry); with five they read it from the element array."""
+    sysm = base.copy()
+    typ = np.where(base.typ == 0, base.mol_id % 2, 2 + (base.mol_id + base.intra_id) % (nt - 2)).astype(np.int32)
+    sysm = dataclasses.replace(sysm, ntype=nt, typ=typ)
+    lj = {}
+    for a in range(nt):
+        for b in range(a, nt):
+            lj[(a, b)] = (0.05 + 0.02 * (a + 2 * b), 2.2 + 0.15 * (a + b))
+    return sysm, lj
+
+
+@pytest.mark.parametrize("nt", [4, 5])
+def test_many_elements_pair_and_gr(nt, water_c0):
+    """Four and five elements: pair forces against the FP32 and FP64 oracles, and the full and
+    inner g(r) tallies (10 and 15 element pairs) bit-exact against the oracle."""
+    sysm, lj = _many_element_box(water_c0, nt)
+    print(f"[elements] ntype={nt} counts={np.bincount(sysm.typ).tolist()}", flush=True)
+    ft, ut = S.lj_table(sysm, RC, 900, lj)
+    e = _engine(sysm)
+    e.build()
+    e.pair_table(RC, ft, ut)
+    e.pair_compute(False)
+    fg = e.get("f")
+    fo = {}
+    for prec in (0, 1):
+        o = _oracle(sysm, prec)
+        o.rebuild(False)
+        o.pair_set(RC, ft, ut)
+        o.clear_f()
+        o.pair_compute()
+        fo[prec] = o.get("f")
+    print(f"[elements] {nt}: pair force rel rms vs fp32 oracle", rel_rms(fg, fo[0]), "vs fp64", rel_rms(fg, fo[1]), flush=True)
+    assert rel_rms(fg, fo[0]) < 2e-6
+    assert rel_rms(fg, fo[1]) < 1e-5
+    nb = e.gr_init(450, RC, RC)
+    o = _oracle(sysm, 0)
+    o.rebuild(False)
+    assert o.gr_init(450, RC, RC) == nb
+    e.gr_tally(full=True)
+    o.gr_tally(True)
+    cg, _ = e.gr_get()
+    co, _ = o.gr_get()
+    assert cg.shape[0] == nt * (nt + 1) // 2 and cg.sum() > 0
+    assert np.array_equal(cg, co)
+    e.gr_tally(full=False)
+    o.gr_tally(False)
+    cg2, _ = e.gr_get()
+    co2, _ = o.gr_get()
+    assert np.array_equal(cg2, co2)
