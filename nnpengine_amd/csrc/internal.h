@@ -190,8 +190,9 @@ struct nnpe_ctx {
 
 // enqueue helpers (no host synchronisation; safe inside stream capture)
 // skin_accum: the MD step's skin check ran before this rebuild; its flag is added to the
-// skin-violation count and cleared by the rebuild's reset kernel
-int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally = false, bool skin_accum = false);
+// skin-violation count and cleared by the rebuild's reset kernel.  defer_shell: a g(r) tally
+// folded into the rebuild leaves its shell pass and fold (gr_shell_finish_enqueue) to the caller
+int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally = false, bool skin_accum = false, bool defer_shell = false);
 int pair_enqueue(nnpe_ctx* ctx, bool ev);
 int bonded_enqueue(nnpe_ctx* ctx, bool ev);
 int verlet_enqueue_x(nnpe_ctx* ctx);

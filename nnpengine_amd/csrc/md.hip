@@ -114,14 +114,33 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   if (!first && (rc = verlet_enqueue_x(ctx))) return rc;
   // a full g(r) tally on a rebuild step rides on the rebuild (same positions, fresh cells)
   const bool gr_fused = (key & kRebuild) && (key & kGr) && (key & kGrFull) && gr_fusable(ctx);
+  // The classical terms (they write f first) and the NNP descriptors, network and adjoints (they
+  // touch no force) are independent: with an NNP term the classical ones go to stream2 and the
+  // NNP force, which adds into f, waits for them.  Captured, the step graph gets two branches.
+  // On a step without a g(r) tally the descriptors run first, alone: the classical kernels
+  // (VALU-bound) then overlap the network (MFMA, latency-bound) and the adjoints instead of the
+  // equally VALU-bound descriptor pass.  On a tally step the tally's own passes (the shell pass
+  // and fold of a tally folded into the rebuild, or the whole tally, then S(q)) read only
+  // positions, cells and lists, so they go to stream2 ahead of the classical terms, beside the
+  // descriptors; with RMDF, whose table rewrite the same step's forces read, they stay in line.
+  const bool fork = P.nnp != 0 && ctx->md_fork;
+  const bool gr_side = fork && (key & kGr) && !P.rmdf;
   if (key & kRebuild) {
     const bool skin = !first && P.rebuild_every > 0;
     if (skin)
       hipLaunchKernelGGL(k_skin_check, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, s, ctx->natom, ctx->dx2, ctx->xq,
                          ctx->xprev, ctx->counters);
-    if ((rc = neigh_enqueue_build(ctx, gr_fused, skin))) return rc;
+    if ((rc = neigh_enqueue_build(ctx, gr_fused, skin, gr_side))) return rc;
   }
-  if (key & kGr) {
+  int rg = NNPE_OK;
+  if (gr_side) {
+    NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+    NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    ctx->stream = ctx->stream2;
+    rg = gr_fused ? gr_shell_finish_enqueue(ctx) : gr_enqueue(ctx, (key & kGrFull) != 0);
+    if (!rg && P.sq) rg = sq_enqueue(ctx);
+    ctx->stream = s;
+  } else if (key & kGr) {
     if (!gr_fused && (rc = gr_enqueue(ctx, (key & kGrFull) != 0))) return rc;
     if (P.sq && (rc = sq_enqueue(ctx))) return rc;
     // RMDF compute_Qspace / compute_Rspace, and compute_Potential on logged steps when
@@ -129,22 +148,20 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
     if (P.rmdf && (rc = rmdf_enqueue(ctx, (key & kLog) && rmdf_gamma(ctx) != 1.0f))) return rc;
   }
   const bool ev = (key & kLog) != 0;
-  // The classical terms (they write f first) and the NNP descriptors, network and adjoints (they
-  // touch no force) are independent: with an NNP term the classical ones go to stream2 and the
-  // NNP force, which adds into f, waits for them.  Captured, the step graph gets two branches.
-  // The descriptors run first, alone: the classical kernels (VALU-bound) then overlap the network
-  // (MFMA, latency-bound) and the adjoints instead of the equally VALU-bound descriptor pass.
-  const bool fork = P.nnp != 0 && ctx->md_fork;
   int rn = NNPE_OK;
   if (P.nnp == 1) rn = nnp_tbl_desc_enqueue(ctx);
   else if (P.nnp == 2) rn = sf_desc_enqueue(ctx);
-  if (rn) return rn;
+  if (rn && !gr_side) return rn;  // (with the branch already open, it is closed below first)
   if (fork) {
-    NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
-    NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    if (!gr_side) {
+      NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+      NNPE_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    }
     ctx->stream = ctx->stream2;
   }
-  if (P.pair) {
+  rc = rg;
+  if (rc) {
+  } else if (P.pair) {
     rc = pair_enqueue(ctx, ev);
   } else {
     hipLaunchKernelGGL(k_clear_force, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, ctx->stream, ctx->natom, ctx->f);
@@ -155,7 +172,8 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
     ctx->stream = s;  // the join is recorded even after an error, so a capture always closes
     NNPE_CHECK_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->stream2));
   }
-  if (P.nnp == 1) {
+  if (rn) {
+  } else if (P.nnp == 1) {
     rn = mlp_enqueue(ctx, ev);
   } else if (P.nnp == 2) {
     rn = mlp_enqueue(ctx, ev);

@@ -738,7 +738,7 @@ int box_upload_single(nnpe_ctx* ctx) {
 }
 
 // Enqueue the rebuild kernels (no host synchronisation; graph-capturable).
-int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum) {
+int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum, bool defer_shell) {
   const int N = ctx->natom, G = ctx->num_grid, bs = 256;
   hipStream_t s = ctx->stream;
   if (ctx->n_stencil_pair > kBuildMaxRanges / 2) return ctx_fail(ctx, NNPE_ERR_UNSUPPORTED, "pair stencil too large");
@@ -792,7 +792,7 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum) {
 #undef NNPE_BUILD
   NNPE_CHECK_LAUNCH(ctx);
   ctx->nbuild++;
-  if (gr_tally) {
+  if (gr_tally && !defer_shell) {
     const int rc = gr_shell_finish_enqueue(ctx);
     if (rc) return rc;
   }

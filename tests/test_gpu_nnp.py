@@ -323,6 +323,36 @@ def test_md_graph_equals_eager(water_c0):
     assert np.array_equal(xs[0][1], xs[1][1])
 
 
+def test_md_gr_on_second_stream_equals_standalone(water_c1):
+    """NNP MD steps whose g(r) tally (folded into the rebuild) and S(q) run on the second stream
+    beside the NNP passes: the counts and h(r) left by the step equal a standalone tally on the
+    same positions, bit for bit (24k atoms, 12 A, BF16 one-term network)."""
+    from nnpengine_amd.engine import Engine
+    sysm = water_c1
+    ft, ut = S.lj_table(sysm, RC_PAIR, 900, S.WATER_LJ)
+    sfp = M.sf_params(RC_NNP)
+    widths = [80, 128, 128, 128, 1]
+    par = M.mlp_init(widths, sysm.ntype, 5) * 0.1
+    e = Engine(sysm)
+    e.neigh_init(RC_PAIR, 2.0, 12.0, RC_NNP)
+    e.pair_table(RC_PAIR, ft, ut)
+    e.bonded_set_from_system()
+    e.sf_set(sfp["rc"], sfp["mu"], sfp["eta_r"], sfp["eta_a"])
+    e.mlp_set(widths, par, 0.5, "bf16x1")
+    e.gr_init(450, 12.0, RC_PAIR)
+    e.sq_init(1000, 0.3, 20.0)
+    # steps 0..20: step 20 rebuilds and tallies; the positions stay as tallied
+    e.md_run(21, dt=0.5, fp64=True, rebuild_every=10, nnp=2, gr_every=10, gr_long_every=10, sq=True)
+    side, h_side = e.gr_get()
+    e.gr_tally(full=True)
+    alone, h_alone = e.gr_get()
+    print(f"[gr side] total counts {int(side.sum())} vs {int(alone.sum())}", flush=True)
+    assert side.sum() > 0
+    np.testing.assert_array_equal(side, alone)
+    np.testing.assert_array_equal(h_side, h_alone)
+    assert np.all(np.isfinite(e.sq_compute()))
+
+
 def test_md_nve_energy_conservation(water_c0):
     """FP64 integration, dt = 0.5 fs, 400 steps: total energy drift small against its fluctuation."""
     from nnpengine_amd.engine import Engine
