@@ -70,6 +70,8 @@ struct MlpDev {
   float bout[kMaxType];
   float out_scale;
   float s_w[kMaxLayers], s_d[kMaxLayers];
+  // their reciprocals (powers of two, so exact): the kernels multiply instead of dividing
+  float r_w[kMaxLayers], r_dw[kMaxLayers];  // 1 / s_w[l], 1 / (s_d[l] s_w[l])
 };
 
 template <int KIND> struct Prec;
@@ -313,7 +315,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
     const int ldd = P.ld[l + 1];
     const T* wt = reinterpret_cast<const T*>(P.Wt[e][l]);
     const float* bias = P.b[e][l];
-    const float un = 1.0f / P.s_w[l];
+    const float un = P.r_w[l];
     const bool have = cb < P.H[l] / 32;
     float es = 0.0f;  // last layer: this lane's part of its row's energy
     if (have) {
@@ -372,7 +374,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
     T* prev = lds + P.slab_off[l];
     const int ldp = P.ld[l];
     const T* w = reinterpret_cast<const T*>(P.W[e][l]);
-    const float un = 1.0f / (P.s_d[l] * P.s_w[l]);
+    const float un = P.r_dw[l];
     const float sd = P.s_d[l - 1];
     if (cb < P.H[l - 1] / 32) {
       f32x16 acc;
@@ -401,7 +403,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_pe
   {
     const T* d1 = lds + P.slab_off[1];
     const T* w = reinterpret_cast<const T*>(P.W[e][0]);
-    const float un = 1.0f / (P.s_d[0] * P.s_w[0]);
+    const float un = P.r_dw[0];
     for (int cb2 = wave; cb2 < Dp / 32; cb2 += kMlpWaves) {
       f32x16 acc;
       if constexpr (KIND == 1)
@@ -567,7 +569,7 @@ k_mlp_w(MlpDev P, const float* __restrict__ G, float* __restrict__ dG, float* __
       load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][NH - 1]), kW, cb, lane);
     else if (cb < (Dp >> 5))
       load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][0]), kW, cb, lane);
-    const float kun = kTanh / P.s_w[l];
+    const float kun = kTanh * P.r_w[l];
     float es = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -622,7 +624,7 @@ k_mlp_w(MlpDev P, const float* __restrict__ G, float* __restrict__ dG, float* __
       load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][l - 1]), kW, cb, lane);
     else if (cb < (Dp >> 5))
       load_b<KIND>(b, reinterpret_cast<const T*>(P.W[e][0]), kW, cb, lane);
-    const float c = 4.0f / (P.s_d[l] * P.s_w[l]) * P.s_d[l - 1];
+    const float c = 4.0f * P.r_dw[l] * P.s_d[l - 1];
     T* prev = act + (l - 1) * 32 * kLdW + row * kLdW + f0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -636,7 +638,7 @@ k_mlp_w(MlpDev P, const float* __restrict__ G, float* __restrict__ dG, float* __
   // dE/dG = (d_1 W_1^T) * scale: column block cb of the padded descriptor width
   if (cb < (Dp >> 5)) {
     const f32x16 acc = gemm_b<KIND>(act, kLdW, b, kW, lane);
-    const float un = 1.0f / (P.s_d[0] * P.s_w[0]);
+    const float un = P.r_dw[0];
     if (row < nrows) {
       float* out = dG + (size_t)srow[row] * D;
       const float* scv = P.scale[e];
@@ -811,6 +813,7 @@ extern "C" int nnpe_mlp_set(nnpe_ctx* ctx, int nlayer, const int* widths, const 
     d.s_d[d.nh - 1] = pow2_scale(bnd);
     for (int l = d.nh - 1; l >= 1; --l) { bnd *= rowsum[l]; d.s_d[l - 1] = pow2_scale(bnd); }
   }
+  for (int l = 0; l < kMaxLayers; ++l) { d.r_w[l] = 1.0f / d.s_w[l]; d.r_dw[l] = 1.0f / (d.s_d[l] * d.s_w[l]); }
   pofs = 0;
   for (int t = 0; t < ctx->ntype; ++t) {
     for (int l = 0; l < nlayer; ++l) {
