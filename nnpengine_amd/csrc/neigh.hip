@@ -320,6 +320,7 @@ template <bool kGr>
 struct BuildRowArgs {
   float4 xi;
   int mol_i, h, tn;
+  uint32_t home_ent;  // the home atom's own (real) candidate entry, the one candidate never taken
   float cutsq, cutsq_s;
   int ntype, stride, cap, cap_s;
   uint32_t* rowl;
@@ -330,73 +331,85 @@ struct BuildRowArgs {
   unsigned int* s_gh;
 };
 
+// dword 3 of a raw (unswizzled, stride 0) buffer resource on CDNA: 32-bit data format
+constexpr int kBufDword3 = 0x00020000;
+
 // One home atom against the staged candidates: 64 candidates per step, ballot compaction into
 // the full row and the per-element short rows.  kBigMol: molecules of more than 64 atoms read
 // the special class from the molecule type's table; smaller ones take it from the lane masks
 // (bit 0 of class in sp0, bit 1 in sp1, indexed by intra id).
+// The rows are written through buffer resources sized to their capacity: the hardware range
+// check drops an entry past the end (the count still grows and reports the overflow), so a
+// store is a shift and the store itself.  Each element's short row has its own resource, and a
+// lane stores under its element's exec mask, which keeps the per-lane row selection out of
+// the step.  The special class is looked up only in steps where a lane sees its own molecule.
 template <bool kShort, bool kGr, bool kBigMol, int NT>
 __device__ __forceinline__ void build_row(const BuildRowArgs<kGr>& A, const float4* c_pos, const uint32_t* c_ent,
                                           const uint32_t* c_aux, const int* sp_row, unsigned long long sp0,
                                           unsigned long long sp1, int lane, int& nl, int (&ns)[kShortMaxType]) {
-  // short rows of the NT elements: wave-uniform base pointers
-  uint32_t* srow[NT];
+  const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc(A.rowl, 0, A.cap * 4, kBufDword3);
+  __amdgpu_buffer_rsrc_t rsh[NT];
 #pragma unroll
-  for (int el = 0; el < NT; ++el) srow[el] = A.nbr_s + ((size_t)el * A.stride + A.h) * A.cap_s;
+  for (int el = 0; el < NT; ++el)
+    rsh[el] = kShort ? __builtin_amdgcn_make_buffer_rsrc(A.nbr_s + ((size_t)el * A.stride + A.h) * A.cap_s, 0,
+                                                         A.cap_s * 4, kBufDword3)
+                     : rl;
   for (int u0 = 0; u0 < A.tn; u0 += 64) {
-    // every lane reads and tests its slot (u < kBuildBatch: u0 <= tn - 1 rounded down to 64);
-    // slots past tn hold stale candidates and are masked, so the step has no branch around
-    // the LDS reads and the compiler issues them together
+    // slots from tn up to the next multiple of 64 hold padding candidates placed far away (never
+    // taken, never tallied), so the step has no bound test and no branch around its LDS reads
     const int u = u0 + lane;
     const uint32_t ce = c_ent[u];
     const float4 xj = c_pos[u];
     const uint32_t ax = c_aux[u];
-    const int j = NNPE_ENTRY_J(ce);
-    const bool ghost = NNPE_ENTRY_CODE(ce) != 13;
-    const bool valid = (u < A.tn) & (ghost | (j != A.h));
+    uint32_t e = ce & 0x7FFFFFFFu;
+    const bool valid = e != A.home_ent;
+    const bool nock = (int)ce < 0;
     const float rsq = dist2_rn(__fsub_rn(A.xi.x, xj.x), __fsub_rn(A.xi.y, xj.y), __fsub_rn(A.xi.z, xj.z));
-    const bool take = valid & (((ce & 0x80000000u) != 0) | (rsq < A.cutsq));
-    const bool take_s = kShort & take & (rsq < A.cutsq_s);
-    const int tj = (int)(ax >> 24);
-    const int jm = (int)(ax & 0xFFFFFFu);
-    const bool same_mol = valid & (__float_as_int(xj.w) == A.mol_i);
-    int sp = 0;
-    if (kBigMol) {
-      if (same_mol) sp = sp_row[jm];
-    } else {
-      // intra ids of the home molecule are < 64; other lanes' shifts are discarded
-      sp = same_mol ? (int)(((sp0 >> (jm & 63)) & 1ull) | (((sp1 >> (jm & 63)) & 1ull) << 1)) : 0;
+    const bool take = valid && (nock || rsq < A.cutsq);
+    const bool same_mol = __float_as_int(xj.w) == A.mol_i;
+    if (__ballot(same_mol)) {
+      const int jm = (int)(ax & 0xFFFFFFu);
+      int sp = 0;
+      if (kBigMol) {
+        if (same_mol) sp = sp_row[jm];
+      } else {
+        // intra ids of the home molecule are < 64; other lanes' shifts are discarded
+        sp = same_mol ? (int)(((sp0 >> (jm & 63)) & 1ull) | (((sp1 >> (jm & 63)) & 1ull) << 1)) : 0;
+      }
+      e |= (uint32_t)sp << 30;
     }
-    const uint32_t e = (ce & 0x7FFFFFFFu) | ((uint32_t)sp << 30);
     // a real pair once, from its lower row, with weight 2; a ghost image from each side
-    if (kGr && valid && (ghost || j > A.h) && rsq < A.gr_rmaxsq) {
-      const int b = gr_bin_of(rsq, A.gr_rdr);
-      if (b < A.gr_nbin) {
-        const int a = min(A.ti_gr, tj), bb = max(A.ti_gr, tj);
-        atomicAdd(&A.s_gh[(a * A.ntype - (a * (a - 1)) / 2 + (bb - a)) * A.gr_nbin + b], ghost ? 1u : 2u);
+    if (kGr) {
+      const int j = NNPE_ENTRY_J(ce);
+      const bool ghost = NNPE_ENTRY_CODE(ce) != 13;
+      if (valid && (ghost || j > A.h) && rsq < A.gr_rmaxsq) {
+        const int b = gr_bin_of(rsq, A.gr_rdr);
+        if (b < A.gr_nbin) {
+          const int tj = (int)(ax >> 24);
+          const int a = min(A.ti_gr, tj), bb = max(A.ti_gr, tj);
+          atomicAdd(&A.s_gh[(a * A.ntype - (a * (a - 1)) / 2 + (bb - a)) * A.gr_nbin + b], ghost ? 1u : 2u);
+        }
       }
     }
     const unsigned long long bal = __ballot(take);
-    const unsigned rank = (unsigned)nl + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-    if (take && rank < (unsigned)A.cap) A.rowl[rank] = e;
+    if (take) {
+      const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, (unsigned)nl));
+      __builtin_amdgcn_raw_buffer_store_b32(e, rl, (int)(rank * 4u), 0, 0);
+    }
     nl += __popcll(bal);
     if (kShort) {
-      // one ballot for the short test, split by the chunk's element masks: each lane ranks
-      // itself within its own element, and one store covers every element's row
-      const unsigned long long bs = __ballot(take_s);
-      unsigned long long em[NT];
+      const bool take_s = take && rsq < A.cutsq_s;
+      const int tj = (int)(ax >> 24);
 #pragma unroll
-      for (int el = 0; el < NT; ++el) em[el] = __ballot(tj == el);
-      unsigned long long mine = em[0];
-      int base = ns[0];
-      uint32_t* row = srow[0];
-#pragma unroll
-      for (int el = 1; el < NT; ++el)
-        if (tj == el) { mine = em[el]; base = ns[el]; row = srow[el]; }
-      const unsigned long long bm = bs & mine;
-      const unsigned rs = (unsigned)base + __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0));
-      if (take_s && rs < (unsigned)A.cap_s) row[rs] = e;
-#pragma unroll
-      for (int el = 0; el < NT; ++el) ns[el] += __popcll(bs & em[el]);
+      for (int el = 0; el < NT; ++el) {
+        const bool mine = take_s && tj == el;
+        const unsigned long long bm = __ballot(mine);
+        if (mine) {
+          const unsigned rs = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, (unsigned)ns[el]));
+          __builtin_amdgcn_raw_buffer_store_b32(e, rsh[el], (int)(rs * 4u), 0, 0);
+        }
+        ns[el] += __popcll(bm);
+      }
     }
   }
 }
@@ -482,8 +495,16 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
     }
     for (int t0 = 0; t0 < total; t0 += kBuildBatch) {
       const int tn = min(kBuildBatch, total - t0);
+      const int tpad = (tn + 63) & ~63;  // <= kBuildBatch, a multiple of 64
       __syncthreads();
-      for (int u = tid; u < tn; u += blockDim.x) {
+      for (int u = tid; u < tpad; u += blockDim.x) {
+        if (u >= tn) {
+          // padding: beyond every cutoff from any home position, no molecule, no flag
+          c_pos[u] = make_float4(1.0e18f, 1.0e18f, 1.0e18f, __int_as_float(-1));
+          c_ent[u] = 0u;
+          c_aux[u] = 0u;
+          continue;
+        }
         const int t = t0 + u;
         int lo = 0, hi = nr2 - 1;  // last range with r_pref[k] <= t
         while (lo < hi) {
@@ -506,7 +527,9 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
       for (int h = h0 + wave; h < h1; h += kBuildThreads / 64) {
         const float4 xi = xq[h];
         const int4 ii = info[h];
-        const int ti_gr = kGr ? typ[h] : 0;
+        const int th = typ[h];
+        const int ti_gr = kGr ? th : 0;
+        const uint32_t hent = NNPE_MAKE_ENTRY_T(h, ntype <= NNPE_ENTRY_MAX_TYPES ? th : 0, 13, 0);
         const int mtn = mt_natom[ii.w];
         const int* sp_row = mt_sp + mt_off[ii.w] + ii.z * mtn;
         int nl = h_nl[h - h0];
@@ -514,7 +537,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
 #pragma unroll
         for (int e = 0; e < kShortMaxType; ++e) ns[e] = h_ns[h - h0][e];
         uint32_t* rowl = nbr + (size_t)h * cap;
-        BuildRowArgs<kGr> A{xi, ii.y, h, tn, cutsq, cutsq_s, ntype, stride, cap, cap_s, rowl, nbr_s, ti_gr,
+        BuildRowArgs<kGr> A{xi, ii.y, h, tn, hent, cutsq, cutsq_s, ntype, stride, cap, cap_s, rowl, nbr_s, ti_gr,
                             gr_rmaxsq, gr_rdr, gr_nbin, s_gh};
         if (mtn <= 64) {
           // the home atom's special classes against its molecule, as two lane masks: no memory
