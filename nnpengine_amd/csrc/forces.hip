@@ -51,8 +51,21 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
 #pragma clang fp contract(fast)
   extern __shared__ float s_tab[];  // [npair][nb1] F/r
   __shared__ float4 s_img[27];
+  // TYE: the table row of each element pair, pair_index(a, b) * nb1, at s_prow[4a + b]; the
+  // special-pair factors (LJ, Coulomb) of each class at s_fac[class]: two LDS reads per entry
+  // in place of the index arithmetic and the selects
+  __shared__ int s_prow[NNPE_ENTRY_MAX_TYPES * NNPE_ENTRY_MAX_TYPES];
+  __shared__ float2 s_fac[4];
   for (int k = threadIdx.x; k < npair * nb1; k += blockDim.x) s_tab[k] = tab[k];
   image_table_fill(s_img);
+  if (TYE && threadIdx.x < NNPE_ENTRY_MAX_TYPES * NNPE_ENTRY_MAX_TYPES) {
+    const int a = threadIdx.x / NNPE_ENTRY_MAX_TYPES, b = threadIdx.x % NNPE_ENTRY_MAX_TYPES;
+    s_prow[threadIdx.x] = a < ntype && b < ntype ? pair_index(a, b, ntype) * nb1 : 0;
+  }
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    s_fac[c] = c == 0 ? make_float2(1.0f, 1.0f) : (c == 3 ? make_float2(kSpecialLj3, kSpecialCoul3) : make_float2(0.0f, 0.0f));
+  }
   __syncthreads();
   const int i = (xcd_block(gridDim.x) * 256 + (int)threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1);
@@ -60,6 +73,7 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
   if (i < n) {
     const float4 xi = xq[i];
     const int ti = typ[i];
+    const int* prow = s_prow + NNPE_ENTRY_MAX_TYPES * (TYE ? ti : 0);
     const BoxPar& B = boxes[rbox[i]];
     const float bl = B.bl;
     float fx = 0, fy = 0, fz = 0;
@@ -95,21 +109,20 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
       for (int u = 0; u < kPairU; ++u) {
         if (k0 + u * G >= m) break;
         const uint32_t ent = ec[u];
-        const int code = NNPE_ENTRY_CODE(ent), sp = NNPE_ENTRY_SP(ent);
+        const int code = NNPE_ENTRY_CODE(ent);
         const float4 xj = image_pos_t(xr[u], code, bl, s_img);
         float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float rsq = dist2_rn(dx, dy, dz);  // the cut as the reference rounds it
         if (!(rsq < cutsq)) continue;
         const float ir = __builtin_amdgcn_rsqf(rsq), r = rsq * ir;
-        const int p = pair_index(ti, tr[u], ntype);
+        const int p = TYE ? 0 : pair_index(ti, tr[u], ntype);
         float ipw1 = r * rdr - 0.5f;
         const int t1 = int(ipw1), t2 = t1 + 1;
         const float ipw2 = ipw1 - float(t1);
         ipw1 = 1.0f - ipw2;
-        const float* ft = s_tab + __mul24(p, nb1);
-        // special-pair factors by select (a per-lane index into constant memory is a load)
-        const float slj = sp == 0 ? 1.0f : (sp == 3 ? kSpecialLj3 : 0.0f);
-        const float scl = sp == 0 ? 1.0f : (sp == 3 ? kSpecialCoul3 : 0.0f);
+        const float* ft = s_tab + (TYE ? prow[tr[u]] : __mul24(p, nb1));
+        const float2 fac = s_fac[NNPE_ENTRY_SP(ent)];
+        const float slj = fac.x, scl = fac.y;
         float fpair = (ft[t1] * ipw1 + ft[t2] * ipw2) * slj;
         const float qq = xi.w * xj.w * kElectricK;
         fpair += qq * (ir * ir * ir + fc * ir) * scl;
@@ -127,7 +140,7 @@ __global__ void __launch_bounds__(256) k_pair(int n, int cap, const uint32_t* __
           const int s1 = int(h1);
           const double h2 = h1 - double(s1);
           h1 = 1.0 - h2;
-          const float* ut = tab + __mul24(npair + p, nb1);
+          const float* ut = tab + size_t(npair) * nb1 + (ft - s_tab);
           const double q2 = double(xi.w) * double(xj.w) * kElectricK64;
           const double u2 = (double(ut[s1]) * h1 + double(ut[s1 + 1]) * h2) * slj +
                             q2 * (1.0 / r2 + P64.vc + P64.fc * (P64.cut - r2)) * scl;
