@@ -40,6 +40,7 @@ extern "C" int nnpe_create(nnpe_ctx** out, int device) {
   NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
   if (const char* e = std::getenv("NNPE_MD_FORK")) ctx->md_fork = std::atoi(e) != 0;
+  if (const char* e = std::getenv("NNPE_MD_FUSE_X")) ctx->md_fuse_x = std::atoi(e) != 0;
   NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
   NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
   NNPE_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->counters), sizeof(DevCounters)));
@@ -58,7 +59,7 @@ extern "C" int nnpe_create(nnpe_ctx** out, int device) {
 }
 
 static void free_all(nnpe_ctx* c) {
-  dev_free(c->xq); dev_free(c->v); dev_free(c->f); dev_free(c->fprev); dev_free(c->xprev);
+  dev_free(c->xq); dev_free(c->xq_alt); dev_free(c->v); dev_free(c->f); dev_free(c->fprev); dev_free(c->xprev);
   dev_free(c->xd); dev_free(c->vd); dev_free(c->ad);
   dev_free(c->rm); dev_free(c->typ); dev_free(c->info); dev_free(c->inv); dev_free(c->cell); dev_free(c->xmol_code);
   dev_free(c->t_xq); dev_free(c->t_v); dev_free(c->t_fprev); dev_free(c->t_xprev);
@@ -159,6 +160,7 @@ extern "C" int nnpe_system_set(nnpe_ctx* ctx, int natom, int ntype, float bl, co
     return hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice);
   };
   NNPE_CHECK_HIP(ctx, up(ctx->xq, hx.data(), N));
+  NNPE_CHECK_HIP(ctx, up(ctx->xq_alt, hx.data(), N));
   NNPE_CHECK_HIP(ctx, up(ctx->v, hv.data(), N));
   NNPE_CHECK_HIP(ctx, up(ctx->f, hz.data(), N));
   NNPE_CHECK_HIP(ctx, up(ctx->fprev, hz.data(), N));

@@ -75,6 +75,10 @@ struct nnpe_ctx {
   float bl = 0, r_bl = 0, h_bl = 0, m_bl = 0;
   bool have_system = false;
   float4 *xq = nullptr, *v = nullptr, *f = nullptr, *fprev = nullptr, *xprev = nullptr;
+  // second position buffer: inside a captured rebuild interval the NNP force pass of one step
+  // writes the next step's positions here while other atoms still read this step's, and the
+  // two pointers trade places from step to step (md.hip)
+  float4* xq_alt = nullptr;
   double4 *xd = nullptr, *vd = nullptr, *ad = nullptr;
   float* rm = nullptr;
   int* typ = nullptr;
@@ -184,6 +188,7 @@ struct nnpe_ctx {
   long long md_step = 0;                // steps completed by nnpe_md_run / the step calls
   void* md = nullptr;                   // captured step graphs (md.hip)
   bool md_fork = true;                  // NNP steps: classical terms on stream2 beside the NNP passes
+  bool md_fuse_x = true;                // captured intervals: position update in the NNP force pass
   double* md_log = nullptr;             // device log records
   int md_log_cap = 0;
 };
@@ -214,7 +219,9 @@ int nnp_tbl_force_enqueue(nnpe_ctx* ctx, bool ev);
 int sf_desc_enqueue(nnpe_ctx* ctx);
 int sf_force_enqueue(nnpe_ctx* ctx, bool ev);
 int sf_adjoint_enqueue(nnpe_ctx* ctx);
-int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick);  // kick: fused FP64 Verlet half-kick
+// kick: fused FP64 Verlet half-kick; adv (with kick): 1 the next step's position update, 2 the
+// positions copied, into ctx->xq_alt
+int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick, int adv = 0);
 int mlp_enqueue(nnpe_ctx* ctx, bool ev);
 void md_state_free(nnpe_ctx* ctx);
 int neigh_check_and_grow(nnpe_ctx* ctx, bool* rebuilt);
@@ -448,7 +455,38 @@ struct VvArgs {
   const float* rm;
   double4* vd;
   double4* ad;
+  // after the kick, the next step's position update of the same atom (adv 1: x += (v + a dt/2) dt
+  // into xd and, as FP32, into xq_out), or (adv 2) this step's positions copied into xq_out
+  double4* xd = nullptr;
+  float4* xq_out = nullptr;
+  int adv = 0;
 };
+
+// vv_kick64, then the next step's position update (the arithmetic of the separate update kernel,
+// x += (v + a dt/2) dt, on the values just written), or the current position copied, into xq_out
+__device__ __forceinline__ float4 vv_kick64_adv(int i, const VvArgs& V, float4 ff, float4 xcur) {
+#pragma clang fp contract(off)
+  const double r = double(V.rm[i]) * 4.184e-4;
+  const double4 ap = V.ad[i];
+  const double4 an = make_double4(double(ff.x) * r, double(ff.y) * r, double(ff.z) * r, 0);
+  double4 vv = V.vd[i];
+  const double hdt = 0.5 * V.dt;
+  vv.x += (ap.x + an.x) * hdt;
+  vv.y += (ap.y + an.y) * hdt;
+  vv.z += (ap.z + an.z) * hdt;
+  V.vd[i] = vv;
+  V.ad[i] = an;
+  if (V.adv == 1) {
+    double4 x = V.xd[i];
+    x.x += (vv.x + an.x * hdt) * V.dt;
+    x.y += (vv.y + an.y * hdt) * V.dt;
+    x.z += (vv.z + an.z * hdt) * V.dt;
+    V.xd[i] = x;
+    xcur.x = float(x.x); xcur.y = float(x.y); xcur.z = float(x.z);
+  }
+  V.xq_out[i] = xcur;
+  return make_float4(float(an.x), float(an.y), float(an.z), 0);
+}
 
 int ev_finish(nnpe_ctx* ctx, int term, int nblocks);  // add partials of a term into ev_sum
 // add per-block partials held elsewhere (the same fixed-order reduction) into ev_sum

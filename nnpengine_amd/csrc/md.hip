@@ -108,10 +108,16 @@ MdGraphs* graphs_of(nnpe_ctx* ctx) {
   return static_cast<MdGraphs*>(ctx->md);
 }
 
-int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
+// fx: the position update fused into the NNP force pass, inside a captured rebuild interval of
+// the FP64 form (0: not fused; 1: the interval's first step, whose update is the separate kernel;
+// 2: a middle step; 3: the last step, whose force pass copies the positions instead of advancing
+// them).  A fused force pass writes the next positions into ctx->xq_alt while other atoms still
+// read this step's from ctx->xq, and the two pointers trade places after the step; over an
+// interval of even length they end where they began, with the current positions in ctx->xq.
+int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first, int fx = 0) {
   int rc;
   hipStream_t s = ctx->stream;
-  if (!first && (rc = verlet_enqueue_x(ctx))) return rc;
+  if (!first && fx <= 1 && (rc = verlet_enqueue_x(ctx))) return rc;
   // a full g(r) tally on a rebuild step rides on the rebuild (same positions, fresh cells)
   const bool gr_fused = (key & kRebuild) && (key & kGr) && (key & kGrFull) && gr_fusable(ctx);
   // The classical terms (they write f first) and the NNP descriptors, network and adjoints (they
@@ -187,7 +193,7 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
   if (P.nnp == 1) {
     if ((rc = nnp_tbl_force_enqueue(ctx, ev))) return rc;
   } else if (P.nnp == 2) {
-    if ((rc = sf_force_only_enqueue(ctx, ev, kick))) return rc;
+    if ((rc = sf_force_only_enqueue(ctx, ev, kick, fx == 0 ? 0 : (fx == 3 ? 2 : 1)))) return rc;
   }
   if (!kick && (rc = verlet_enqueue_v(ctx))) return rc;
   if (ev) {
@@ -196,6 +202,7 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first) {
                        ctx->counters);
   }
   NNPE_CHECK_LAUNCH(ctx);
+  if (fx) std::swap(ctx->xq, ctx->xq_alt);
   return NNPE_OK;
 }
 
@@ -242,16 +249,27 @@ extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, d
   NNPE_CHECK_LAUNCH(ctx);
   MdGraphs* G = graphs_of(ctx);
   int rc;
+  // a rebuild interval of the FP64 NNP form, of even length, fuses each step's position update
+  // into the force pass of the step before it
+  const bool fuse_x = ctx->md_fuse_x && P.nnp == 2 && ctx->fp64 && P.rebuild_every >= 2 && P.rebuild_every % 2 == 0;
   // captures the steps of `keys` (consecutive, starting at the current step) into one graph, or
-  // replays the graph of that sequence
-  auto run_graph = [&](const std::vector<int>& keys) -> int {
+  // replays the graph of that sequence; fused: the interval with fused position updates (its
+  // graph is cached under the keys and a trailing -1)
+  auto run_graph = [&](std::vector<int> keys, bool fused) -> int {
+    const size_t nk = keys.size();
+    if (fused) keys.push_back(-1);
     auto it = G->exec.find(keys);
     if (it == G->exec.end()) {
       hipGraph_t g = nullptr;
+      float4* const x0 = ctx->xq;
+      float4* const x1 = ctx->xq_alt;
       NNPE_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int rcs = NNPE_OK;
-      for (size_t t = 0; t < keys.size() && !rcs; ++t) rcs = enqueue_step(ctx, P, keys[t], false);
+      for (size_t t = 0; t < nk && !rcs; ++t)
+        rcs = enqueue_step(ctx, P, keys[t], false, fused ? (t == 0 ? 1 : (t + 1 == nk ? 3 : 2)) : 0);
       hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+      ctx->xq = x0;
+      ctx->xq_alt = x1;
       if (rcs) return rcs;
       NNPE_CHECK_HIP(ctx, ce);
       hipGraphExec_t ex = nullptr;
@@ -285,12 +303,12 @@ extern "C" int nnpe_md_run(nnpe_ctx* ctx, const nnpe_md_params* p, int nsteps, d
       // a whole rebuild interval as one graph: no launch gaps between its steps
       std::vector<int> keys(P.rebuild_every);
       for (int t = 0; t < P.rebuild_every; ++t) keys[t] = step_key(P, s + t, false);
-      if ((rc = run_graph(keys))) return rc;
+      if ((rc = run_graph(keys, fuse_x))) return rc;
       ctx->md_step += P.rebuild_every;
       k += P.rebuild_every;
       continue;
     } else if (P.use_graph) {
-      if ((rc = run_graph(std::vector<int>{key}))) return rc;
+      if ((rc = run_graph(std::vector<int>{key}, false))) return rc;
     } else {
       if ((rc = enqueue_step(ctx, P, key, false))) return rc;
     }

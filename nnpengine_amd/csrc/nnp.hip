@@ -740,7 +740,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     if (lane == 0) {
       float4 fo = f[i];
       fo.x += fx; fo.y += fy; fo.z += fz;
-      if constexpr (VV) fo = vv_kick64(i, V.dt, V.rm, V.vd, V.ad, fo);
+      if constexpr (VV) fo = V.adv ? vv_kick64_adv(i, V, fo, xi) : vv_kick64(i, V.dt, V.rm, V.vd, V.ad, fo);
       f[i] = fo;
     }
     wacc = vir;
@@ -1020,13 +1020,14 @@ int sf_adjoint_enqueue(nnpe_ctx* ctx) {
 }
 
 // the pair-force kernel alone (uses the adjoints of the last sf_adjoint_enqueue)
-int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick) {
+int sf_force_only_enqueue(nnpe_ctx* ctx, bool ev, bool kick, int adv) {
   NnpState& S = nnp_state(ctx);
   const int nblk = blocks_for((long long)ctx->natom * kSfG, 256);
   double* pe = ev_part(ctx, NNPE_TERM_SF, 0);
   double* pw = ev_part(ctx, NNPE_TERM_SF, 1);
   if (kick && !ctx->fp64) return ctx_fail(ctx, NNPE_ERR_STATE, "fused half-kick needs the FP64 integrator");
-  const VvArgs V{ctx->dt, ctx->rm, ctx->vd, ctx->ad};
+  if (adv && (!kick || !ctx->xq_alt)) return ctx_fail(ctx, NNPE_ERR_STATE, "fused position update needs the fused half-kick");
+  const VvArgs V{ctx->dt, ctx->rm, ctx->vd, ctx->ad, ctx->xd, ctx->xq_alt, adv};
   const bool half = sf_half_gather(ctx);
 #define NNPE_SF_FORCE3(NT, EV, RR, VV, H)                                                                                \
   hipLaunchKernelGGL((k_sf_force<NT, EV, kSfG, RR, VV, H>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->natom, ctx->stride, \

@@ -372,9 +372,11 @@ def test_md_nve_energy_conservation(water_c0):
 
 def test_md_step_equals_call_sequence(water_c0):
     """nnpe_md_run's NNP step -- graph-replayed, classical terms on a second stream, the FP64
-    half-kick fused into the SF force pass -- against the same step spelt out through the C ABI
-    (update_x, build every 10th step, pair, bonded, SF descriptors, network, SF force, update_v):
-    bit-identical positions and velocities after 25 steps with three rebuilds, BF16 network."""
+    half-kick fused into the SF force pass, and in the captured interval of steps 10-19 the next
+    step's position update fused there too (two position buffers trading places) -- against the
+    same step spelt out through the C ABI (update_x, build every 10th step, pair, bonded, SF
+    descriptors, network, SF force, update_v): bit-identical positions and velocities after 25
+    steps with three rebuilds, BF16 network."""
     widths = [80, 64, 64, 1]
     runs = []
     for mode in ("md", "calls"):
