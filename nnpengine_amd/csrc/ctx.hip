@@ -40,6 +40,7 @@ extern "C" int nnpe_create(nnpe_ctx** out, int device) {
   NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   NNPE_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
   if (const char* e = std::getenv("NNPE_MD_FORK")) ctx->md_fork = std::atoi(e) != 0;
+  if (const char* e = std::getenv("NNPE_MD_FORK_EARLY")) ctx->md_fork_early = std::atoi(e) != 0;
   if (const char* e = std::getenv("NNPE_MD_FUSE_X")) ctx->md_fuse_x = std::atoi(e) != 0;
   NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
   NNPE_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));

@@ -188,6 +188,7 @@ struct nnpe_ctx {
   long long md_step = 0;                // steps completed by nnpe_md_run / the step calls
   void* md = nullptr;                   // captured step graphs (md.hip)
   bool md_fork = true;                  // NNP steps: classical terms on stream2 beside the NNP passes
+  bool md_fork_early = true;            // ... branching before the descriptors (else after them)
   bool md_fuse_x = true;                // captured intervals: position update in the NNP force pass
   double* md_log = nullptr;             // device log records
   int md_log_cap = 0;

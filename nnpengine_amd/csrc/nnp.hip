@@ -344,18 +344,15 @@ __global__ void __launch_bounds__(256) k_sf_desc(int n, int stride, int cap, con
           for (int u = 0; u < kSfU; ++u) {
             if (q0 + u * G >= cn) break;  // uniform across the group
             const int q = q0 + u * G + lane;
-            float ux = 0, uy = 0, uz = 0, rsq = 0;
-            bool in = false;
-            if (q < cn) {
-              const float4 xj = image_pos_t(xr[u], NNPE_ENTRY_CODE(ec[u]), bl, s_img);
-              ux = xj.x - xi.x; uy = xj.y - xi.y; uz = xj.z - xi.z;
-              rsq = dist2_rn(ux, uy, uz);  // rounded as written: the force pass makes the same cut
-              in = rsq < P.rcsq;
-            }
+            // lanes past the count hold entry 0 and their own position: computed, never taken
+            const float4 xj = image_pos_t(xr[u], NNPE_ENTRY_CODE(ec[u]), bl, s_img);
+            const float ux = xj.x - xi.x, uy = xj.y - xi.y, uz = xj.z - xi.z;
+            const float rsq = dist2_rn(ux, uy, uz);  // rounded as written: the force pass makes the same cut
+            const bool in = q < cn && rsq < P.rcsq;
             const unsigned long long bal = __ballot(in) & gmask;
             if (in) {
               const int slot = __popcll(bal & below);
-              arow[na + slot] = ec[u];
+              arow[(unsigned)(na + slot)] = ec[u];
               buf[nb + slot] = make_float4(ux, uy, uz, rsq);
             }
             const int c = __popcll(bal);
