@@ -202,6 +202,8 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally = false, bool skin_accum = 
 int pair_enqueue(nnpe_ctx* ctx, bool ev);
 int bonded_enqueue(nnpe_ctx* ctx, bool ev);
 int verlet_enqueue_x(nnpe_ctx* ctx);
+// ... with the MD step's skin check of the updated positions against those of the last build
+int verlet_enqueue_x_skin(nnpe_ctx* ctx);
 int verlet_enqueue_v(nnpe_ctx* ctx);
 int ke_enqueue(nnpe_ctx* ctx);
 int gr_enqueue(nnpe_ctx* ctx, bool full);

@@ -117,7 +117,10 @@ MdGraphs* graphs_of(nnpe_ctx* ctx) {
 int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first, int fx = 0) {
   int rc;
   hipStream_t s = ctx->stream;
-  if (!first && fx <= 1 && (rc = verlet_enqueue_x(ctx))) return rc;
+  // on a rebuild step the position update also makes the skin check (skin/2 since the last build)
+  const bool skin = (key & kRebuild) && !first && P.rebuild_every > 0;
+  const bool upd = !first && fx <= 1;
+  if (upd && (rc = skin ? verlet_enqueue_x_skin(ctx) : verlet_enqueue_x(ctx))) return rc;
   // a full g(r) tally on a rebuild step rides on the rebuild (same positions, fresh cells)
   const bool gr_fused = (key & kRebuild) && (key & kGr) && (key & kGrFull) && gr_fusable(ctx);
   // The classical terms (they write f first) and the NNP descriptors, network and adjoints (they
@@ -134,8 +137,7 @@ int enqueue_step(nnpe_ctx* ctx, const nnpe_md_params& P, int key, bool first, in
   const bool fork = P.nnp != 0 && ctx->md_fork;
   const bool gr_side = fork && (key & kGr) && !P.rmdf;
   if (key & kRebuild) {
-    const bool skin = !first && P.rebuild_every > 0;
-    if (skin)
+    if (skin && !upd)
       hipLaunchKernelGGL(k_skin_check, dim3(blocks_for(ctx->natom, 256)), dim3(256), 0, s, ctx->natom, ctx->dx2, ctx->xq,
                          ctx->xprev, ctx->counters);
     if ((rc = neigh_enqueue_build(ctx, gr_fused, skin, gr_side))) return rc;

@@ -150,12 +150,12 @@ __global__ void k_gather(int n, bool fp64, const int* order, const int* cell_old
 // ... and, when the network keeps a row index per element (type_idx), the sorted rows of each
 // element appended to it, wave-aggregated: one atomic per wave and element (fill counters
 // cleared by the rebuild's reset)
-__global__ void k_copy_back(int n, bool fp64, const float4* xq2, const float4* v2, const float4* fprev2, const float* rm2,
-                            const int* typ2, const int4* info2, const double4* xd2, const double4* vd2, const double4* ad2,
-                            const int* rbox2, float4* xq, float4* v, float4* fprev, float* rm, int* typ, int4* info,
-                            double4* xd, double4* vd, double4* ad, int* rbox, int ntype, const int* type_off,
-                            int* type_fill, int* type_idx) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void copy_back_row(int i, int n, bool fp64, const float4* xq2, const float4* v2,
+                                              const float4* fprev2, const float* rm2, const int* typ2, const int4* info2,
+                                              const double4* xd2, const double4* vd2, const double4* ad2,
+                                              const int* rbox2, float4* xq, float4* v, float4* fprev, float* rm, int* typ,
+                                              int4* info, double4* xd, double4* vd, double4* ad, int* rbox, int ntype,
+                                              const int* type_off, int* type_fill, int* type_idx) {
   const int t = i < n ? typ2[i] : -1;
   if (i < n) {
     xq[i] = xq2[i]; v[i] = v2[i]; fprev[i] = fprev2[i]; rm[i] = rm2[i]; typ[i] = t; info[i] = info2[i];
@@ -207,11 +207,11 @@ __device__ int xmol_code_of(int i, const BoxPar& B, const float4* xq, const int4
 }
 
 // LOCAL::generate_ghost + update_ghost_pos<true> (src/environ.cpp:401-442, :488-514)
-__global__ void k_ghost_gen(int n, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox, int cap,
-                            const float4* xq, int* g_src, int* g_code, int* g_cell, int* gcell_cnt, DevCounters* cnt,
-                            const int4* __restrict__ info, const int* __restrict__ mol_first, const int* __restrict__ inv,
-                            int* __restrict__ xmol_code) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ghost_gen_row(int i, int n, const BoxPar* __restrict__ boxes, const int* __restrict__ rbox,
+                                              int cap, const float4* xq, int* g_src, int* g_code, int* g_cell,
+                                              int* gcell_cnt, DevCounters* cnt, const int4* __restrict__ info,
+                                              const int* __restrict__ mol_first, const int* __restrict__ inv,
+                                              int* __restrict__ xmol_code) {
   const int lane = threadIdx.x & 63;
   BoxPar B{};
   float4 p = make_float4(0, 0, 0, 0);
@@ -259,6 +259,22 @@ __global__ void k_ghost_gen(int n, const BoxPar* __restrict__ boxes, const int* 
         g_src[sl] = i; g_code[sl] = code; g_cell[sl] = gc;
         atomicAdd(&gcell_cnt[gc], 1);
       }
+}
+
+// The gathered rows copied back and the ghost images of each row generated in one pass: the
+// ghost pass reads the gathered copies (the same values), so it need not wait for the copy.
+__global__ void k_copy_back_ghosts(int n, bool fp64, const float4* xq2, const float4* v2, const float4* fprev2,
+                                   const float* rm2, const int* typ2, const int4* info2, const double4* xd2,
+                                   const double4* vd2, const double4* ad2, const int* rbox2, float4* xq, float4* v,
+                                   float4* fprev, float* rm, int* typ, int4* info, double4* xd, double4* vd, double4* ad,
+                                   int* rbox, int ntype, const int* type_off, int* type_fill, int* type_idx,
+                                   const BoxPar* __restrict__ boxes, int cap, int* g_src, int* g_code, int* g_cell,
+                                   int* gcell_cnt, DevCounters* cnt, const int* __restrict__ mol_first,
+                                   const int* __restrict__ inv, int* __restrict__ xmol_code) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  copy_back_row(i, n, fp64, xq2, v2, fprev2, rm2, typ2, info2, xd2, vd2, ad2, rbox2, xq, v, fprev, rm, typ, info, xd, vd, ad,
+                rbox, ntype, type_off, type_fill, type_idx);
+  ghost_gen_row(i, n, boxes, rbox2, cap, xq2, g_src, g_code, g_cell, gcell_cnt, cnt, info2, mol_first, inv, xmol_code);
 }
 
 __global__ void k_ghost_scatter(int cap, const DevCounters* cnt, const int* g_src, const int* g_code, const int* g_cell,
@@ -770,7 +786,7 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum, bool defe
   gr_tally = gr_tally && gr_fusable(ctx);
   const int gr_n = gr_tally ? ctx->npair * ctx->gr_nbin : 0;
   const int gr_shell_n = gr_tally ? gr_shell_size(ctx) : 0;
-  const int* type_off = mlp_type_off(ctx);  // the per-element row index is refilled by k_copy_back
+  const int* type_off = mlp_type_off(ctx);  // the per-element row index is refilled by k_copy_back_ghosts
   hipLaunchKernelGGL(k_build_reset, dim3(blocks_for(std::max(G, gr_shell_n), bs)), dim3(bs), 0, s, G, ctx->cell_cnt,
                      ctx->cell_fill, ctx->gcell_cnt, ctx->gcell_fill, ctx->counters, gr_n, ctx->gr_count, gr_shell_n,
                      ctx->gr_shell, type_off ? ctx->ntype : 0, ctx->type_fill, int(skin_accum));
@@ -785,15 +801,12 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum, bool defe
                      ctx->t_fprev, ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->t_rbox, ctx->cell,
                      ctx->inv);
   // copy the permuted rows back so every array keeps its buffer (captured graphs replay with
-  // fixed pointers)
-  hipLaunchKernelGGL(k_copy_back, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->t_xq, ctx->t_v, ctx->t_fprev,
-                     ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->t_rbox, ctx->xq, ctx->v,
-                     ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->rbox, ctx->ntype, type_off,
-                     ctx->type_fill, type_off ? ctx->type_idx : nullptr);
-  NNPE_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_ghost_gen, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->d_box, ctx->rbox, ctx->ghost_cap, ctx->xq,
-                     ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_cnt, ctx->counters, ctx->info, ctx->mol_first,
-                     ctx->inv, ctx->xmol_code);
+  // fixed pointers), and generate the ghost images from the gathered copies
+  hipLaunchKernelGGL(k_copy_back_ghosts, dim3(blocks_for(N, bs)), dim3(bs), 0, s, N, ctx->fp64, ctx->t_xq, ctx->t_v,
+                     ctx->t_fprev, ctx->t_rm, ctx->t_typ, ctx->t_info, ctx->t_xd, ctx->t_vd, ctx->t_ad, ctx->t_rbox, ctx->xq,
+                     ctx->v, ctx->fprev, ctx->rm, ctx->typ, ctx->info, ctx->xd, ctx->vd, ctx->ad, ctx->rbox, ctx->ntype,
+                     type_off, ctx->type_fill, type_off ? ctx->type_idx : nullptr, ctx->d_box, ctx->ghost_cap, ctx->g_src,
+                     ctx->g_code, ctx->g_cell, ctx->gcell_cnt, ctx->counters, ctx->mol_first, ctx->inv, ctx->xmol_code);
   hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, ctx->gcell_cnt, ctx->gcell_start, G);
   hipLaunchKernelGGL(k_ghost_scatter, dim3(blocks_for(ctx->ghost_cap, bs)), dim3(bs), 0, s, ctx->ghost_cap, ctx->counters,
                      ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_start, ctx->gcell_fill, ctx->gs_src, ctx->gs_code);

@@ -12,19 +12,33 @@ namespace {
 constexpr float kAccelF = 4.184e-4f;          // src/environ.h:78
 constexpr double kEkinF = 1195.0286802753603; // src/environ.h:79
 
+// the MD step's skin check on the updated position (xprev non-null): a wave with a row that
+// moved more than skin/2 since the last build sets the invalid flag
+__device__ __forceinline__ void skin_flag(bool bad, DevCounters* cnt) {
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&cnt->invalid, 1);
+}
+
+__device__ __forceinline__ bool moved_too_far(float4 a, float4 b, float dx2) {
+  const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return dx * dx + dy * dy + dz * dz > dx2;
+}
+
 // skip (optional): rows of boxes flagged there keep their positions (a box's first step)
 __global__ void k_update_x32(int n, float dt, float hdt, float4* xq, const float4* v, const float4* f, float4* fprev,
-                             const int* rbox, const unsigned char* skip) {
+                             const int* rbox, const unsigned char* skip, const float4* xprev, float dx2, DevCounters* cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (skip && skip[rbox[i]]) return;
-  float4 x = xq[i];
-  const float4 vv = v[i], ff = f[i];
-  x.x += (vv.x + ff.x * hdt) * dt;
-  x.y += (vv.y + ff.y * hdt) * dt;
-  x.z += (vv.z + ff.z * hdt) * dt;
-  xq[i] = x;
-  fprev[i] = ff;
+  bool bad = false;
+  if (i < n && !(skip && skip[rbox[i]])) {
+    float4 x = xq[i];
+    const float4 vv = v[i], ff = f[i];
+    x.x += (vv.x + ff.x * hdt) * dt;
+    x.y += (vv.y + ff.y * hdt) * dt;
+    x.z += (vv.z + ff.z * hdt) * dt;
+    xq[i] = x;
+    fprev[i] = ff;
+    if (xprev) bad = moved_too_far(x, xprev[i], dx2);
+  }
+  if (xprev) skin_flag(bad, cnt);
 }
 
 __global__ void k_update_v32(int n, float hdt, const float* rm, float4* v, float4* f, const float4* fprev) {
@@ -42,20 +56,23 @@ __global__ void k_update_v32(int n, float hdt, const float* rm, float4* v, float
 }
 
 __global__ void k_update_x64(int n, double dt, double4* xd, const double4* vd, double4* ad, const float4* f, float4* xq,
-                             const int* rbox, const unsigned char* skip) {
+                             const int* rbox, const unsigned char* skip, const float4* xprev, float dx2, DevCounters* cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (skip && skip[rbox[i]]) return;
-  double4 x = xd[i];
-  const double4 vv = vd[i], a = ad[i];
-  const double hdt = 0.5 * dt;
-  x.x += (vv.x + a.x * hdt) * dt;
-  x.y += (vv.y + a.y * hdt) * dt;
-  x.z += (vv.z + a.z * hdt) * dt;
-  xd[i] = x;
-  float4 q = xq[i];
-  q.x = float(x.x); q.y = float(x.y); q.z = float(x.z);
-  xq[i] = q;
+  bool bad = false;
+  if (i < n && !(skip && skip[rbox[i]])) {
+    double4 x = xd[i];
+    const double4 vv = vd[i], a = ad[i];
+    const double hdt = 0.5 * dt;
+    x.x += (vv.x + a.x * hdt) * dt;
+    x.y += (vv.y + a.y * hdt) * dt;
+    x.z += (vv.z + a.z * hdt) * dt;
+    xd[i] = x;
+    float4 q = xq[i];
+    q.x = float(x.x); q.y = float(x.y); q.z = float(x.z);
+    xq[i] = q;
+    if (xprev) bad = moved_too_far(q, xprev[i], dx2);
+  }
+  if (xprev) skin_flag(bad, cnt);
   (void)f;
 }
 
@@ -142,21 +159,26 @@ extern "C" int nnpe_verlet_set(nnpe_ctx* ctx, double dt, int fp64) {
   return verlet_switch_precision(ctx, fp64 != 0);
 }
 
-int verlet_enqueue_x_skip(nnpe_ctx* ctx, const unsigned char* box_skip) {
+static int update_x_enqueue(nnpe_ctx* ctx, const unsigned char* box_skip, bool skin) {
   const int N = ctx->natom;
+  const float4* xprev = skin ? ctx->xprev : nullptr;
   if (ctx->fp64)
     hipLaunchKernelGGL(k_update_x64, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->dt, ctx->xd, ctx->vd, ctx->ad,
-                       ctx->f, ctx->xq, ctx->rbox, box_skip);
+                       ctx->f, ctx->xq, ctx->rbox, box_skip, xprev, ctx->dx2, ctx->counters);
   else {
     const float dt = float(ctx->dt), hdt = dt * .5f;  // VERLET::setDt (src/verlet.cpp:206-210)
     hipLaunchKernelGGL(k_update_x32, dim3(blocks_for(N, 256)), dim3(256), 0, ctx->stream, N, dt, hdt, ctx->xq, ctx->v, ctx->f,
-                       ctx->fprev, ctx->rbox, box_skip);
+                       ctx->fprev, ctx->rbox, box_skip, xprev, ctx->dx2, ctx->counters);
   }
   NNPE_CHECK_LAUNCH(ctx);
   return NNPE_OK;
 }
 
-int verlet_enqueue_x(nnpe_ctx* ctx) { return verlet_enqueue_x_skip(ctx, nullptr); }
+int verlet_enqueue_x_skip(nnpe_ctx* ctx, const unsigned char* box_skip) { return update_x_enqueue(ctx, box_skip, false); }
+
+int verlet_enqueue_x(nnpe_ctx* ctx) { return update_x_enqueue(ctx, nullptr, false); }
+
+int verlet_enqueue_x_skin(nnpe_ctx* ctx) { return update_x_enqueue(ctx, nullptr, true); }
 
 int verlet_enqueue_v(nnpe_ctx* ctx) {
   const int N = ctx->natom;

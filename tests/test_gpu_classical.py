@@ -320,6 +320,45 @@ def test_md_steps_match_fp32_oracle(water_c0):
     assert rel_rms(e.get("v"), o.get("v")) < 1e-4
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_md_run_counts_skin_violations(fp64, water_c0):
+    """nnpe_md_run with a fixed rebuild interval counts the rebuilds that came too late (an atom
+    moved more than skin/2 since the last build; the check rides on the position update of the
+    rebuild step): the same count as NEIGHLIST::isvalid (src/neighlist.cpp:178-187) asked before
+    each rebuild of the same trajectory spelt out through the C ABI, with bit-identical positions.
+    With a 0.7 A skin the first 10-step interval of this box is short enough and the later ones
+    are not."""
+    ft, ut = S.lj_table(water_c0, RC, 900, S.WATER_LJ)
+    nstep, every = 80, 10
+    out = []
+    for mode in ("md", "calls"):
+        e = _engine(water_c0, skin=0.7)
+        e.pair_table(RC, ft, ut)
+        e.bonded_set_from_system()
+        late = 0
+        if mode == "md":
+            e.md_run(nstep, dt=0.5, fp64=fp64, rebuild_every=every, pair=True, bonded=True, use_graph=True)
+            late = e.md_stats()["skin_violations"]
+        else:
+            e.verlet_set(0.5, fp64=fp64)
+            for step in range(nstep):
+                if step > 0:
+                    e.update_x()
+                if step % every == 0:
+                    if step > 0 and not e.isvalid():
+                        late += 1
+                    e.build()
+                e.pair_compute(False)
+                e.bonded_compute(False)
+                e.update_v()
+        out.append((late, e.get("x")))
+        e.close()
+    print(f"[skin] late rebuilds: md {out[0][0]}, calls {out[1][0]} of {nstep // every - 1}", flush=True)
+    assert out[0][0] == out[1][0]
+    assert 0 < out[1][0] < nstep // every
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def _trajectory(e, o, nstep):
     """The reference MD loop's call sequence (src/run.cpp:230-346) on the engine and the oracle
     side by side; returns the number of rebuilds (equal on both sides)."""
