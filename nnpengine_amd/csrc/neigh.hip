@@ -462,9 +462,14 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
   // kGr: the full g(r) tally of the rebuild's positions rides along -- a real candidate pair
   // within r_max adds 2 from its lower row, a ghost image 1 (2 per unordered pair, the
   // reference's rule), into the workgroup's LDS histogram
-  extern __shared__ unsigned int s_gh[];  // [npair][gr_nbin] when kGr
-  __shared__ int r_begin[kBuildMaxRanges], r_pref[kBuildMaxRanges + 1];
-  __shared__ unsigned char r_flag[kBuildMaxRanges];  // bit 0: ghost, bit 1: no distance check
+  // dynamic LDS, sized at launch so that the g(r) variant still fits four workgroups per CU:
+  // [npair][gr_nbin] counts when kGr, then the 2 nst candidate ranges (begin, prefix + total,
+  // flags: bit 0 ghost, bit 1 no distance check)
+  extern __shared__ unsigned int s_dyn[];
+  unsigned int* const s_gh = s_dyn;
+  int* const r_begin = reinterpret_cast<int*>(s_dyn + (kGr ? npair * gr_nbin : 0));
+  int* const r_pref = r_begin + 2 * nst;
+  unsigned char* const r_flag = reinterpret_cast<unsigned char*>(r_pref + 2 * nst + 1);
   __shared__ float4 c_pos[kBuildBatch];              // x, y, z, molecule id (int bits)
   __shared__ uint32_t c_ent[kBuildBatch];            // entry (special class 0) | nocheck << 31
   __shared__ uint32_t c_aux[kBuildBatch];            // intra id | element << 24
@@ -812,7 +817,9 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum, bool defe
                      ctx->g_src, ctx->g_code, ctx->g_cell, ctx->gcell_start, ctx->gcell_fill, ctx->gs_src, ctx->gs_code);
   hipLaunchKernelGGL(k_ghost_cell_sort, dim3(blocks_for((long long)G * 64, 256)), dim3(256), 0, s, G, ctx->gcell_start,
                      ctx->gs_src, ctx->gs_code, ctx->info);
-  const size_t gh = gr_tally ? sizeof(unsigned int) * ctx->npair * ctx->gr_nbin : 0;
+  // dynamic LDS of the list kernel: the g(r) counts, then the candidate ranges (two per stencil cell)
+  const size_t nr2 = 2 * size_t(ctx->n_stencil_pair);
+  const size_t gh = (gr_tally ? sizeof(unsigned int) * ctx->npair * ctx->gr_nbin : 0) + sizeof(int) * (2 * nr2 + 1) + nr2;
 #define NNPE_BUILD(SHORT, GR)                                                                                             \
   hipLaunchKernelGGL((k_build_cells<SHORT, GR>), dim3(G), dim3(kBuildThreads), gh, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair, \
                      ctx->n_stencil_pair, ctx->cutsq_list, SHORT ? ctx->cutsq_short : 0.0f, ctx->xq, ctx->info,            \
