@@ -67,7 +67,7 @@ static void free_all(nnpe_ctx* c) {
   dev_free(c->t_xd); dev_free(c->t_vd); dev_free(c->t_ad);
   dev_free(c->t_rm); dev_free(c->t_typ); dev_free(c->t_cell); dev_free(c->order); dev_free(c->t_info);
   dev_free(c->mol_first); dev_free(c->mt_natom); dev_free(c->mt_special_off); dev_free(c->mt_special);
-  dev_free(c->rbox); dev_free(c->t_rbox); dev_free(c->d_box); dev_free(c->cell_box);
+  dev_free(c->rbox); dev_free(c->t_rbox); dev_free(c->d_box); dev_free(c->cell_box); dev_free(c->home_cells);
   dev_free(c->cell_start); dev_free(c->cell_cnt); dev_free(c->cell_fill);
   dev_free(c->gcell_start); dev_free(c->gcell_cnt); dev_free(c->gcell_fill);
   dev_free(c->g_src); dev_free(c->g_code); dev_free(c->g_cell); dev_free(c->gs_src); dev_free(c->gs_code);

@@ -53,7 +53,8 @@ struct DevCounters {
 struct BoxPar {
   float bl, r_bl, h_bl, m_bl;           // L, 1/L, L/2, -L
   float g_offset, g_rdx, ghost_lim, pad0;
-  int num_gx, cell_base, pad1, pad2;    // cells per axis, first global cell of the box
+  int num_gx, cell_base;                // cells per axis, first global cell of the box
+  int home_lo, home_hi;                 // cell coordinates a refolded atom can take, per axis
 };
 
 struct NnpState;  // descriptor tables and moment buffers (nnp.hip)
@@ -92,6 +93,8 @@ struct nnpe_ctx {
   int* rbox = nullptr;      // box of each row (permuted with the rows)
   int* t_rbox = nullptr;
   int* cell_box = nullptr;  // box of each cell [num_grid]
+  int* home_cells = nullptr;  // cells that can hold atoms, in cell order [n_home_cells]
+  int n_home_cells = 0;
   // permutation scratch
   float4 *t_xq = nullptr, *t_v = nullptr, *t_fprev = nullptr, *t_xprev = nullptr;
   double4 *t_xd = nullptr, *t_vd = nullptr, *t_ad = nullptr;
@@ -501,6 +504,8 @@ double* ev_part(nnpe_ctx* ctx, int term, int which);  // partial array for term 
 int counters_read(nnpe_ctx* ctx);
 // upload ctx->h_box and a cell -> box map for the single-box layout
 int box_upload_single(nnpe_ctx* ctx);
+// the cells that can hold atoms of the boxes in h_box (their home ranges), the list kernel's grid
+int home_cells_upload(nnpe_ctx* ctx);
 // grid and ghost parameters of one box of edge bl (cell_base 0)
 BoxPar box_geometry(float bl, float pair_cutoff, float skin, float gr_cutoff);
 // MLP tiles from ctx->h_typecount (after the element counts change)

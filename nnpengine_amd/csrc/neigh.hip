@@ -68,7 +68,8 @@ __global__ void k_refold(int n, bool fp64, const BoxPar* __restrict__ boxes, con
   xq[i] = p;
   int cx = cell_coord(B.g_offset, p.x, B.g_rdx), cy = cell_coord(B.g_offset, p.y, B.g_rdx),
       cz = cell_coord(B.g_offset, p.z, B.g_rdx);
-  if (cx < 0 || cy < 0 || cz < 0 || cx >= num_gx || cy >= num_gx || cz >= num_gx) {
+  // outside the box's home cells (which the list kernel covers) is an error like outside the grid
+  if (cx < B.home_lo || cy < B.home_lo || cz < B.home_lo || cx > B.home_hi || cy > B.home_hi || cz > B.home_hi) {
     atomicAdd(&cnt->cell_range, 1);
     cx = min(max(cx, 0), num_gx - 1); cy = min(max(cy, 0), num_gx - 1); cz = min(max(cz, 0), num_gx - 1);
   }
@@ -445,7 +446,8 @@ __device__ __forceinline__ void build_row_nt(const BuildRowArgs<kGr>& A, const f
 }
 
 template <bool kShort, bool kGr>
-__global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const BoxPar* __restrict__ boxes,
+__global__ void __launch_bounds__(kBuildThreads) k_build_cells(int nhome, const int* __restrict__ home_cells,
+                                                     const BoxPar* __restrict__ boxes,
                                                      const int* __restrict__ cell_box, const int4* __restrict__ stencil,
                                                      int nst, float cutsq, float cutsq_s,
                                                      const float4* __restrict__ xq, const int4* __restrict__ info,
@@ -476,8 +478,11 @@ __global__ void __launch_bounds__(kBuildThreads) k_build_cells(int ncell, const 
   __shared__ int h_nl[kBuildHome], h_ns[kBuildHome][kShortMaxType];
   __shared__ int s_wsum[kBuildThreads / 64 + 1];
   __shared__ int s_max[2];  // longest full and short row of this workgroup
-  const int c = xcd_block(gridDim.x);
-  if (c >= ncell) return;
+  // one workgroup per cell that can hold atoms (the home cells of every box), so each XCD's
+  // contiguous share of the grid is real work
+  const int hk = xcd_block(gridDim.x);
+  if (hk >= nhome) return;
+  const int c = home_cells[hk];
   const int hb = cstart[c], he = cstart[c + 1];
   if (hb == he) return;  // uniform across the workgroup
   const int tid = threadIdx.x, lane = tid & 63;
@@ -769,7 +774,24 @@ BoxPar box_geometry(float bl, float pair_cutoff, float skin, float gr_cutoff) {
   B.g_offset = comm + B.h_bl;
   B.num_gx = int(std::ceil((B.g_offset * 2) * B.g_rdx));
   B.cell_base = 0;
+  // a refolded coordinate lies in [-L/2, L/2] (to a few roundings): its cell, by the list
+  // kernels' FP32 operations, lies between those of the slightly widened edges
+  const float edge = B.h_bl * (1.0f + 4e-6f);
+  B.home_lo = std::max(0, int((B.g_offset + (-edge)) * B.g_rdx));
+  B.home_hi = std::min(B.num_gx - 1, int((B.g_offset + edge) * B.g_rdx));
   return B;
+}
+
+int home_cells_upload(nnpe_ctx* ctx) {
+  std::vector<int> hc;
+  for (const BoxPar& b : ctx->h_box)
+    for (int x = b.home_lo; x <= b.home_hi; ++x)
+      for (int y = b.home_lo; y <= b.home_hi; ++y)
+        for (int z = b.home_lo; z <= b.home_hi; ++z) hc.push_back(b.cell_base + (x * b.num_gx + y) * b.num_gx + z);
+  ctx->n_home_cells = int(hc.size());
+  NNPE_CHECK_HIP(ctx, dev_alloc(ctx->home_cells, hc.size()));
+  if (!hc.empty()) NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->home_cells, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice));
+  return NNPE_OK;
 }
 
 // one box: every cell belongs to box 0
@@ -778,6 +800,8 @@ int box_upload_single(nnpe_ctx* ctx) {
   NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->d_box, ctx->h_box.data(), sizeof(BoxPar) * ctx->h_box.size(), hipMemcpyHostToDevice));
   NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_box, ctx->num_grid));
   NNPE_CHECK_HIP(ctx, hipMemset(ctx->cell_box, 0, sizeof(int) * ctx->num_grid));
+  const int rh = home_cells_upload(ctx);
+  if (rh) return rh;
   return NNPE_OK;
 }
 
@@ -821,7 +845,8 @@ int neigh_enqueue_build(nnpe_ctx* ctx, bool gr_tally, bool skin_accum, bool defe
   const size_t nr2 = 2 * size_t(ctx->n_stencil_pair);
   const size_t gh = (gr_tally ? sizeof(unsigned int) * ctx->npair * ctx->gr_nbin : 0) + sizeof(int) * (2 * nr2 + 1) + nr2;
 #define NNPE_BUILD(SHORT, GR)                                                                                             \
-  hipLaunchKernelGGL((k_build_cells<SHORT, GR>), dim3(G), dim3(kBuildThreads), gh, s, G, ctx->d_box, ctx->cell_box, ctx->stencil_pair, \
+  hipLaunchKernelGGL((k_build_cells<SHORT, GR>), dim3(ctx->n_home_cells), dim3(kBuildThreads), gh, s, ctx->n_home_cells,           \
+                     ctx->home_cells, ctx->d_box, ctx->cell_box, ctx->stencil_pair,                                               \
                      ctx->n_stencil_pair, ctx->cutsq_list, SHORT ? ctx->cutsq_short : 0.0f, ctx->xq, ctx->info,            \
                      ctx->cell_start, ctx->gcell_start, ctx->gs_src, ctx->gs_code, ctx->mt_natom, ctx->mt_special_off,      \
                      ctx->mt_special, ctx->typ, ctx->ntype, ctx->stride, ctx->cap_long, SHORT ? ctx->cap_short : 0,          \

@@ -357,6 +357,8 @@ static int serve_relayout(nnpe_serve* s, const std::vector<SReq*>& keep, const s
     NNPE_CHECK_HIP(ctx, dev_alloc(ctx->cell_box, cb.size()));
     NNPE_CHECK_HIP(ctx, hipMemcpy(ctx->cell_box, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
   }
+  const int rh = home_cells_upload(ctx);
+  if (rh) return rh;
   ctx->num_grid = cell;
   const long long gneed = (long long)ghost_est + 4096;
   if (gneed > ctx->ghost_cap) {
