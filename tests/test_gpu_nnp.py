@@ -370,13 +370,15 @@ def test_md_nve_energy_conservation(water_c0):
     assert abs(et[-1] - et[1]) < 0.02 * abs(log[1, 14])
 
 
-def test_md_step_equals_call_sequence(water_c0):
+@pytest.mark.parametrize("every", [10, 5])
+def test_md_step_equals_call_sequence(every, water_c0):
     """nnpe_md_run's NNP step -- graph-replayed, classical terms on a second stream, the FP64
     half-kick fused into the SF force pass, and in the captured interval of steps 10-19 the next
     step's position update fused there too (two position buffers trading places) -- against the
     same step spelt out through the C ABI (update_x, build every 10th step, pair, bonded, SF
     descriptors, network, SF force, update_v): bit-identical positions and velocities after 25
-    steps with three rebuilds, BF16 network."""
+    steps with a rebuild every 10 steps, BF16 network.  With a rebuild every 5 steps (an odd
+    interval) the position update stays a kernel of its own in every step."""
     widths = [80, 64, 64, 1]
     runs = []
     for mode in ("md", "calls"):
@@ -385,13 +387,13 @@ def test_md_step_equals_call_sequence(water_c0):
         e.bonded_set_from_system()
         e.mlp_set(widths, par, 0.5, "bf16", *norm)
         if mode == "md":
-            e.md_run(25, dt=0.5, fp64=True, rebuild_every=10, pair=True, bonded=True, nnp=2, use_graph=True)
+            e.md_run(25, dt=0.5, fp64=True, rebuild_every=every, pair=True, bonded=True, nnp=2, use_graph=True)
         else:
             e.verlet_set(0.5, fp64=True)
             for step in range(25):
                 if step > 0:
                     e.update_x()
-                if step % 10 == 0:
+                if step % every == 0:
                     e.build()
                 e.pair_compute(False)
                 e.bonded_compute(False)
